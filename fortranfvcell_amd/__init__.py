@@ -1,0 +1,252 @@
+"""fortranfvcell_amd -- fortranFVCell's linear-solver path on MI355X (gfx950).
+
+The reference's solver object (src/Solver.f: ``solver`` and ``getRes``, with
+src/Iterativos*.f and src/Matvec*.f underneath) re-built as hand-written HIP
+kernels behind a C ABI (include/fvcell.h, libfvcell.so).  This module is the
+Python side of that ABI: it mirrors the reference's call surface with numpy
+arrays in the reference's layouts, and gives device-resident buffers for
+callers that keep systems in HBM.
+
+No CPU fallback exists: every call goes to the GPU library, and a missing
+library or GPU raises.
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from . import _lib
+from ._lib import (BICGSTAB, CSR, CSRC, CSRD, ELLPACK, ENGINE_AUTO, ENGINE_KERNELS,
+                   ENGINE_PERSISTENT, GMRES, PCG, STOP_REFERENCE, STOP_RELRES, X0_AS_OPENMP,
+                   X0_AS_SEQUENTIAL, X0_GIVEN, X0_ZERO, FvcError, Options, Stats, check,
+                   default_options, load)
+
+__all__ = ["solver", "solve", "solve_dev", "matvec", "matvec_dev", "dot", "dot_dev", "getres",
+           "DeviceArray", "DeviceSystem", "Options", "Stats", "FvcError", "default_options", "load",
+           "PCG", "BICGSTAB", "GMRES", "CSRD", "CSR", "CSRC", "ELLPACK"]
+
+_NONFATAL = (_lib.FVC_OK, _lib.FVC_MAXIT, _lib.FVC_BREAKDOWN, _lib.FVC_NONFINITE)
+
+
+def _ptr(a):
+    if a is None:
+        return None
+    if isinstance(a, DeviceArray):
+        return C.c_void_p(a.ptr)
+    if not a.flags["C_CONTIGUOUS"]:
+        raise ValueError("arrays must be contiguous")
+    return a.ctypes.data_as(C.c_void_p)
+
+
+def _f64(a, name):
+    if a is None:
+        return None
+    if isinstance(a, DeviceArray):
+        return a
+    if a.dtype != np.float64 or not a.flags["C_CONTIGUOUS"]:
+        raise TypeError(f"{name} must be a contiguous float64 array (real*8)")
+    return a
+
+
+def _i32(a, name):
+    if a is None:
+        return None
+    if isinstance(a, DeviceArray):
+        return a
+    if a.dtype != np.int32 or not a.flags["C_CONTIGUOUS"]:
+        raise TypeError(f"{name} must be a contiguous int32 array (integer)")
+    return a
+
+
+def init(device: int = 0) -> None:
+    check(load().fvc_init(device))
+
+
+def device_info() -> dict:
+    cus = C.c_int32(0)
+    name = C.create_string_buffer(256)
+    check(load().fvc_device_info(C.byref(cus), name, 256))
+    return {"compute_units": cus.value, "name": name.value.decode()}
+
+
+# ----------------------------------------------------------- the drop-in --
+
+def solver(ad, au, al, x, b, iax, ja, neq, nad, unsym, tol, maxit, code, matriz, nlit=2,
+           omp_solver=True, raise_on_failure=True) -> dict:
+    """``solver(ad,au,al,x,b,iax,ja,neq,nad,unsym,tol,maxit,code,matriz,nlit)``
+    of src/Solver.f:28-29 on the GPU.  ``x`` (float64, in/out) is updated in
+    place; ``b`` is left intact (the reference overwrites it).
+    ``omp_solver`` is the reference's openmpSolver flag: it selects which
+    paths warm-start from ``x`` (src/Solver.f:94-260).  Returns the stats;
+    non-convergence raises FvcError like the reference stops, unless
+    ``raise_on_failure`` is False."""
+    L = load()
+    st = Stats()
+    au = al if au is None else au
+    rc = L.fvc_solver(_ptr(_f64(ad, "ad")), _ptr(_f64(au, "au")), _ptr(_f64(al, "al")),
+                      _ptr(_f64(x, "x")), _ptr(_f64(b, "b")), _ptr(_i32(iax, "iax")),
+                      _ptr(_i32(ja, "ja")), int(neq), int(nad), int(bool(unsym)), float(tol),
+                      int(maxit), int(code), int(matriz), int(nlit), int(bool(omp_solver)),
+                      C.byref(st))
+    check(rc, (_lib.FVC_OK,) if raise_on_failure else _NONFATAL)
+    return st.as_dict()
+
+
+def _fmt_arrays(fmt: dict):
+    ad = np.ascontiguousarray(fmt["ad"], dtype=np.float64)
+    al = fmt.get("al")
+    al = None if al is None else np.ascontiguousarray(al, dtype=np.float64)
+    au = fmt.get("au")
+    au = al if au is None else np.ascontiguousarray(au, dtype=np.float64)
+    iax = np.ascontiguousarray(fmt["iax"], dtype=np.int32)
+    ja = np.ascontiguousarray(fmt["ja"], dtype=np.int32)
+    return ad, au, al, iax, ja
+
+
+def solve(fmt: dict, code: int, x0=None, b=None, tol: float = 1e-10, maxit: int = 50000,
+          unsym: bool | None = None, options: Options | None = None, raise_on_failure=False, **opt):
+    """Solve a system given in one of the reference layouts (a dict as built by
+    ``problems.to_ell/to_csrd/to_csr/to_csrc``).  Returns (x, stats)."""
+    L = load()
+    ad, au, al, iax, ja = _fmt_arrays(fmt)
+    if unsym is None:
+        unsym = not fmt.get("sym", False)
+    neq = int(fmt["neq"])
+    x = np.zeros(neq) if x0 is None else np.array(x0, dtype=np.float64, copy=True)
+    bb = np.ascontiguousarray(fmt["b"] if b is None else b, dtype=np.float64)
+    o = options if options is not None else default_options(**opt)
+    st = Stats()
+    rc = L.fvc_solve(int(code), int(fmt["matriz"]), int(unsym), neq, int(fmt["nad"]), _ptr(ad),
+                     _ptr(au if au is not None else ad), _ptr(al if al is not None else ad), _ptr(x),
+                     _ptr(bb), _ptr(iax), _ptr(ja), float(tol), int(maxit), C.byref(o), C.byref(st))
+    check(rc, (_lib.FVC_OK,) if raise_on_failure else _NONFATAL)
+    return x, st.as_dict()
+
+
+def matvec(fmt: dict, x, unsym: bool | None = None) -> np.ndarray:
+    """y = A x with the reference's matvec_* for this layout (src/Matvec.f)."""
+    ad, au, al, iax, ja = _fmt_arrays(fmt)
+    if unsym is None:
+        unsym = not fmt.get("sym", False)
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    y = np.empty(int(fmt["neq"]))
+    check(load().fvc_matvec(int(fmt["matriz"]), int(unsym), int(fmt["neq"]), int(fmt["nad"]),
+                            _ptr(ad), _ptr(au if au is not None else ad),
+                            _ptr(al if al is not None else ad), _ptr(iax), _ptr(ja), _ptr(x), _ptr(y)))
+    return y
+
+
+def dot(a, b) -> float:
+    a = np.ascontiguousarray(a, dtype=np.float64)
+    b = np.ascontiguousarray(b, dtype=np.float64)
+    r = C.c_double(0.0)
+    check(load().fvc_dot(_ptr(a), _ptr(b), len(a), C.byref(r)))
+    return r.value
+
+
+def getres(u, x, num, neq, numel, flag) -> None:
+    """getRes (src/Solver.f:368-391) on host arrays, in place."""
+    check(load().fvc_getres(_ptr(_f64(u, "u")), _ptr(_f64(x, "x")), _ptr(_i32(num, "num")),
+                            int(neq), int(numel), int(bool(flag))))
+
+
+# ------------------------------------------------------- device residency --
+
+class DeviceArray:
+    """A buffer in HBM allocated through the library (fvc_malloc)."""
+
+    def __init__(self, nbytes: int, dtype, shape):
+        self.dtype = np.dtype(dtype)
+        self.shape = tuple(shape)
+        self.nbytes = int(nbytes)
+        p = C.c_void_p(0)
+        check(load().fvc_malloc(C.byref(p), max(self.nbytes, 1)))
+        self.ptr = p.value
+
+    @classmethod
+    def from_host(cls, a: np.ndarray) -> "DeviceArray":
+        a = np.ascontiguousarray(a)
+        d = cls(a.nbytes, a.dtype, a.shape)
+        if a.nbytes:
+            check(load().fvc_memcpy_htod(C.c_void_p(d.ptr), a.ctypes.data_as(C.c_void_p), a.nbytes))
+        return d
+
+    @classmethod
+    def empty(cls, shape, dtype=np.float64) -> "DeviceArray":
+        shape = (shape,) if np.isscalar(shape) else tuple(shape)
+        return cls(int(np.prod(shape)) * np.dtype(dtype).itemsize, dtype, shape)
+
+    def upload(self, a: np.ndarray) -> None:
+        a = np.ascontiguousarray(a, dtype=self.dtype)
+        assert a.nbytes == self.nbytes
+        check(load().fvc_memcpy_htod(C.c_void_p(self.ptr), a.ctypes.data_as(C.c_void_p), a.nbytes))
+
+    def get(self) -> np.ndarray:
+        out = np.empty(self.shape, dtype=self.dtype)
+        if self.nbytes:
+            check(load().fvc_memcpy_dtoh(out.ctypes.data_as(C.c_void_p), C.c_void_p(self.ptr), self.nbytes))
+        return out
+
+    def free(self) -> None:
+        if self.ptr:
+            load().fvc_free(C.c_void_p(self.ptr))
+            self.ptr = 0
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+class DeviceSystem:
+    """A system resident in HBM in the reference layout: the arrays the
+    reference's solver receives, uploaded once."""
+
+    def __init__(self, fmt: dict, unsym: bool | None = None):
+        ad, au, al, iax, ja = _fmt_arrays(fmt)
+        self.matriz = int(fmt["matriz"])
+        self.neq = int(fmt["neq"])
+        self.nad = int(fmt["nad"])
+        self.unsym = (not fmt.get("sym", False)) if unsym is None else unsym
+        self.ad = DeviceArray.from_host(ad)
+        self.al = DeviceArray.from_host(al) if al is not None else None
+        self.au = DeviceArray.from_host(au) if (au is not None and au is not al) else self.al
+        self.iax = DeviceArray.from_host(iax)
+        self.ja = DeviceArray.from_host(ja)
+        self.b = DeviceArray.from_host(np.ascontiguousarray(fmt["b"], dtype=np.float64))
+        self.x = DeviceArray.empty(self.neq)
+
+    def solve(self, code: int, tol: float = 1e-10, maxit: int = 50000, x0=None, options=None,
+              stream=None, **opt) -> dict:
+        if x0 is not None:
+            self.x.upload(np.asarray(x0, dtype=np.float64))
+        o = options if options is not None else default_options(**opt)
+        st = Stats()
+        rc = load().fvc_solve_dev(code, self.matriz, int(self.unsym), self.neq, self.nad,
+                                  _ptr(self.ad), _ptr(self.au or self.ad), _ptr(self.al or self.ad),
+                                  _ptr(self.x), _ptr(self.b), _ptr(self.iax), _ptr(self.ja),
+                                  float(tol), int(maxit), C.byref(o), C.byref(st), stream)
+        check(rc, _NONFATAL)
+        return st.as_dict()
+
+    def matvec(self, x: DeviceArray, y: DeviceArray, stream=None) -> None:
+        check(load().fvc_matvec_dev(self.matriz, int(self.unsym), self.neq, self.nad, _ptr(self.ad),
+                                    _ptr(self.au or self.ad), _ptr(self.al or self.ad), _ptr(self.iax),
+                                    _ptr(self.ja), _ptr(x), _ptr(y), stream))
+
+
+def matvec_dev(system: DeviceSystem, x: DeviceArray, y: DeviceArray, stream=None) -> None:
+    system.matvec(x, y, stream)
+
+
+def solve_dev(system: DeviceSystem, code: int, **kw) -> dict:
+    return system.solve(code, **kw)
+
+
+def dot_dev(a: DeviceArray, b: DeviceArray, n: int | None = None, stream=None) -> float:
+    r = C.c_double(0.0)
+    n = a.shape[0] if n is None else n
+    check(load().fvc_dot_dev(_ptr(a), _ptr(b), int(n), C.byref(r), stream))
+    return r.value

@@ -1,0 +1,69 @@
+"""Builds the native parts in-tree:
+
+  libfvcell.so          hipcc, gfx950: kernels + C ABI (include/fvcell.h)
+  libfvcell_fortran.so  amdflang: the ISO_C_BINDING drop-in for the
+                        reference's Solver.o (subroutines solver, getRes)
+
+Run ``python -m fortranfvcell_amd.build``.
+"""
+from __future__ import annotations
+
+import os
+import shutil
+import subprocess
+import sys
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(HERE)
+ARCH = os.environ.get("FVC_OFFLOAD_ARCH", "gfx950")
+
+HIP_FLAGS = [
+    f"--offload-arch={ARCH}", "-O3", "-std=c++17", "-fPIC", "-shared",
+    # reference-order rounding: no FMA contraction of products into sums
+    "-ffp-contract=off",
+    # no packed FP32 instructions
+    "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    "-Wall", "-Wno-unused-function", "-Wno-unused-command-line-argument",
+]
+
+
+def _run(cmd, cwd=None):
+    r = subprocess.run(cmd, cwd=cwd, capture_output=True, text=True)
+    if r.returncode != 0:
+        sys.stderr.write(r.stdout + r.stderr)
+        raise RuntimeError(f"build step failed: {' '.join(cmd)}")
+    return r
+
+
+def build_hip() -> str:
+    out = os.path.join(HERE, "libfvcell.so")
+    tmp = out + ".tmp"
+    _run(["hipcc", *HIP_FLAGS, "-o", tmp, os.path.join(HERE, "csrc", "fvcell.hip")])
+    os.replace(tmp, out)
+    return out
+
+
+def build_fortran() -> str | None:
+    fc = shutil.which("amdflang") or (os.path.exists("/opt/rocm/bin/amdflang") and "/opt/rocm/bin/amdflang")
+    if not fc:
+        return None
+    out = os.path.join(HERE, "libfvcell_fortran.so")
+    objdir = os.path.join(HERE, "fortran", "_obj")
+    os.makedirs(objdir, exist_ok=True)
+    src = os.path.join(HERE, "fortran", "fvc_solver.f90")
+    obj = os.path.join(objdir, "fvc_solver.o")
+    _run([fc, "-O2", "-fPIC", "-module-dir", objdir, "-c", src, "-o", obj])
+    tmp = out + ".tmp"
+    _run([fc, "-shared", "-o", tmp, obj, f"-L{HERE}", "-lfvcell", "-Wl,-rpath,$ORIGIN", "-Wl,--exclude-libs,ALL"])
+    os.replace(tmp, out)
+    return out
+
+
+def build() -> None:
+    build_hip()
+    build_fortran()
+
+
+if __name__ == "__main__":
+    build()
+    print("built", os.listdir(HERE))

@@ -1,0 +1,272 @@
+// fvc_device.h -- device-side building blocks of the gfx950 solver library:
+//   * the matrix view and per-row operators for every reference storage
+//     format, each summing in the reference's sequential order;
+//   * deterministic block and grid reductions (fixed trees, no FP atomics);
+//   * a two-level grid barrier fused with the reduction epilogue, used by
+//     the persistent (one launch per solve) engine, whose waits are bounded.
+//
+// The library is compiled with -ffp-contract=off: every product below is
+// rounded before it is added, exactly like the reference's x86-64 code, so a
+// row result is bit-identical to the reference's.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace fvc {
+
+constexpr int kWave = 64;
+constexpr int kMaxWaves = 16;   // up to 1024 threads per block
+constexpr int kMaxK = 4;        // partial sums per reduction
+
+// Row-operator kinds (device formats).
+enum Fmt : int {
+  F_ELL4 = 0,   // ELLPACK width 4, reference layout a(4,neq) slot-fastest
+  F_ELL3 = 1,   // ELLPACK width 3
+  F_ELLW = 2,   // ELLPACK any other width (the reference skips these)
+  F_CSRD = 3,   // CSR without diagonal + ad
+  F_CSR = 4,    // full CSR, diagonal inside the values, t starts at 0
+  F_LSYM = 5,   // lower triangle + transposed gather (csrd_sym / csrc_sym)
+  F_CSRC = 6,   // CSRC unsymmetric: al row values, au transposed upper
+};
+
+struct Mat {
+  int neq, w;
+  const double *ad;    // diagonal (unused by F_CSR)
+  const double *av;    // off-diagonal values: ELL a(w,neq); CSR-like values
+  const double *au;    // F_CSRC transposed-upper values
+  const int *ia;       // row pointer, 1-based (CSR-like formats)
+  const int *ja;       // column indices, 1-based
+  const int *tptr;     // F_LSYM/F_CSRC: 0-based offsets per row into tsrc/tpos
+  const int *tsrc;     // source row of each transposed entry (0-based, ascending)
+  const int *tpos;     // position (0-based) of that entry in al/au
+};
+
+// (A x)_i with x supplied by a gather functor xf(j), j 0-based.
+template <int F, class X>
+__device__ __forceinline__ double row_apply(const Mat &A, int i, const X &xf) {
+  if constexpr (F == F_ELL4) {
+    // src/Matvec.f:289-296: y = t1+t2+t3+t4+t5, t5 = ad*x(i)
+    const double *a = A.av + 4 * (size_t)i;
+    const int *c = A.ja + 4 * (size_t)i;
+    const double a0 = a[0], a1 = a[1], a2 = a[2], a3 = a[3];
+    const int c0 = c[0], c1 = c[1], c2 = c[2], c3 = c[3];
+    const double t5 = A.ad[i] * xf(i);
+    const double t1 = a0 * xf(c0 - 1);
+    const double t2 = a1 * xf(c1 - 1);
+    const double t3 = a2 * xf(c2 - 1);
+    const double t4 = a3 * xf(c3 - 1);
+    return (((t1 + t2) + t3) + t4) + t5;
+  } else if constexpr (F == F_ELL3) {
+    // src/Matvec.f:281-287: y = t1+t2+t3+t4, t4 = ad*x(i)
+    const double *a = A.av + 3 * (size_t)i;
+    const int *c = A.ja + 3 * (size_t)i;
+    const double a0 = a[0], a1 = a[1], a2 = a[2];
+    const int c0 = c[0], c1 = c[1], c2 = c[2];
+    const double t4 = A.ad[i] * xf(i);
+    const double t1 = a0 * xf(c0 - 1);
+    const double t2 = a1 * xf(c1 - 1);
+    const double t3 = a2 * xf(c2 - 1);
+    return ((t1 + t2) + t3) + t4;
+  } else if constexpr (F == F_ELLW) {
+    // Same pattern for any width: slots left to right, diagonal last.
+    const double *a = A.av + (size_t)A.w * (size_t)i;
+    const int *c = A.ja + (size_t)A.w * (size_t)i;
+    double t = a[0] * xf(c[0] - 1);
+    for (int s = 1; s < A.w; ++s) t = t + a[s] * xf(c[s] - 1);
+    return t + A.ad[i] * xf(i);
+  } else if constexpr (F == F_CSRD) {
+    // src/Matvec.f:28-36
+    double t = A.ad[i] * xf(i);
+    const int k1 = A.ia[i] - 1, k2 = A.ia[i + 1] - 1;
+    for (int k = k1; k < k2; ++k) t = t + A.av[k] * xf(A.ja[k] - 1);
+    return t;
+  } else if constexpr (F == F_CSR) {
+    // src/Matvec.f:68-75
+    double t = 0.0;
+    const int k1 = A.ia[i] - 1, k2 = A.ia[i + 1] - 1;
+    for (int k = k1; k < k2; ++k) t = t + A.av[k] * xf(A.ja[k] - 1);
+    return t;
+  } else {
+    // F_LSYM (src/Matvec.f:157-192,216-250) and F_CSRC (src/Matvec.f:102-135):
+    // the row's own lower products, then the contributions the reference
+    // scatters into y(i) from later rows, in increasing row order.
+    double t = A.ad[i] * xf(i);
+    const int k1 = A.ia[i] - 1, k2 = A.ia[i + 1] - 1;
+    for (int k = k1; k < k2; ++k) t = t + A.av[k] * xf(A.ja[k] - 1);
+    const double *up = (F == F_CSRC) ? A.au : A.av;
+    const int e1 = A.tptr[i], e2 = A.tptr[i + 1];
+    for (int e = e1; e < e2; ++e) t = t + up[A.tpos[e]] * xf(A.tsrc[e]);
+    return t;
+  }
+}
+
+// ------------------------------------------------------------- reductions --
+
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, kWave);
+  return v;   // valid in lane 0
+}
+
+// Sums K values over the block; the totals land in thread 0's v[].
+template <int K>
+__device__ __forceinline__ void block_sum(double *v, double *lds) {
+  const int lane = threadIdx.x & (kWave - 1), wid = threadIdx.x / kWave;
+  const int nw = (blockDim.x + kWave - 1) / kWave;
+#pragma unroll
+  for (int k = 0; k < K; ++k) v[k] = wave_sum(v[k]);
+  if (lane == 0) {
+#pragma unroll
+    for (int k = 0; k < K; ++k) lds[k * kMaxWaves + wid] = v[k];
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+      double s = lds[k * kMaxWaves];
+      for (int w = 1; w < nw; ++w) s += lds[k * kMaxWaves + w];
+      v[k] = s;
+    }
+  }
+  __syncthreads();
+}
+
+// Deterministic sum of G partials per value (partials[k*G + c]).
+template <int K>
+__device__ __forceinline__ void sum_partials(const double *partials, int G, double *v,
+                                             double *lds) {
+#pragma unroll
+  for (int k = 0; k < K; ++k) {
+    double s = 0.0;
+    for (int c = threadIdx.x; c < G; c += blockDim.x) s += partials[(size_t)k * G + c];
+    v[k] = s;
+  }
+  block_sum<K>(v, lds);
+}
+
+// -------------------------------------------------- grid barrier + reduce --
+//
+// ctr layout (unsigned, 128-byte strides): [g*32] arrival counter of group g
+// (g = block % 8, matching the round-robin placement of blocks on the eight
+// XCDs), [8*32] top-level counter, [9*32] generation, [10*32] error flag.
+constexpr int kCtrStride = 32;
+constexpr int kCtrTop = 8 * kCtrStride;
+constexpr int kCtrGen = 9 * kCtrStride;
+constexpr int kCtrErr = 10 * kCtrStride;
+constexpr int kCtrWords = 11 * kCtrStride;
+// Bounded wait: ~2^22 polls with s_sleep in between (well over a second);
+// a barrier that never completes sets the error flag and every block leaves.
+constexpr unsigned kSpinLimit = 1u << 22;
+
+__device__ __forceinline__ unsigned ld_acquire(unsigned *p) {
+  return __hip_atomic_load(p, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ unsigned ld_relaxed(unsigned *p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void st_release(unsigned *p, unsigned v) {
+  __hip_atomic_store(p, v, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void st_relaxed(unsigned *p, unsigned v) {
+  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ unsigned add_acq_rel(unsigned *p, unsigned v) {
+  return __hip_atomic_fetch_add(p, v, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void fence_acquire() { __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent"); }
+__device__ __forceinline__ void fence_release() { __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent"); }
+
+struct Grid {
+  int G;                 // blocks = row chunks
+  int R;                 // rows per chunk
+  int neq;
+  unsigned *ctr;
+  double *partials;      // kMaxK * G
+  bool persistent;
+
+  // XCD-aware chunk: consecutive block ids go to different XCDs, so give each
+  // XCD a contiguous 1/8 of the rows (neighbour gathers stay in its L2).
+  __device__ __forceinline__ int chunk() const {
+    const int b = blockIdx.x;
+    return ((G & 7) == 0) ? (b & 7) * (G >> 3) + (b >> 3) : b;
+  }
+  __device__ __forceinline__ int row_begin() const { return chunk() * R; }
+  __device__ __forceinline__ int row_end() const {
+    const long e = (long)(chunk() + 1) * R;
+    return e < neq ? (int)e : neq;
+  }
+};
+
+__device__ __forceinline__ int group_size(int G, int g) { return (G - g + 7) / 8; }
+
+// Reduce K per-thread values over the whole grid and run `epi(tot)` once, on
+// thread 0 of the last block to arrive.  In persistent mode every block then
+// waits for the epilogue to be published; in kernel-per-phase mode the other
+// blocks return at once and the kernel boundary publishes it.
+// Returns false if a bounded wait expired (the caller must leave).
+template <int K, class Epi>
+__device__ bool grid_reduce(const Grid &g, double *v, double *lds, Epi &&epi) {
+  __shared__ int s_last;
+  __shared__ unsigned s_gen;
+  __shared__ int s_ok;
+  if constexpr (K > 0) block_sum<K>(v, lds);
+  if (threadIdx.x == 0) {
+    const int c = g.chunk();
+#pragma unroll
+    for (int k = 0; k < K; ++k) g.partials[(size_t)k * g.G + c] = v[k];
+    const unsigned gen = g.persistent ? ld_relaxed(g.ctr + kCtrGen) : 0u;
+    fence_release();
+    const int grp = blockIdx.x & 7;
+    const int ngroups = g.G < 8 ? g.G : 8;
+    int last = 0;
+    const unsigned t = add_acq_rel(g.ctr + grp * kCtrStride, 1u);
+    if ((int)t + 1 == group_size(g.G, grp)) {
+      st_relaxed(g.ctr + grp * kCtrStride, 0u);
+      const unsigned t2 = add_acq_rel(g.ctr + kCtrTop, 1u);
+      if ((int)t2 + 1 == ngroups) {
+        st_relaxed(g.ctr + kCtrTop, 0u);
+        last = 1;
+      }
+    }
+    s_last = last;
+    s_gen = gen;
+    s_ok = 1;
+  }
+  __syncthreads();
+  if (s_last) {
+    fence_acquire();
+    double tot[K > 0 ? K : 1];
+    if constexpr (K > 0) {
+      double w[K];
+      sum_partials<K>(g.partials, g.G, w, lds);
+#pragma unroll
+      for (int k = 0; k < K; ++k) tot[k] = w[k];
+    }
+    if (threadIdx.x == 0) epi(tot);
+    __syncthreads();
+    if (threadIdx.x == 0 && g.persistent) {
+      fence_release();
+      st_release(g.ctr + kCtrGen, s_gen + 1u);
+    }
+  } else if (g.persistent) {
+    if (threadIdx.x == 0) {
+      unsigned spins = 0;
+      while (ld_acquire(g.ctr + kCtrGen) == s_gen) {
+        __builtin_amdgcn_s_sleep(2);
+        if (++spins > kSpinLimit || ld_relaxed(g.ctr + kCtrErr) != 0u) {
+          st_relaxed(g.ctr + kCtrErr, 1u);
+          s_ok = 0;
+          break;
+        }
+      }
+    }
+  }
+  if (g.persistent) {
+    __syncthreads();
+    fence_acquire();
+    return s_ok != 0;
+  }
+  return true;
+}
+
+}  // namespace fvc

@@ -1,0 +1,690 @@
+// fvc_krylov.h -- the Krylov solvers of src/Iterativos.f / src/Iterativos_omp.f
+// as sequences of row-parallel phases separated by grid-wide reductions.
+//
+// Each method is one phase function, pcg_phase / bicg_phase / gmres_phase,
+// that runs one phase over this block's chunk of rows and ends with a
+// grid_reduce whose epilogue (run once, by the last block) advances the
+// scalar recurrences held in KState.  Two engines drive the same functions:
+//   persistent : one cooperative launch loops over the phases with grid
+//                barriers (two barriers per PCG iteration);
+//   kernels    : one launch per phase; the next launch sees the epilogue.
+// Both engines use the same chunking and reduction trees, so they produce
+// bit-identical results.
+#pragma once
+#include "fvc_device.h"
+
+namespace fvc {
+
+// Status codes (mirrors include/fvcell.h).
+constexpr int S_OK = 0, S_MAXIT = 1, S_BREAKDOWN = 2, S_NONFINITE = 3;
+
+struct alignas(128) KState {
+  double d, conv, alpha, beta, rr0, omega, energy, last;
+  double bnorm2, res2, econv, rnorm2, pad0, pad1, pad2, pad3;
+  int iter, status, done, inner;      // inner: GMRES Arnoldi steps (nit)
+  int step_i, mgs_j, inner_done, ni;  // GMRES position in the cycle
+  int cycle, pad4, pad5, pad6;
+};
+
+__device__ __forceinline__ int ld_int(const int *p) {
+  return __hip_atomic_load(const_cast<int *>(p), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ double ld_dbl(const double *p) {
+  unsigned long long u = __hip_atomic_load(reinterpret_cast<unsigned long long *>(const_cast<double *>(p)),
+                                           __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  return __builtin_bit_cast(double, u);
+}
+__device__ __forceinline__ bool finite(double v) { return __builtin_isfinite(v); }
+
+// Gather functors: the vector an SpMV multiplies, element j (0-based).
+struct XVec {
+  const double *x;
+  __device__ __forceinline__ double operator()(int j) const { return x[j]; }
+};
+// PCG direction formed on the fly: p_j = z_j + beta * p_old_j
+// (src/Iterativos.f:69-71 evaluated where the SpMV needs it).
+struct XPcgDir {
+  const double *z, *p;
+  double beta;
+  __device__ __forceinline__ double operator()(int j) const { return z[j] + beta * p[j]; }
+};
+// GMRES basis vector normalised on the fly: g_j / nrm (src/Iterativos.f:359-361).
+struct XScaled {
+  const double *g;
+  double nrm;
+  __device__ __forceinline__ double operator()(int j) const { return g[j] / nrm; }
+};
+
+// ------------------------------------------------------------------ PCG --
+
+enum { PCG_SETUP = 0, PCG_A = 1, PCG_B = 2, PCG_FINAL = 3 };
+
+struct PcgArgs {
+  Mat A;
+  Grid g;
+  KState *st;
+  const double *m;   // Jacobi diagonal (m = ad, src/Solver.f:84)
+  double *x;
+  const double *b;
+  double *r, *z, *p0, *p1, *q;
+  double tol;
+  int maxit, x0_zero, stop_rule, phase;
+};
+
+// src/Iterativos.f:29-106 / src/Iterativos_omp.f:30-127.
+template <int F>
+__device__ bool pcg_phase(const PcgArgs &a, int ph, double *lds) {
+  const int i0 = a.g.row_begin(), i1 = a.g.row_end();
+  KState *st = a.st;
+  if (ph == PCG_SETUP) {
+    // z = A x; r = b - z; z = r/m; p = z; d = r.z; conv = tol*sqrt|d|
+    double v[3] = {0.0, 0.0, 0.0};
+    for (int i = i0 + (int)threadIdx.x; i < i1; i += blockDim.x) {
+      double ax;
+      if (a.x0_zero) {
+        a.x[i] = 0.0;
+        ax = 0.0;
+      } else {
+        ax = row_apply<F>(a.A, i, XVec{a.x});
+      }
+      const double bi = a.b[i];
+      const double ri = bi - ax;
+      const double zi = ri / a.m[i];
+      a.r[i] = ri;
+      a.z[i] = zi;
+      v[0] += ri * zi;
+      v[1] += bi * bi;
+      v[2] += ri * ri;
+    }
+    return grid_reduce<3>(a.g, v, lds, [&](const double *t) {
+      st->d = t[0];
+      st->bnorm2 = t[1];
+      st->rnorm2 = t[2];
+      st->iter = 0;
+      st->inner = 0;
+      st->status = S_MAXIT;
+      st->done = 0;
+      st->beta = 0.0;
+      if (a.stop_rule == 1) {
+        st->conv = a.tol * sqrt(t[1]);
+        st->last = sqrt(t[2]);
+        if (st->last <= st->conv) { st->status = S_OK; st->done = 1; }
+      } else {
+        st->conv = a.tol * sqrt(fabs(t[0]));
+        st->last = sqrt(fabs(t[0]));
+      }
+      if (!finite(t[0])) { st->status = S_NONFINITE; st->done = 1; }
+      if (a.maxit <= 0 && !st->done) st->done = 1;
+    });
+  }
+  if (ph == PCG_A) {
+    // q = A p with p = z + beta*p formed on the fly; alpha = d / p.q
+    const int j = ld_int(&st->iter) + 1;
+    double *pn = (j & 1) ? a.p1 : a.p0;
+    const double *po = (j & 1) ? a.p0 : a.p1;
+    double v[1] = {0.0};
+    if (j == 1) {
+      for (int i = i0 + (int)threadIdx.x; i < i1; i += blockDim.x) {
+        const double qi = row_apply<F>(a.A, i, XVec{a.z});
+        const double pi = a.z[i];
+        pn[i] = pi;
+        a.q[i] = qi;
+        v[0] += pi * qi;
+      }
+    } else {
+      const XPcgDir xf{a.z, po, ld_dbl(&st->beta)};
+      for (int i = i0 + (int)threadIdx.x; i < i1; i += blockDim.x) {
+        const double qi = row_apply<F>(a.A, i, xf);
+        const double pi = xf(i);
+        pn[i] = pi;
+        a.q[i] = qi;
+        v[0] += pi * qi;
+      }
+    }
+    return grid_reduce<1>(a.g, v, lds, [&](const double *t) {
+      const double alpha = st->d / t[0];
+      st->alpha = alpha;
+      if (t[0] == 0.0) { st->status = S_BREAKDOWN; st->done = 1; st->iter = j; }
+      else if (!finite(alpha)) { st->status = S_NONFINITE; st->done = 1; st->iter = j; }
+    });
+  }
+  if (ph == PCG_B) {
+    // x += alpha p; r -= alpha q; z = r/m; beta = r.z/d; d = beta*d
+    const int j = ld_int(&st->iter) + 1;
+    const double *pn = (j & 1) ? a.p1 : a.p0;
+    const double alpha = ld_dbl(&st->alpha);
+    double v[2] = {0.0, 0.0};
+    for (int i = i0 + (int)threadIdx.x; i < i1; i += blockDim.x) {
+      const double xi = a.x[i] + alpha * pn[i];
+      const double ri = a.r[i] - alpha * a.q[i];
+      const double zi = ri / a.m[i];
+      a.x[i] = xi;
+      a.r[i] = ri;
+      a.z[i] = zi;
+      v[0] += ri * zi;
+      v[1] += ri * ri;
+    }
+    return grid_reduce<2>(a.g, v, lds, [&](const double *t) {
+      const double beta = t[0] / st->d;
+      const double d = beta * st->d;
+      st->beta = beta;
+      st->d = d;
+      st->rnorm2 = t[1];
+      st->iter = j;
+      bool conv;
+      if (a.stop_rule == 1) {
+        st->last = sqrt(t[1]);
+        conv = st->last <= st->conv;
+      } else {
+        st->last = sqrt(fabs(d));
+        conv = st->last < st->conv;   // src/Iterativos.f:73
+      }
+      if (conv) { st->status = S_OK; st->done = 1; }
+      else if (!finite(d) || !finite(beta)) { st->status = S_NONFINITE; st->done = 1; }
+      else if (j >= a.maxit) { st->status = S_MAXIT; st->done = 1; }
+    });
+  }
+  // PCG_FINAL: energy norm x.Ax (src/Iterativos.f:82-83) and ||b - Ax||^2
+  double v[2] = {0.0, 0.0};
+  for (int i = i0 + (int)threadIdx.x; i < i1; i += blockDim.x) {
+    const double ax = row_apply<F>(a.A, i, XVec{a.x});
+    const double xi = a.x[i];
+    const double ri = a.b[i] - ax;
+    v[0] += xi * ax;
+    v[1] += ri * ri;
+  }
+  return grid_reduce<2>(a.g, v, lds, [&](const double *t) {
+    st->energy = t[0];
+    st->res2 = t[1];
+  });
+}
+
+template <int F>
+__global__ __launch_bounds__(1024) void pcg_persistent(PcgArgs a) {
+  __shared__ double lds[kMaxK * kMaxWaves];
+  if (!pcg_phase<F>(a, PCG_SETUP, lds)) return;
+  while (!ld_int(&a.st->done)) {
+    if (!pcg_phase<F>(a, PCG_A, lds)) return;
+    if (ld_int(&a.st->done)) break;
+    if (!pcg_phase<F>(a, PCG_B, lds)) return;
+  }
+  pcg_phase<F>(a, PCG_FINAL, lds);
+}
+
+template <int F>
+__global__ __launch_bounds__(1024) void pcg_kernel(PcgArgs a) {
+  __shared__ double lds[kMaxK * kMaxWaves];
+  if ((a.phase == PCG_A || a.phase == PCG_B) && ld_int(&a.st->done)) return;
+  pcg_phase<F>(a, a.phase, lds);
+}
+
+// ------------------------------------------------------------- BiCGSTAB --
+
+enum { BI_SETUP = 0, BI_P1 = 1, BI_P2 = 2, BI_P3 = 3, BI_P4 = 4, BI_P5 = 5, BI_FINAL = 6 };
+
+struct BicgArgs {
+  Mat A;
+  Grid g;
+  KState *st;
+  const double *m;
+  double *x;
+  const double *b;
+  double *bb;   // running residual (the reference's overwritten b)
+  double *r, *p, *z, *v, *t;
+  double tol;
+  int maxit, x0_zero, stop_rule, phase;
+};
+
+// src/Iterativos.f:140-234 / src/Iterativos_omp.f:161-274.
+template <int F>
+__device__ bool bicg_phase(const BicgArgs &a, int ph, double *lds) {
+  const int i0 = a.g.row_begin(), i1 = a.g.row_end();
+  KState *st = a.st;
+  if (ph == BI_SETUP) {
+    // z = Ax; r = b - z; p = r; b = p; z = p/m; d = r.z
+    double v[3] = {0.0, 0.0, 0.0};
+    for (int i = i0 + (int)threadIdx.x; i < i1; i += blockDim.x) {
+      double ax;
+      if (a.x0_zero) {
+        a.x[i] = 0.0;
+        ax = 0.0;
+      } else {
+        ax = row_apply<F>(a.A, i, XVec{a.x});
+      }
+      const double bi = a.b[i];
+      const double ri = bi - ax;
+      const double zi = ri / a.m[i];
+      a.r[i] = ri;
+      a.p[i] = ri;
+      a.bb[i] = ri;
+      a.z[i] = zi;
+      v[0] += ri * zi;
+      v[1] += ri * ri;   // = b.r of the first iteration (b = p = r)
+      v[2] += bi * bi;
+    }
+    return grid_reduce<3>(a.g, v, lds, [&](const double *t) {
+      st->d = t[0];
+      st->rr0 = t[1];
+      st->rnorm2 = t[1];
+      st->bnorm2 = t[2];
+      st->iter = 0;
+      st->inner = 0;
+      st->status = S_MAXIT;
+      st->done = 0;
+      if (a.stop_rule == 1) {
+        st->conv = a.tol * sqrt(t[2]);
+        st->last = sqrt(t[1]);
+        if (st->last <= st->conv) { st->status = S_OK; st->done = 1; }
+      } else {
+        st->conv = a.tol * sqrt(fabs(t[0]));
+        st->last = sqrt(fabs(t[0]));
+      }
+      if (!finite(t[0])) { st->status = S_NONFINITE; st->done = 1; }
+      if (a.maxit <= 0 && !st->done) st->done = 1;
+    });
+  }
+  if (ph == BI_P1) {
+    // v = A z; alpha = rr0 / v.r
+    double v[1] = {0.0};
+    for (int i = i0 + (int)threadIdx.x; i < i1; i += blockDim.x) {
+      const double vi = row_apply<F>(a.A, i, XVec{a.z});
+      a.v[i] = vi;
+      v[0] += vi * a.r[i];
+    }
+    return grid_reduce<1>(a.g, v, lds, [&](const double *t) {
+      const double alpha = st->rr0 / t[0];
+      st->alpha = alpha;
+      if (t[0] == 0.0) { st->status = S_BREAKDOWN; st->done = 1; st->iter = st->iter + 1; }
+      else if (!finite(alpha)) { st->status = S_NONFINITE; st->done = 1; st->iter = st->iter + 1; }
+    });
+  }
+  if (ph == BI_P2) {
+    // x += alpha z; b -= alpha v; z = b/m
+    const double alpha = ld_dbl(&st->alpha);
+    for (int i = i0 + (int)threadIdx.x; i < i1; i += blockDim.x) {
+      const double zi = a.z[i];
+      a.x[i] = a.x[i] + alpha * zi;
+      const double bi = a.bb[i] - alpha * a.v[i];
+      a.bb[i] = bi;
+      a.z[i] = bi / a.m[i];
+    }
+    double v[1] = {0.0};
+    return grid_reduce<0>(a.g, v, lds, [&](const double *) {});
+  }
+  if (ph == BI_P3) {
+    // t = A z; w = t.b / t.t
+    double v[2] = {0.0, 0.0};
+    for (int i = i0 + (int)threadIdx.x; i < i1; i += blockDim.x) {
+      const double ti = row_apply<F>(a.A, i, XVec{a.z});
+      a.t[i] = ti;
+      v[0] += ti * a.bb[i];
+      v[1] += ti * ti;
+    }
+    return grid_reduce<2>(a.g, v, lds, [&](const double *t) {
+      const double w = t[0] / t[1];
+      st->omega = w;
+      if (t[1] == 0.0 || w == 0.0) { st->status = S_BREAKDOWN; st->done = 1; st->iter = st->iter + 1; }
+      else if (!finite(w)) { st->status = S_NONFINITE; st->done = 1; st->iter = st->iter + 1; }
+    });
+  }
+  if (ph == BI_P4) {
+    // x += w z; b -= w t; beta = b.z / d (stop test); beta = (r.b/rr0)(alpha/w)
+    const double w = ld_dbl(&st->omega);
+    double v[3] = {0.0, 0.0, 0.0};
+    for (int i = i0 + (int)threadIdx.x; i < i1; i += blockDim.x) {
+      const double zi = a.z[i];
+      a.x[i] = a.x[i] + w * zi;
+      const double bi = a.bb[i] - w * a.t[i];
+      a.bb[i] = bi;
+      v[0] += bi * zi;
+      v[1] += a.r[i] * bi;
+      v[2] += bi * bi;
+    }
+    return grid_reduce<3>(a.g, v, lds, [&](const double *t) {
+      const int j = st->iter + 1;
+      st->iter = j;
+      st->rnorm2 = t[2];
+      const double beta1 = t[0] / st->d;
+      bool conv;
+      if (a.stop_rule == 1) {
+        st->last = sqrt(t[2]);
+        conv = st->last <= st->conv;
+      } else {
+        st->last = sqrt(fabs(beta1));
+        conv = st->last < st->conv;   // src/Iterativos.f:189-190
+      }
+      const double beta = (t[1] / st->rr0) * (st->alpha / st->omega);
+      st->beta = beta;
+      st->rr0 = t[1];
+      if (conv) { st->status = S_OK; st->done = 1; }
+      else if (!finite(beta1) || !finite(beta)) { st->status = S_NONFINITE; st->done = 1; }
+      else if (j >= a.maxit) { st->status = S_MAXIT; st->done = 1; }
+    });
+  }
+  if (ph == BI_P5) {
+    // p = b + beta (p - w v); z = p/m
+    const double beta = ld_dbl(&st->beta), w = ld_dbl(&st->omega);
+    for (int i = i0 + (int)threadIdx.x; i < i1; i += blockDim.x) {
+      const double pi = a.bb[i] + beta * (a.p[i] - w * a.v[i]);
+      a.p[i] = pi;
+      a.z[i] = pi / a.m[i];
+    }
+    double v[1] = {0.0};
+    return grid_reduce<0>(a.g, v, lds, [&](const double *) {});
+  }
+  // BI_FINAL: energy x.Ax (src/Iterativos.f:204-205) and ||b - Ax||^2
+  double v[2] = {0.0, 0.0};
+  for (int i = i0 + (int)threadIdx.x; i < i1; i += blockDim.x) {
+    const double ax = row_apply<F>(a.A, i, XVec{a.x});
+    const double ri = a.b[i] - ax;
+    v[0] += a.x[i] * ax;
+    v[1] += ri * ri;
+  }
+  return grid_reduce<2>(a.g, v, lds, [&](const double *t) {
+    st->energy = t[0];
+    st->res2 = t[1];
+  });
+}
+
+template <int F>
+__global__ __launch_bounds__(1024) void bicg_persistent(BicgArgs a) {
+  __shared__ double lds[kMaxK * kMaxWaves];
+  if (!bicg_phase<F>(a, BI_SETUP, lds)) return;
+  while (!ld_int(&a.st->done)) {
+    if (!bicg_phase<F>(a, BI_P1, lds)) return;
+    if (ld_int(&a.st->done)) break;
+    if (!bicg_phase<F>(a, BI_P2, lds)) return;
+    if (!bicg_phase<F>(a, BI_P3, lds)) return;
+    if (ld_int(&a.st->done)) break;
+    if (!bicg_phase<F>(a, BI_P4, lds)) return;
+    if (ld_int(&a.st->done)) break;
+    if (!bicg_phase<F>(a, BI_P5, lds)) return;
+  }
+  bicg_phase<F>(a, BI_FINAL, lds);
+}
+
+template <int F>
+__global__ __launch_bounds__(1024) void bicg_kernel(BicgArgs a) {
+  __shared__ double lds[kMaxK * kMaxWaves];
+  if (a.phase != BI_SETUP && a.phase != BI_FINAL && ld_int(&a.st->done)) return;
+  bicg_phase<F>(a, a.phase, lds);
+}
+
+// ---------------------------------------------------------------- GMRES --
+
+constexpr int kMaxGmresK = 256;
+
+enum { GM_START = 0, GM_MV = 1, GM_MGS = 2, GM_XUPD = 3, GM_FINAL = 4 };
+
+struct GmresArgs {
+  Mat A;
+  Grid g;
+  KState *st;
+  const double *m;
+  double *x;
+  const double *b;
+  double *gv;      // basis g(neq, k+1), column-major
+  double *h;       // h(k+1, k)
+  double *cs;      // c(k)  (1-based, k+1 entries)
+  double *sn;      // s(k)
+  double *e;       // e(k+1) (k+2 entries)
+  double *y;       // y(k)
+  double *nrm;     // normaliser of each basis column (k+2 entries)
+  double tol;
+  int k, maxit, phase;
+  size_t ld;       // leading dimension of gv (= neq)
+};
+
+// src/Iterativos.f:271-446: GMRES(k), left Jacobi, MGS, Givens.  Column c
+// (1-based) of g is gv + (c-1)*ld.  A column is stored unnormalised until the
+// first MGS phase of the step that uses it; until then its readers divide by
+// its norm on the fly, which is the same rounding as the reference's stored
+// g(i,c)/norm.
+template <int F>
+__device__ bool gmres_phase(const GmresArgs &a, int ph, double *lds) {
+  const int i0 = a.g.row_begin(), i1 = a.g.row_end();
+  KState *st = a.st;
+  const int k = a.k;
+  auto col = [&](int c) { return a.gv + (size_t)(c - 1) * a.ld; };
+  auto H = [&](int i, int j) -> double & { return a.h[(size_t)(j - 1) * (k + 1) + (i - 1)]; };
+  if (ph == GM_START) {
+    // g1 = (b - A x)/m; e(1) = ||g1||; cycle 1 starts from x = 0 and sets econv
+    const int first = ld_int(&st->cycle) == 0;
+    double *g1 = col(1);
+    double v[2] = {0.0, 0.0};
+    for (int i = i0 + (int)threadIdx.x; i < i1; i += blockDim.x) {
+      double ax;
+      const double bi = a.b[i];
+      if (first) {
+        a.x[i] = 0.0;
+        ax = 0.0;
+        v[1] += bi * bi;
+      } else {
+        ax = row_apply<F>(a.A, i, XVec{a.x});
+      }
+      const double gi = (bi - ax) / a.m[i];
+      g1[i] = gi;
+      v[0] += gi * gi;
+    }
+    return grid_reduce<2>(a.g, v, lds, [&](const double *t) {
+      if (first) {
+        st->bnorm2 = t[1];
+        st->econv = a.tol * sqrt(t[1]);
+        st->conv = st->econv;
+        st->inner = 0;
+        st->status = S_MAXIT;
+        st->done = 0;
+      }
+      st->cycle = st->cycle + 1;
+      const double e1 = sqrt(t[0]);
+      a.e[1] = e1;
+      a.nrm[1] = e1;
+      st->step_i = 1;
+      st->mgs_j = 1;
+      st->inner_done = 0;
+      st->ni = 0;
+      if (e1 == 0.0) {          // exact solution already: nothing to orthogonalise
+        st->status = S_OK;
+        st->done = 1;
+        st->inner_done = 1;
+        st->last = 0.0;
+        st->iter = st->cycle;
+      } else if (!finite(e1)) {
+        st->status = S_NONFINITE;
+        st->done = 1;
+        st->inner_done = 1;
+      }
+    });
+  }
+  if (ph == GM_MV) {
+    // g(i+1) = A g(i) / m; h(1,i) = g(i+1).g(1)
+    const int i = ld_int(&st->step_i);
+    const double ni = ld_dbl(&a.nrm[i]);
+    const double n1 = ld_dbl(&a.nrm[1]);
+    const double *gi = col(i);
+    const double *g1 = col(1);
+    double *gn = col(i + 1);
+    const XScaled xf{gi, ni};
+    double v[1] = {0.0};
+    for (int r = i0 + (int)threadIdx.x; r < i1; r += blockDim.x) {
+      const double w = row_apply<F>(a.A, r, xf) / a.m[r];
+      gn[r] = w;
+      const double g1r = (i == 1) ? g1[r] / n1 : g1[r];
+      v[0] += w * g1r;
+    }
+    return grid_reduce<1>(a.g, v, lds, [&](const double *t) {
+      H(1, i) = t[0];
+      st->mgs_j = 1;
+    });
+  }
+  if (ph == GM_MGS) {
+    // g(i+1) -= h(j,i) g(j); then h(j+1,i) = g(i+1).g(j+1), or the norm at j = i
+    const int i = ld_int(&st->step_i);
+    const int j = ld_int(&st->mgs_j);
+    const double hj = ld_dbl(&a.h[(size_t)(i - 1) * (k + 1) + (j - 1)]);
+    double *gi = col(i);
+    double *gn = col(i + 1);
+    const double *gj = col(j);
+    const double *gj1 = col(j + 1);
+    const double ni = ld_dbl(&a.nrm[i]);
+    double v[1] = {0.0};
+    for (int r = i0 + (int)threadIdx.x; r < i1; r += blockDim.x) {
+      if (j == 1) gi[r] = gi[r] / ni;      // store column i normalised
+      const double w = gn[r] - hj * gj[r];
+      gn[r] = w;
+      v[0] += (j < i) ? w * gj1[r] : w * w;
+    }
+    return grid_reduce<1>(a.g, v, lds, [&](const double *t) {
+      if (j < i) {
+        H(j + 1, i) = t[0];
+        st->mgs_j = j + 1;
+        return;
+      }
+      const double norm = sqrt(t[0]);
+      H(i + 1, i) = norm;
+      a.nrm[i + 1] = norm;
+      for (int jj = 1; jj <= i - 1; ++jj) {
+        const double aux1 = a.cs[jj] * H(jj, i) + a.sn[jj] * H(jj + 1, i);
+        const double aux2 = -a.sn[jj] * H(jj, i) + a.cs[jj] * H(jj + 1, i);
+        H(jj, i) = aux1;
+        H(jj + 1, i) = aux2;
+      }
+      const double rr = sqrt(H(i, i) * H(i, i) + H(i + 1, i) * H(i + 1, i));
+      a.cs[i] = H(i, i) / rr;
+      a.sn[i] = H(i + 1, i) / rr;
+      H(i, i) = rr;
+      H(i + 1, i) = 0.0;
+      a.e[i + 1] = -a.sn[i] * a.e[i];
+      a.e[i] = a.cs[i] * a.e[i];
+      st->inner = st->inner + 1;
+      const bool conv = fabs(a.e[i + 1]) <= st->econv;   // src/Iterativos.f:376
+      if (conv || i == k || !finite(a.e[i + 1])) {
+        const int nI = i;
+        st->ni = nI;
+        st->inner_done = 1;
+        // back substitution h y = e (src/Iterativos.f:382-389)
+        a.y[nI] = a.e[nI] / H(nI, nI);
+        for (int ii = nI - 1; ii >= 1; --ii) {
+          double yi = 0.0;
+          for (int jj = ii + 1; jj <= nI; ++jj) yi = yi - H(ii, jj) * a.y[jj];
+          a.y[ii] = (yi + a.e[ii]) / H(ii, ii);
+        }
+      } else {
+        st->step_i = i + 1;
+        st->mgs_j = 1;
+      }
+    });
+  }
+  if (ph == GM_XUPD) {
+    // x += sum_j y(j) g(j), j in order (src/Iterativos.f:393-397); convergence check
+    const int nI = ld_int(&st->ni);
+    __shared__ double ysh[kMaxGmresK + 2];
+    for (int j = (int)threadIdx.x + 1; j <= nI; j += blockDim.x) ysh[j] = ld_dbl(&a.y[j]);
+    __syncthreads();
+    double v[1] = {0.0};
+    for (int r = i0 + (int)threadIdx.x; r < i1; r += blockDim.x) {
+      double xr = a.x[r];
+      for (int j = 1; j <= nI; ++j) xr = xr + ysh[j] * a.gv[(size_t)(j - 1) * a.ld + r];
+      a.x[r] = xr;
+    }
+    return grid_reduce<0>(a.g, v, lds, [&](const double *) {
+      const double en = a.e[nI + 1];
+      st->last = fabs(en);
+      st->iter = st->cycle;
+      if (fabs(en) <= st->econv) { st->status = S_OK; st->done = 1; }   // :402
+      else if (!finite(en)) { st->status = S_NONFINITE; st->done = 1; }
+      else if (st->cycle >= a.maxit) { st->status = S_MAXIT; st->done = 1; }
+      st->inner_done = 0;
+    });
+  }
+  // GM_FINAL: energy x.b (src/Iterativos.f:412) and ||b - Ax||^2
+  double v[2] = {0.0, 0.0};
+  for (int i = i0 + (int)threadIdx.x; i < i1; i += blockDim.x) {
+    const double ax = row_apply<F>(a.A, i, XVec{a.x});
+    const double bi = a.b[i];
+    const double ri = bi - ax;
+    v[0] += a.x[i] * bi;
+    v[1] += ri * ri;
+  }
+  return grid_reduce<2>(a.g, v, lds, [&](const double *t) {
+    st->energy = t[0];
+    st->res2 = t[1];
+  });
+}
+
+template <int F>
+__global__ __launch_bounds__(1024) void gmres_persistent(GmresArgs a) {
+  __shared__ double lds[kMaxK * kMaxWaves];
+  while (!ld_int(&a.st->done)) {
+    if (!gmres_phase<F>(a, GM_START, lds)) return;
+    if (ld_int(&a.st->done)) break;
+    for (int i = 1; i <= a.k; ++i) {
+      if (!gmres_phase<F>(a, GM_MV, lds)) return;
+      for (int j = 1; j <= i; ++j)
+        if (!gmres_phase<F>(a, GM_MGS, lds)) return;
+      if (ld_int(&a.st->inner_done)) break;
+    }
+    if (!gmres_phase<F>(a, GM_XUPD, lds)) return;
+  }
+  gmres_phase<F>(a, GM_FINAL, lds);
+}
+
+template <int F>
+__global__ __launch_bounds__(1024) void gmres_kernel(GmresArgs a) {
+  __shared__ double lds[kMaxK * kMaxWaves];
+  const int ph = a.phase;
+  if (ph != GM_FINAL && ld_int(&a.st->done)) return;
+  if ((ph == GM_MV || ph == GM_MGS) && ld_int(&a.st->inner_done)) return;
+  gmres_phase<F>(a, ph, lds);
+}
+
+// ---------------------------------------------------------- plain kernels --
+
+template <int F>
+__global__ __launch_bounds__(256) void spmv_kernel(Mat A, const double *__restrict__ x,
+                                                   double *__restrict__ y) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < A.neq) y[i] = row_apply<F>(A, i, XVec{x});
+}
+
+// a.b over G chunks, reduced by the last block into *out (device).
+__global__ __launch_bounds__(1024) void dot_kernel(Grid g, const double *a, const double *b,
+                                                   double *out) {
+  __shared__ double lds[kMaxK * kMaxWaves];
+  const int i0 = g.row_begin(), i1 = g.row_end();
+  double v[1] = {0.0};
+  for (int i = i0 + (int)threadIdx.x; i < i1; i += blockDim.x) v[0] += a[i] * b[i];
+  grid_reduce<1>(g, v, lds, [&](const double *t) { *out = t[0]; });
+}
+
+// Resets the barrier counters and the solver state (a kernel rather than a
+// memset, so that the sequence can be captured in a graph).
+__global__ void reset_kernel(unsigned *ctr, int nctr, KState *st) {
+  for (int i = threadIdx.x; i < nctr; i += blockDim.x) ctr[i] = 0u;
+  if (threadIdx.x == 0) {
+    KState z{};
+    z.status = S_MAXIT;
+    *st = z;
+  }
+}
+
+// The Jacobi diagonal of a full CSR matrix (diagonal stored in the row).
+__global__ void csr_diag_kernel(int neq, const int *ia, const int *ja, const double *a,
+                                double *m) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= neq) return;
+  double d = 0.0;
+  for (int k = ia[i] - 1; k < ia[i + 1] - 1; ++k)
+    if (ja[k] - 1 == i) d = a[k];
+  m[i] = d;
+}
+
+// getRes (src/Solver.f:368-391)
+__global__ void getres_kernel(double *u, double *x, const int *num, int numel, int flag) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= numel) return;
+  if (flag) x[num[i] - 1] = u[i];
+  else u[i] = x[num[i] - 1];
+}
+
+}  // namespace fvc

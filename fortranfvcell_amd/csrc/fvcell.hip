@@ -1,0 +1,828 @@
+// fvcell.hip -- host side of the gfx950 solver library: the C ABI declared
+// in include/fvcell.h, device workspaces, launch configuration and the two
+// engines (persistent cooperative kernel, kernel per phase).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "../../include/fvcell.h"
+#include "fvc_krylov.h"
+
+using namespace fvc;
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const char *fmt, ...) {
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof buf, fmt, ap);
+  va_end(ap);
+  g_err = buf;
+  return code;
+}
+
+#define HIPCHK(call)                                                                        \
+  do {                                                                                      \
+    hipError_t e_ = (call);                                                                 \
+    if (e_ != hipSuccess)                                                                   \
+      return fail(FVC_DEVICE_ERROR, "%s failed: %s (%s:%d)", #call, hipGetErrorString(e_),  \
+                  __FILE__, __LINE__);                                                      \
+  } while (0)
+
+constexpr int kDefaultBlock = 512;
+
+// ------------------------------------------------------- device workspace --
+
+struct DevState {
+  bool ready = false;
+  int device = -1;
+  int cus = 0;
+  char name[256] = {0};
+  std::mutex mu;
+  // one grow-only device arena for the solver's scratch vectors and state
+  char *arena = nullptr;
+  size_t arena_bytes = 0;
+  KState *host_st = nullptr;   // pinned
+  double *host_scalar = nullptr;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+};
+
+DevState g_dev[64];
+std::mutex g_init_mu;
+
+int current_device(DevState **out) {
+  int dev = 0;
+  HIPCHK(hipGetDevice(&dev));
+  if (dev < 0 || dev >= 64) return fail(FVC_DEVICE_ERROR, "device index %d out of range", dev);
+  DevState &d = g_dev[dev];
+  if (!d.ready) {
+    std::lock_guard<std::mutex> lk(g_init_mu);
+    if (!d.ready) {
+      hipDeviceProp_t prop;
+      HIPCHK(hipGetDeviceProperties(&prop, dev));
+      d.device = dev;
+      d.cus = prop.multiProcessorCount;
+      std::snprintf(d.name, sizeof d.name, "%s (%s)", prop.name, prop.gcnArchName);
+      HIPCHK(hipHostMalloc((void **)&d.host_st, sizeof(KState), hipHostMallocDefault));
+      HIPCHK(hipHostMalloc((void **)&d.host_scalar, 64, hipHostMallocDefault));
+      HIPCHK(hipEventCreate(&d.ev0));
+      HIPCHK(hipEventCreate(&d.ev1));
+      d.ready = true;
+    }
+  }
+  *out = &d;
+  return FVC_OK;
+}
+
+size_t align_up(size_t v, size_t a = 256) { return (v + a - 1) / a * a; }
+
+// Carves the arena into the pieces a solve needs.
+struct Carve {
+  char *base;
+  size_t off = 0;
+  template <class T>
+  T *take(size_t n) {
+    T *p = reinterpret_cast<T *>(base + off);
+    off = align_up(off + n * sizeof(T));
+    return p;
+  }
+};
+
+int ensure_arena(DevState &d, size_t bytes) {
+  if (d.arena_bytes >= bytes) return FVC_OK;
+  if (d.arena) {
+    HIPCHK(hipDeviceSynchronize());
+    HIPCHK(hipFree(d.arena));
+    d.arena = nullptr;
+    d.arena_bytes = 0;
+  }
+  size_t want = std::max(bytes, d.arena_bytes * 5 / 4);
+  HIPCHK(hipMalloc((void **)&d.arena, want));
+  d.arena_bytes = want;
+  return FVC_OK;
+}
+
+// ------------------------------------------------------------ matrix view --
+
+struct MatDesc {
+  int F = -1;
+  Mat m{};
+  int nval = 0;          // entries of av (and au)
+  bool needs_trans = false;
+  bool csr_diag = false; // Jacobi diagonal must be extracted (full CSR)
+};
+
+int fmt_for(int matriz, int unsym, int w, int *F) {
+  switch (matriz) {
+    case FVC_ELLPACK: *F = (w == 4) ? F_ELL4 : (w == 3) ? F_ELL3 : F_ELLW; return FVC_OK;
+    case FVC_CSRD: *F = unsym ? F_CSRD : F_LSYM; return FVC_OK;
+    case FVC_CSR: *F = F_CSR; return FVC_OK;
+    case FVC_CSRC: *F = unsym ? F_CSRC : F_LSYM; return FVC_OK;
+  }
+  return fail(FVC_UNSUPPORTED, "matriz %d is not a storage format (1 CSRD, 2 CSR, 3 CSRC, 4 ELLPACK)", matriz);
+}
+
+// Transposed-gather structure of a lower-triangle pattern: for each row c,
+// the entries (row i > c, position k) with ja(k) = c, in increasing i.
+void build_transpose(int neq, const int *ia, const int *ja, std::vector<int> &tptr,
+                     std::vector<int> &tsrc, std::vector<int> &tpos) {
+  const int nnz = ia[neq] - ia[0];
+  tptr.assign(neq + 1, 0);
+  for (int k = 0; k < nnz; ++k) {
+    const int c = ja[k] - 1;
+    if (c >= 0 && c < neq) tptr[c + 1]++;
+  }
+  for (int i = 0; i < neq; ++i) tptr[i + 1] += tptr[i];
+  tsrc.assign(std::max(nnz, 1), 0);
+  tpos.assign(std::max(nnz, 1), 0);
+  std::vector<int> fill(tptr.begin(), tptr.end() - 1);
+  for (int i = 0; i < neq; ++i)
+    for (int k = ia[i] - 1; k < ia[i + 1] - 1; ++k) {
+      const int c = ja[k] - 1;
+      if (c < 0 || c >= neq) continue;
+      const int e = fill[c]++;
+      tsrc[e] = i;
+      tpos[e] = k;
+    }
+}
+
+// --------------------------------------------------------- launch config --
+
+template <class K>
+int occupancy(K kernel, int bs, int *nb) {
+  *nb = 0;
+  HIPCHK(hipOccupancyMaxActiveBlocksPerMultiprocessor(nb, kernel, bs, 0));
+  return FVC_OK;
+}
+
+Grid make_grid(int neq, int G, unsigned *ctr, double *partials, bool persistent) {
+  Grid g;
+  g.G = G;
+  g.neq = neq;
+  g.R = (neq + G - 1) / G;
+  g.ctr = ctr;
+  g.partials = partials;
+  g.persistent = persistent;
+  return g;
+}
+
+int choose_blocks(int neq, int cus, int occ_blocks, int requested) {
+  int G = requested > 0 ? requested : cus * std::max(1, std::min(occ_blocks, 2));
+  G = std::min(G, std::max(1, (neq + 63) / 64));
+  if (G >= 64) G -= G % 8;
+  return std::max(G, 1);
+}
+
+// ------------------------------------------------------------ the solves --
+
+struct SolveIn {
+  int code, F;
+  Mat A;
+  const double *m;
+  double *x;
+  const double *b;
+  double tol;
+  int maxit, x0_zero, stop_rule, gmres_k, engine, bs, G_req;
+};
+
+template <class KArgs>
+int launch(const void *fn, bool coop, int G, int bs, KArgs &args, hipStream_t s) {
+  void *params[] = {&args};
+  if (coop) {
+    HIPCHK(hipLaunchCooperativeKernel(fn, dim3(G), dim3(bs), params, 0, s));
+  } else {
+    HIPCHK(hipLaunchKernel(fn, dim3(G), dim3(bs), params, 0, s));
+  }
+  return FVC_OK;
+}
+
+int read_state(DevState &d, KState *st, hipStream_t s) {
+  HIPCHK(hipMemcpyAsync(d.host_st, st, sizeof(KState), hipMemcpyDeviceToHost, s));
+  HIPCHK(hipStreamSynchronize(s));
+  return FVC_OK;
+}
+
+template <int F>
+int run_solve(DevState &d, const SolveIn &in, fvc_stats *stats, hipStream_t s) {
+  const int neq = in.A.neq;
+  const int bs = in.bs > 0 ? in.bs : kDefaultBlock;
+  if (bs % 64 != 0 || bs > 1024) return fail(FVC_BAD_ARGUMENT, "block size %d must be a multiple of 64 <= 1024", bs);
+  const void *pfn = nullptr, *kfn = nullptr;
+  if (in.code == FVC_PCG) {
+    pfn = (const void *)&pcg_persistent<F>;
+    kfn = (const void *)&pcg_kernel<F>;
+  } else if (in.code == FVC_BICGSTAB) {
+    pfn = (const void *)&bicg_persistent<F>;
+    kfn = (const void *)&bicg_kernel<F>;
+  } else {
+    pfn = (const void *)&gmres_persistent<F>;
+    kfn = (const void *)&gmres_kernel<F>;
+  }
+  int occ = 0;
+  int rc = occupancy(pfn, bs, &occ);
+  if (rc) return rc;
+  if (occ < 1) return fail(FVC_DEVICE_ERROR, "persistent kernel does not fit a CU at block size %d", bs);
+  const int G = choose_blocks(neq, d.cus, occ, in.G_req);
+  bool persistent = in.engine != FVC_ENGINE_KERNELS;
+  if (persistent && G > occ * d.cus) {
+    if (in.engine == FVC_ENGINE_PERSISTENT)
+      return fail(FVC_BAD_ARGUMENT, "%d blocks exceed the %d co-resident blocks of a persistent launch", G, occ * d.cus);
+    persistent = false;
+  }
+  const int k = in.gmres_k;
+  if (in.code == FVC_GMRES && (k < 1 || k > kMaxGmresK))
+    return fail(FVC_BAD_ARGUMENT, "GMRES restart k = %d outside [1, %d]", k, kMaxGmresK);
+
+  // workspace
+  const size_t n = (size_t)neq;
+  int nvec = in.code == FVC_PCG ? 5 : in.code == FVC_BICGSTAB ? 6 : (k + 1);
+  size_t bytes = align_up(kCtrWords * sizeof(unsigned)) + align_up(sizeof(KState)) +
+                 align_up(kMaxK * (size_t)G * sizeof(double)) + (size_t)nvec * align_up(n * sizeof(double)) +
+                 align_up((size_t)(k + 2) * (k + 2) * sizeof(double)) * 2 + 4096;
+  rc = ensure_arena(d, bytes);
+  if (rc) return rc;
+  Carve cv{d.arena};
+  unsigned *ctr = cv.take<unsigned>(kCtrWords);
+  KState *st = cv.take<KState>(1);
+  double *partials = cv.take<double>(kMaxK * (size_t)G);
+  std::vector<double *> vec(nvec);
+  for (int i = 0; i < nvec; ++i) vec[i] = cv.take<double>(n);
+  double *gsmall = cv.take<double>((size_t)(k + 2) * (k + 2) * 2);
+
+  Grid g = make_grid(neq, G, ctr, partials, persistent);
+  HIPCHK(hipEventRecord(d.ev0, s));
+  hipLaunchKernelGGL(reset_kernel, dim3(1), dim3(256), 0, s, ctr, (int)kCtrWords, st);
+  HIPCHK(hipGetLastError());
+
+  if (in.code == FVC_PCG) {
+    PcgArgs a{};
+    a.A = in.A; a.g = g; a.st = st; a.m = in.m; a.x = in.x; a.b = in.b;
+    a.r = vec[0]; a.z = vec[1]; a.p0 = vec[2]; a.p1 = vec[3]; a.q = vec[4];
+    a.tol = in.tol; a.maxit = in.maxit; a.x0_zero = in.x0_zero; a.stop_rule = in.stop_rule;
+    if (persistent) {
+      a.phase = -1;
+      rc = launch(pfn, true, G, bs, a, s);
+      if (rc) return rc;
+      HIPCHK(hipEventRecord(d.ev1, s));
+      rc = read_state(d, st, s);
+      if (rc) return rc;
+    } else {
+      a.phase = PCG_SETUP;
+      if ((rc = launch(kfn, false, G, bs, a, s))) return rc;
+      int batch = 8, launched = 0;
+      for (;;) {
+        for (int it = 0; it < batch && launched < in.maxit; ++it, ++launched) {
+          a.phase = PCG_A;
+          if ((rc = launch(kfn, false, G, bs, a, s))) return rc;
+          a.phase = PCG_B;
+          if ((rc = launch(kfn, false, G, bs, a, s))) return rc;
+        }
+        if ((rc = read_state(d, st, s))) return rc;
+        if (d.host_st->done || launched >= in.maxit) break;
+        batch = std::min(batch * 2, 256);
+      }
+      a.phase = PCG_FINAL;
+      if ((rc = launch(kfn, false, G, bs, a, s))) return rc;
+      HIPCHK(hipEventRecord(d.ev1, s));
+      if ((rc = read_state(d, st, s))) return rc;
+    }
+  } else if (in.code == FVC_BICGSTAB) {
+    BicgArgs a{};
+    a.A = in.A; a.g = g; a.st = st; a.m = in.m; a.x = in.x; a.b = in.b;
+    a.bb = vec[0]; a.r = vec[1]; a.p = vec[2]; a.z = vec[3]; a.v = vec[4]; a.t = vec[5];
+    a.tol = in.tol; a.maxit = in.maxit; a.x0_zero = in.x0_zero; a.stop_rule = in.stop_rule;
+    if (persistent) {
+      a.phase = -1;
+      if ((rc = launch(pfn, true, G, bs, a, s))) return rc;
+      HIPCHK(hipEventRecord(d.ev1, s));
+      if ((rc = read_state(d, st, s))) return rc;
+    } else {
+      a.phase = BI_SETUP;
+      if ((rc = launch(kfn, false, G, bs, a, s))) return rc;
+      int batch = 8, launched = 0;
+      for (;;) {
+        for (int it = 0; it < batch && launched < in.maxit; ++it, ++launched)
+          for (int ph = BI_P1; ph <= BI_P5; ++ph) {
+            a.phase = ph;
+            if ((rc = launch(kfn, false, G, bs, a, s))) return rc;
+          }
+        if ((rc = read_state(d, st, s))) return rc;
+        if (d.host_st->done || launched >= in.maxit) break;
+        batch = std::min(batch * 2, 256);
+      }
+      a.phase = BI_FINAL;
+      if ((rc = launch(kfn, false, G, bs, a, s))) return rc;
+      HIPCHK(hipEventRecord(d.ev1, s));
+      if ((rc = read_state(d, st, s))) return rc;
+    }
+  } else {
+    GmresArgs a{};
+    a.A = in.A; a.g = g; a.st = st; a.m = in.m; a.x = in.x; a.b = in.b;
+    a.gv = vec[0];
+    a.ld = n;
+    // vec[] are carved consecutively with 256-byte alignment: use one stride
+    if (nvec > 1) a.ld = (size_t)(vec[1] - vec[0]);
+    double *sm = gsmall;
+    a.h = sm; sm += (size_t)(k + 1) * k;
+    a.cs = sm; sm += k + 2;
+    a.sn = sm; sm += k + 2;
+    a.e = sm; sm += k + 2;
+    a.y = sm; sm += k + 2;
+    a.nrm = sm; sm += k + 2;
+    a.tol = in.tol; a.k = k; a.maxit = in.maxit;
+    if (persistent) {
+      a.phase = -1;
+      if ((rc = launch(pfn, true, G, bs, a, s))) return rc;
+      HIPCHK(hipEventRecord(d.ev1, s));
+      if ((rc = read_state(d, st, s))) return rc;
+    } else {
+      for (int cyc = 0; cyc < in.maxit; ++cyc) {
+        a.phase = GM_START;
+        if ((rc = launch(kfn, false, G, bs, a, s))) return rc;
+        for (int i = 1; i <= k; ++i) {
+          a.phase = GM_MV;
+          if ((rc = launch(kfn, false, G, bs, a, s))) return rc;
+          a.phase = GM_MGS;
+          for (int j = 1; j <= i; ++j)
+            if ((rc = launch(kfn, false, G, bs, a, s))) return rc;
+        }
+        a.phase = GM_XUPD;
+        if ((rc = launch(kfn, false, G, bs, a, s))) return rc;
+        if ((rc = read_state(d, st, s))) return rc;
+        if (d.host_st->done) break;
+      }
+      a.phase = GM_FINAL;
+      if ((rc = launch(kfn, false, G, bs, a, s))) return rc;
+      HIPCHK(hipEventRecord(d.ev1, s));
+      if ((rc = read_state(d, st, s))) return rc;
+    }
+  }
+  HIPCHK(hipGetLastError());
+  const KState &hs = *d.host_st;
+  float ms = 0.f;
+  HIPCHK(hipEventElapsedTime(&ms, d.ev0, d.ev1));
+  if (stats) {
+    std::memset(stats, 0, sizeof *stats);
+    stats->status = hs.status;
+    stats->iterations = hs.iter;
+    stats->inner_iterations = in.code == FVC_GMRES ? hs.inner : hs.iter;
+    stats->engine = persistent ? FVC_ENGINE_PERSISTENT : FVC_ENGINE_KERNELS;
+    stats->energy = hs.energy;
+    stats->conv = in.code == FVC_GMRES ? hs.econv : hs.conv;
+    stats->last = hs.last;
+    stats->true_relres = hs.bnorm2 > 0 ? std::sqrt(hs.res2 / hs.bnorm2) : std::sqrt(hs.res2);
+    stats->seconds = ms * 1e-3;
+    stats->grid_blocks = G;
+    stats->block_size = bs;
+  }
+  // A bounded grid wait that expired is a device error, never a result.
+  unsigned err = 0;
+  HIPCHK(hipMemcpy(&err, ctr + kCtrErr, sizeof err, hipMemcpyDeviceToHost));
+  if (err) return fail(FVC_DEVICE_ERROR, "persistent solve: a grid barrier wait expired");
+  if (hs.status != S_OK) {
+    static const char *names[] = {"converged", "maximum iterations reached", "breakdown", "non-finite value"};
+    return fail(hs.status, "%s after %d iterations", names[std::min(std::max(hs.status, 0), 3)], hs.iter);
+  }
+  return FVC_OK;
+}
+
+int dispatch_solve(DevState &d, const SolveIn &in, fvc_stats *stats, hipStream_t s) {
+  switch (in.F) {
+    case F_ELL4: return run_solve<F_ELL4>(d, in, stats, s);
+    case F_ELL3: return run_solve<F_ELL3>(d, in, stats, s);
+    case F_ELLW: return run_solve<F_ELLW>(d, in, stats, s);
+    case F_CSRD: return run_solve<F_CSRD>(d, in, stats, s);
+    case F_CSR: return run_solve<F_CSR>(d, in, stats, s);
+    case F_LSYM: return run_solve<F_LSYM>(d, in, stats, s);
+    case F_CSRC: return run_solve<F_CSRC>(d, in, stats, s);
+  }
+  return fail(FVC_UNSUPPORTED, "no kernel for format %d", in.F);
+}
+
+// The reference's x0 choice per path (src/Solver.f:89-263, src/Iterativos.f:288).
+bool x0_is_zero(int code, int matriz, int unsym, int rule) {
+  if (code == FVC_GMRES) return true;
+  if (rule == FVC_X0_ZERO) return true;
+  if (rule == FVC_X0_GIVEN) return false;
+  const bool omp = rule == FVC_X0_AS_OPENMP;
+  if (matriz == FVC_ELLPACK) return false;
+  if (matriz == FVC_CSR || matriz == FVC_CSRC) return true;
+  // CSRD
+  if (code == FVC_PCG) return !omp;   // pcg_omp init=.false. / pcg init=.true.
+  return true;                        // pbicgstab[_omp] init=.true.
+}
+
+fvc_options defaults() {
+  fvc_options o;
+  std::memset(&o, 0, sizeof o);
+  o.x0_rule = FVC_X0_AS_OPENMP;
+  o.gmres_k = 15;
+  o.stop_rule = FVC_STOP_REFERENCE;
+  o.true_residual = 1;
+  o.engine = FVC_ENGINE_AUTO;
+  return o;
+}
+
+int check_common(int code, int matriz, int neq, const void *ad, const void *x, const void *b,
+                 const void *iax, const void *ja) {
+  if (code < FVC_PCG || code > FVC_GMRES) return fail(FVC_UNSUPPORTED, "solver code %d (1 PCG, 2 BiCGSTAB, 3 GMRES)", code);
+  if (matriz < FVC_CSRD || matriz > FVC_ELLPACK) return fail(FVC_UNSUPPORTED, "matriz %d", matriz);
+  if (neq <= 0) return fail(FVC_BAD_ARGUMENT, "neq = %d", neq);
+  if (!ad || !x || !b || !iax || !ja) return fail(FVC_BAD_ARGUMENT, "null array argument");
+  return FVC_OK;
+}
+
+// Builds the device matrix from host arrays (uploading them) or device
+// arrays (used in place).  Owned uploads land in `own`.
+struct Owned {
+  std::vector<void *> ptrs;
+  ~Owned() {
+    for (void *p : ptrs) (void)hipFree(p);
+  }
+  template <class T>
+  int up(T **dst, const T *src, size_t n) {
+    if (n == 0) { *dst = nullptr; return FVC_OK; }
+    void *p = nullptr;
+    HIPCHK(hipMalloc(&p, n * sizeof(T)));
+    ptrs.push_back(p);
+    if (src) HIPCHK(hipMemcpy(p, src, n * sizeof(T), hipMemcpyHostToDevice));
+    *dst = (T *)p;
+    return FVC_OK;
+  }
+};
+
+int build_device_matrix(int matriz, int unsym, int neq, int nad, const double *ad,
+                        const double *au, const double *al, const int32_t *iax,
+                        const int32_t *ja, bool on_device, Owned &own, MatDesc &md,
+                        const double **m_out) {
+  int w = 0;
+  std::vector<int> h_ia;
+  const int *ia_host = nullptr;
+  if (matriz == FVC_ELLPACK) {
+    int nl[2] = {0, 0};
+    if (on_device) HIPCHK(hipMemcpy(nl, iax, sizeof nl, hipMemcpyDefault));
+    else std::memcpy(nl, iax, sizeof nl);
+    w = nl[0];
+    if (w < 1 || w > 64) return fail(FVC_BAD_ARGUMENT, "ELLPACK width iax(1) = %d", w);
+  } else {
+    if (on_device) {
+      h_ia.resize(neq + 1);
+      HIPCHK(hipMemcpy(h_ia.data(), iax, (neq + 1) * sizeof(int), hipMemcpyDeviceToHost));
+      ia_host = h_ia.data();
+    } else {
+      ia_host = iax;
+    }
+    if (ia_host[0] != 1) return fail(FVC_BAD_ARGUMENT, "ia(1) = %d, expected 1", ia_host[0]);
+  }
+  int F = 0;
+  int rc = fmt_for(matriz, unsym, w, &F);
+  if (rc) return rc;
+  md.F = F;
+  md.m.neq = neq;
+  md.m.w = w;
+  const size_t nval = matriz == FVC_ELLPACK ? (size_t)w * neq : (size_t)(ia_host[neq] - 1);
+  md.nval = (int)nval;
+  (void)nad;
+  const double *vals = (matriz == FVC_CSR) ? ad : (F == F_CSRC ? al : (unsym ? (au ? au : al) : al));
+  if (matriz == FVC_ELLPACK) vals = au ? au : al;
+  if (!vals && nval > 0) return fail(FVC_BAD_ARGUMENT, "null off-diagonal array");
+  if (F == F_CSRC && !au) return fail(FVC_BAD_ARGUMENT, "CSRC unsymmetric needs au");
+  if (on_device) {
+    md.m.ad = ad;
+    md.m.av = (matriz == FVC_CSR) ? ad : vals;
+    md.m.au = au;
+    md.m.ia = matriz == FVC_ELLPACK ? nullptr : iax;
+    md.m.ja = ja;
+  } else {
+    double *d_ad = nullptr, *d_av = nullptr, *d_au = nullptr;
+    int *d_ia = nullptr, *d_ja = nullptr;
+    if (matriz == FVC_CSR) {
+      if ((rc = own.up(&d_av, ad, nval))) return rc;
+      d_ad = d_av;
+    } else {
+      if ((rc = own.up(&d_ad, ad, (size_t)neq))) return rc;
+      if ((rc = own.up(&d_av, vals, nval))) return rc;
+    }
+    if (F == F_CSRC && (rc = own.up(&d_au, au, nval))) return rc;
+    if (matriz != FVC_ELLPACK && (rc = own.up(&d_ia, iax, (size_t)neq + 1))) return rc;
+    if ((rc = own.up(&d_ja, ja, nval))) return rc;
+    md.m.ad = d_ad;
+    md.m.av = d_av;
+    md.m.au = d_au;
+    md.m.ia = d_ia;
+    md.m.ja = d_ja;
+  }
+  if (F == F_LSYM || F == F_CSRC) {
+    std::vector<int> hja;
+    const int *ja_host = ja;
+    if (on_device) {
+      hja.resize(std::max<size_t>(nval, 1));
+      if (nval) HIPCHK(hipMemcpy(hja.data(), ja, nval * sizeof(int), hipMemcpyDeviceToHost));
+      ja_host = hja.data();
+    }
+    std::vector<int> tptr, tsrc, tpos;
+    build_transpose(neq, ia_host, ja_host, tptr, tsrc, tpos);
+    int *p1, *p2, *p3;
+    if ((rc = own.up(&p1, tptr.data(), tptr.size()))) return rc;
+    if ((rc = own.up(&p2, tsrc.data(), tsrc.size()))) return rc;
+    if ((rc = own.up(&p3, tpos.data(), tpos.size()))) return rc;
+    md.m.tptr = p1;
+    md.m.tsrc = p2;
+    md.m.tpos = p3;
+  }
+  // Jacobi preconditioner m = diag(A)
+  if (matriz == FVC_CSR) {
+    double *dm = nullptr;
+    if ((rc = own.up(&dm, (const double *)nullptr, (size_t)neq))) return rc;
+    hipLaunchKernelGGL(csr_diag_kernel, dim3((neq + 255) / 256), dim3(256), 0, 0, neq,
+                       md.m.ia, md.m.ja, md.m.av, dm);
+    HIPCHK(hipGetLastError());
+    *m_out = dm;
+  } else {
+    *m_out = md.m.ad;
+  }
+  return FVC_OK;
+}
+
+int solve_impl(int code, int matriz, int unsym, int neq, int nad, const double *ad,
+               const double *au, const double *al, double *x, const double *b,
+               const int32_t *iax, const int32_t *ja, double tol, int maxit,
+               const fvc_options *opt, fvc_stats *stats, bool on_device, hipStream_t s) {
+  int rc = check_common(code, matriz, neq, ad, x, b, iax, ja);
+  if (stats) { std::memset(stats, 0, sizeof *stats); stats->status = rc; stats->true_relres = -1; }
+  if (rc) return rc;
+  fvc_options o = opt ? *opt : defaults();
+  if (o.gmres_k <= 0) o.gmres_k = 15;
+  DevState *d = nullptr;
+  if ((rc = current_device(&d))) return rc;
+  std::lock_guard<std::mutex> lk(d->mu);
+  Owned own;
+  MatDesc md;
+  const double *m = nullptr;
+  if ((rc = build_device_matrix(matriz, unsym, neq, nad, ad, au, al, iax, ja, on_device, own, md, &m))) {
+    if (stats) stats->status = rc;
+    return rc;
+  }
+  double *dx = x;
+  const double *db = b;
+  if (!on_device) {
+    double *tx = nullptr, *tb = nullptr;
+    if ((rc = own.up(&tx, x, (size_t)neq))) return rc;
+    if ((rc = own.up(&tb, b, (size_t)neq))) return rc;
+    dx = tx;
+    db = tb;
+  }
+  SolveIn in{};
+  in.code = code;
+  in.F = md.F;
+  in.A = md.m;
+  in.m = m;
+  in.x = dx;
+  in.b = db;
+  in.tol = tol;
+  in.maxit = maxit;
+  in.x0_zero = x0_is_zero(code, matriz, unsym, o.x0_rule) ? 1 : 0;
+  in.stop_rule = o.stop_rule;
+  in.gmres_k = o.gmres_k;
+  in.engine = o.engine;
+  in.bs = o.block_size;
+  in.G_req = o.grid_blocks;
+  if (code == FVC_GMRES && o.stop_rule != FVC_STOP_REFERENCE)
+    return fail(FVC_UNSUPPORTED, "GMRES supports only the reference stopping rule");
+  rc = dispatch_solve(*d, in, stats, s);
+  if (!on_device && (rc == FVC_OK || rc == FVC_MAXIT || rc == FVC_BREAKDOWN || rc == FVC_NONFINITE))
+    HIPCHK(hipMemcpy(x, dx, (size_t)neq * sizeof(double), hipMemcpyDeviceToHost));
+  return rc;
+}
+
+template <int F>
+int spmv_launch(const Mat &A, const double *x, double *y, hipStream_t s) {
+  hipLaunchKernelGGL(spmv_kernel<F>, dim3((A.neq + 255) / 256), dim3(256), 0, s, A, x, y);
+  HIPCHK(hipGetLastError());
+  return FVC_OK;
+}
+
+int matvec_impl(int matriz, int unsym, int neq, int nad, const double *ad, const double *au,
+                const double *al, const int32_t *iax, const int32_t *ja, const double *x,
+                double *y, bool on_device, hipStream_t s) {
+  if (neq <= 0 || !ad || !iax || !ja || !x || !y) return fail(FVC_BAD_ARGUMENT, "bad matvec arguments");
+  DevState *d = nullptr;
+  int rc = current_device(&d);
+  if (rc) return rc;
+  Owned own;
+  MatDesc md;
+  const double *m = nullptr;
+  if ((rc = build_device_matrix(matriz, unsym, neq, nad, ad, au, al, iax, ja, on_device, own, md, &m))) return rc;
+  const double *dx = x;
+  double *dy = y;
+  if (!on_device) {
+    double *tx = nullptr, *ty = nullptr;
+    if ((rc = own.up(&tx, x, (size_t)neq))) return rc;
+    if ((rc = own.up(&ty, (const double *)nullptr, (size_t)neq))) return rc;
+    dx = tx;
+    dy = ty;
+  }
+  switch (md.F) {
+    case F_ELL4: rc = spmv_launch<F_ELL4>(md.m, dx, dy, s); break;
+    case F_ELL3: rc = spmv_launch<F_ELL3>(md.m, dx, dy, s); break;
+    case F_ELLW: rc = spmv_launch<F_ELLW>(md.m, dx, dy, s); break;
+    case F_CSRD: rc = spmv_launch<F_CSRD>(md.m, dx, dy, s); break;
+    case F_CSR: rc = spmv_launch<F_CSR>(md.m, dx, dy, s); break;
+    case F_LSYM: rc = spmv_launch<F_LSYM>(md.m, dx, dy, s); break;
+    case F_CSRC: rc = spmv_launch<F_CSRC>(md.m, dx, dy, s); break;
+  }
+  if (rc) return rc;
+  if (!on_device) HIPCHK(hipMemcpy(y, dy, (size_t)neq * sizeof(double), hipMemcpyDeviceToHost));
+  else if (!own.ptrs.empty()) HIPCHK(hipStreamSynchronize(s));   // temporaries die with `own`
+  return FVC_OK;
+}
+
+int dot_impl(const double *a, const double *b, int n, double *result, bool on_device, hipStream_t s) {
+  if (n < 0 || !result || (n > 0 && (!a || !b))) return fail(FVC_BAD_ARGUMENT, "bad dot arguments");
+  if (n == 0) { *result = 0.0; return FVC_OK; }
+  DevState *d = nullptr;
+  int rc = current_device(&d);
+  if (rc) return rc;
+  std::lock_guard<std::mutex> lk(d->mu);
+  Owned own;
+  const double *da = a, *db = b;
+  if (!on_device) {
+    double *ta, *tb;
+    if ((rc = own.up(&ta, a, (size_t)n))) return rc;
+    if ((rc = own.up(&tb, b, (size_t)n))) return rc;
+    da = ta;
+    db = tb;
+  }
+  const int bs = kDefaultBlock;
+  const int G = choose_blocks(n, d->cus, 2, 0);
+  size_t bytes = align_up(kCtrWords * sizeof(unsigned)) + align_up(sizeof(KState)) +
+                 align_up(kMaxK * (size_t)G * sizeof(double)) + 4096;
+  if ((rc = ensure_arena(*d, bytes))) return rc;
+  Carve cv{d->arena};
+  unsigned *ctr = cv.take<unsigned>(kCtrWords);
+  KState *st = cv.take<KState>(1);
+  double *partials = cv.take<double>(kMaxK * (size_t)G);
+  double *out = cv.take<double>(1);
+  hipLaunchKernelGGL(reset_kernel, dim3(1), dim3(256), 0, s, ctr, (int)kCtrWords, st);
+  Grid g = make_grid(n, G, ctr, partials, false);
+  hipLaunchKernelGGL(dot_kernel, dim3(G), dim3(bs), 0, s, g, da, db, out);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipMemcpyAsync(d->host_scalar, out, sizeof(double), hipMemcpyDeviceToHost, s));
+  HIPCHK(hipStreamSynchronize(s));
+  *result = d->host_scalar[0];
+  return FVC_OK;
+}
+
+}  // namespace
+
+// =================================================================== C ABI ==
+
+extern "C" {
+
+void fvc_default_options(fvc_options *opt) {
+  if (opt) *opt = defaults();
+}
+
+int fvc_init(int32_t device) {
+  int n = 0;
+  HIPCHK(hipGetDeviceCount(&n));
+  if (n <= 0) return fail(FVC_DEVICE_ERROR, "no HIP device");
+  if (device < 0 || device >= n) return fail(FVC_BAD_ARGUMENT, "device %d of %d", device, n);
+  HIPCHK(hipSetDevice(device));
+  DevState *d = nullptr;
+  return current_device(&d);
+}
+
+const char *fvc_last_error(void) { return g_err.c_str(); }
+
+int fvc_solver(const double *ad, const double *au, const double *al, double *x, double *b,
+               const int32_t *iax, const int32_t *ja, int32_t neq, int32_t nad, int32_t unsym,
+               double tol, int32_t maxit, int32_t code, int32_t matriz, int32_t nlit,
+               int32_t omp_solver, fvc_stats *stats) {
+  (void)nlit;
+  fvc_options o = defaults();
+  o.x0_rule = omp_solver ? FVC_X0_AS_OPENMP : FVC_X0_AS_SEQUENTIAL;
+  return solve_impl(code, matriz, unsym, neq, nad, ad, au, al, x, b, iax, ja, tol, maxit, &o,
+                    stats, false, nullptr);
+}
+
+int fvc_solve(int32_t code, int32_t matriz, int32_t unsym, int32_t neq, int32_t nad,
+              const double *ad, const double *au, const double *al, double *x, const double *b,
+              const int32_t *iax, const int32_t *ja, double tol, int32_t maxit,
+              const fvc_options *opt, fvc_stats *stats) {
+  return solve_impl(code, matriz, unsym, neq, nad, ad, au, al, x, b, iax, ja, tol, maxit, opt,
+                    stats, false, nullptr);
+}
+
+int fvc_solve_dev(int32_t code, int32_t matriz, int32_t unsym, int32_t neq, int32_t nad,
+                  const double *ad, const double *au, const double *al, double *x,
+                  const double *b, const int32_t *iax, const int32_t *ja, double tol,
+                  int32_t maxit, const fvc_options *opt, fvc_stats *stats, void *stream) {
+  return solve_impl(code, matriz, unsym, neq, nad, ad, au, al, x, b, iax, ja, tol, maxit, opt,
+                    stats, true, (hipStream_t)stream);
+}
+
+int fvc_matvec(int32_t matriz, int32_t unsym, int32_t neq, int32_t nad, const double *ad,
+               const double *au, const double *al, const int32_t *iax, const int32_t *ja,
+               const double *x, double *y) {
+  return matvec_impl(matriz, unsym, neq, nad, ad, au, al, iax, ja, x, y, false, nullptr);
+}
+
+int fvc_matvec_dev(int32_t matriz, int32_t unsym, int32_t neq, int32_t nad, const double *ad,
+                   const double *au, const double *al, const int32_t *iax, const int32_t *ja,
+                   const double *x, double *y, void *stream) {
+  return matvec_impl(matriz, unsym, neq, nad, ad, au, al, iax, ja, x, y, true, (hipStream_t)stream);
+}
+
+int fvc_dot(const double *a, const double *b, int32_t n, double *result) {
+  return dot_impl(a, b, n, result, false, nullptr);
+}
+
+int fvc_dot_dev(const double *a, const double *b, int32_t n, double *result, void *stream) {
+  return dot_impl(a, b, n, result, true, (hipStream_t)stream);
+}
+
+int fvc_getres(double *u, double *x, const int32_t *num, int32_t neq, int32_t numel, int32_t flag) {
+  if (!u || !x || !num || numel < 0) return fail(FVC_BAD_ARGUMENT, "bad getres arguments");
+  for (int32_t i = 0; i < numel; ++i) {
+    const int32_t e = num[i];
+    if (e < 1 || e > neq) return fail(FVC_BAD_ARGUMENT, "num(%d) = %d outside 1..%d", i + 1, e, neq);
+  }
+  if (flag) for (int32_t i = 0; i < numel; ++i) x[num[i] - 1] = u[i];
+  else for (int32_t i = 0; i < numel; ++i) u[i] = x[num[i] - 1];
+  return FVC_OK;
+}
+
+int fvc_getres_dev(double *u, double *x, const int32_t *num, int32_t neq, int32_t numel,
+                   int32_t flag, void *stream) {
+  (void)neq;
+  if (!u || !x || !num || numel < 0) return fail(FVC_BAD_ARGUMENT, "bad getres arguments");
+  if (numel == 0) return FVC_OK;
+  hipLaunchKernelGGL(getres_kernel, dim3((numel + 255) / 256), dim3(256), 0, (hipStream_t)stream,
+                     u, x, num, numel, flag);
+  HIPCHK(hipGetLastError());
+  return FVC_OK;
+}
+
+int fvc_malloc(void **ptr, size_t bytes) {
+  if (!ptr) return fail(FVC_BAD_ARGUMENT, "null pointer");
+  HIPCHK(hipMalloc(ptr, bytes));
+  return FVC_OK;
+}
+int fvc_free(void *ptr) {
+  HIPCHK(hipFree(ptr));
+  return FVC_OK;
+}
+int fvc_memcpy_htod(void *dst, const void *src, size_t bytes) {
+  HIPCHK(hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice));
+  return FVC_OK;
+}
+int fvc_memcpy_dtoh(void *dst, const void *src, size_t bytes) {
+  HIPCHK(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost));
+  return FVC_OK;
+}
+int fvc_memset_dev(void *dst, int value, size_t bytes) {
+  HIPCHK(hipMemset(dst, value, bytes));
+  return FVC_OK;
+}
+int fvc_synchronize(void) {
+  HIPCHK(hipDeviceSynchronize());
+  return FVC_OK;
+}
+int fvc_device_info(int32_t *compute_units, char *name, int32_t name_len) {
+  DevState *d = nullptr;
+  int rc = current_device(&d);
+  if (rc) return rc;
+  if (compute_units) *compute_units = d->cus;
+  if (name && name_len > 0) std::snprintf(name, (size_t)name_len, "%s", d->name);
+  return FVC_OK;
+}
+int fvc_event_create(void **ev) {
+  HIPCHK(hipEventCreate((hipEvent_t *)ev));
+  return FVC_OK;
+}
+int fvc_event_destroy(void *ev) {
+  HIPCHK(hipEventDestroy((hipEvent_t)ev));
+  return FVC_OK;
+}
+int fvc_event_record(void *ev, void *stream) {
+  HIPCHK(hipEventRecord((hipEvent_t)ev, (hipStream_t)stream));
+  return FVC_OK;
+}
+int fvc_event_elapsed_ms(void *start, void *stop, float *ms) {
+  HIPCHK(hipEventSynchronize((hipEvent_t)stop));
+  HIPCHK(hipEventElapsedTime(ms, (hipEvent_t)start, (hipEvent_t)stop));
+  return FVC_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,186 @@
+/*
+ * fvcell.h -- C ABI of the MI355X (gfx950) sparse-solver library that
+ * replaces fortranFVCell's linear-solver object (src/Solver.f: solver and
+ * getRes, with the Krylov routines of src/Iterativos*.f and the SpMV / dot
+ * kernels of src/Matvec*.f underneath).
+ *
+ * Conventions shared by every entry point:
+ *   - arrays are passed as plain pointers to element 1 of the reference's
+ *     Fortran arrays; index arrays hold the reference's 1-based int32 values;
+ *     values are IEEE double;
+ *   - layouts are the reference's (src/Datastruct.f):
+ *       matriz 1 CSRD   : iax = ia(neq+1), ja(nad) without the diagonal, sorted;
+ *                         unsym: au == al hold both triangles; else al = lower only
+ *       matriz 2 CSR    : iax = ia(neq+1), ja(nad) with the diagonal, all values in ad
+ *       matriz 3 CSRC   : lower pattern; al = row values, au = transposed upper
+ *                         (unsym); symmetric: al only
+ *       matriz 4 ELLPACK: iax = {w, neq+1}; ja(w,neq), al(w,neq) slot-fastest,
+ *                         padding ja = 1, al = 0; au == al
+ *   - "host" entry points take host memory and copy it to and from the GPU;
+ *     "_dev" entry points take device memory (hipMalloc'd) and a hipStream_t
+ *     passed as void* (NULL = the default stream);
+ *   - every entry point returns an fvc status code (FVC_OK = 0).
+ */
+#ifndef FVCELL_H
+#define FVCELL_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Status codes.  The reference stops the program where these are nonzero
+ * (src/Iterativos.f:76-77,198-199,405-406; src/Solver.f:225-227,306-308,338-339). */
+#define FVC_OK 0            /* converged under the requested stopping rule        */
+#define FVC_MAXIT 1         /* maxit reached (x holds the last iterate)           */
+#define FVC_BREAKDOWN 2     /* a Krylov denominator became exactly zero           */
+#define FVC_NONFINITE 3     /* a NaN or Inf appeared in the iteration scalars     */
+#define FVC_UNSUPPORTED 4   /* method x storage combination not available        */
+#define FVC_DEVICE_ERROR 5  /* a HIP call failed, or no usable GPU                */
+#define FVC_BAD_ARGUMENT 6  /* inconsistent sizes or null pointers                */
+
+/* Krylov methods, the reference's `code` (src/Solver.f:19-21). */
+#define FVC_PCG 1
+#define FVC_BICGSTAB 2
+#define FVC_GMRES 3
+
+/* Storage formats, the reference's `matriz` (src/Datastruct.f:12-15). */
+#define FVC_CSRD 1
+#define FVC_CSR 2
+#define FVC_CSRC 3
+#define FVC_ELLPACK 4
+
+/* x0 rules: which initial guess each path uses.  The reference zeroes x
+ * (init = .true.) on some paths and warm-starts on others, depending on the
+ * openmpSolver flag (src/Solver.f:94-165,196-260; src/Iterativos.f:288-290). */
+#define FVC_X0_AS_OPENMP 0      /* the reference with openmpSolver = .true. (the decks' setting) */
+#define FVC_X0_AS_SEQUENTIAL 1  /* the reference with openmpSolver = .false.                     */
+#define FVC_X0_ZERO 2
+#define FVC_X0_GIVEN 3
+
+/* Stopping rules. */
+#define FVC_STOP_REFERENCE 0    /* each method's own test (src/Iterativos.f:73,190,376)     */
+#define FVC_STOP_RELRES 1       /* recursive ||r||_2 <= tol*||b||_2, then a true-residual check */
+
+/* Engines. */
+#define FVC_ENGINE_AUTO 0
+#define FVC_ENGINE_KERNELS 1     /* one kernel per phase, reductions by the last block */
+#define FVC_ENGINE_PERSISTENT 2  /* one cooperative kernel per solve, grid barriers     */
+
+typedef struct fvc_options {
+  int32_t x0_rule;        /* FVC_X0_*; default FVC_X0_AS_OPENMP                      */
+  int32_t gmres_k;        /* restart length; 0 = 15 as src/Solver.f:63               */
+  int32_t stop_rule;      /* FVC_STOP_*                                              */
+  int32_t true_residual;  /* nonzero: also compute ||b - A x||_2 / ||b||_2 in FP64    */
+  int32_t engine;         /* FVC_ENGINE_*                                            */
+  int32_t block_size;     /* 0 = tuned default                                       */
+  int32_t grid_blocks;    /* 0 = tuned default (number of row chunks)                */
+  int32_t reserved[9];
+} fvc_options;
+
+typedef struct fvc_stats {
+  int32_t status;           /* FVC_* status                                          */
+  int32_t iterations;       /* PCG/BiCGSTAB: iterations j; GMRES: cycles l            */
+  int32_t inner_iterations; /* GMRES: Arnoldi steps (nit); else = iterations          */
+  int32_t engine;           /* engine that ran                                       */
+  double energy;            /* the "energy norm" the reference logs: x.Ax (PCG,
+                               BiCGSTAB; src/Iterativos.f:82-83,204-205) or x.b
+                               (GMRES; src/Iterativos.f:412)                         */
+  double conv;              /* stopping threshold                                    */
+  double last;              /* last tested quantity                                  */
+  double true_relres;       /* ||b - A x||_2 / ||b||_2 (FP64), or -1 if not computed  */
+  double seconds;           /* device time of the solve (HIP events)                 */
+  int32_t grid_blocks;      /* row chunks used                                       */
+  int32_t block_size;
+  double reserved[4];
+} fvc_stats;
+
+/* Fills *opt with the defaults listed above. */
+void fvc_default_options(fvc_options *opt);
+
+/* Library set-up on the current HIP device (implicit on first use). */
+int fvc_init(int32_t device);
+/* Human-readable text of the last error of this thread. */
+const char *fvc_last_error(void);
+
+/* ----------------------------------------------------------------------------
+ * The drop-in solve.  Replaces
+ *   subroutine solver(ad,au,al,x,b,iax,ja,neq,nad,unsym,tol,maxit,code,matriz,nlit)
+ * of src/Solver.f:28-29.  `unsym` is the LOGICAL unsym as 0/1; `omp_solver` is
+ * the caller's openmpSolver flag (include/openmp.fi:16-18), which selects the
+ * reference's x0 rule per path.  Differences from the reference, by design:
+ * b is preserved (the reference overwrites it, src/Iterativos.f:55,168);
+ * non-convergence returns FVC_MAXIT instead of stopping the program; GMRES
+ * also runs on ELLPACK and PCG on CSR/CSRC with a proper dot (the reference
+ * has no branch / a malformed call there, src/Solver.f:142-149,291-323).
+ * `nlit` is accepted for signature parity (it only selects the reference's
+ * console output, which the Fortran shim reproduces). */
+int fvc_solver(const double *ad, const double *au, const double *al, double *x,
+               double *b, const int32_t *iax, const int32_t *ja, int32_t neq,
+               int32_t nad, int32_t unsym, double tol, int32_t maxit, int32_t code,
+               int32_t matriz, int32_t nlit, int32_t omp_solver, fvc_stats *stats);
+
+/* The same with explicit options (host memory). */
+int fvc_solve(int32_t code, int32_t matriz, int32_t unsym, int32_t neq, int32_t nad,
+              const double *ad, const double *au, const double *al, double *x,
+              const double *b, const int32_t *iax, const int32_t *ja, double tol,
+              int32_t maxit, const fvc_options *opt, fvc_stats *stats);
+
+/* The same on device memory; runs on `stream`.  For ELLPACK, `iax` may be a
+ * host or device pointer (its first entry, the width, is read once). */
+int fvc_solve_dev(int32_t code, int32_t matriz, int32_t unsym, int32_t neq, int32_t nad,
+                  const double *ad, const double *au, const double *al, double *x,
+                  const double *b, const int32_t *iax, const int32_t *ja, double tol,
+                  int32_t maxit, const fvc_options *opt, fvc_stats *stats, void *stream);
+
+/* ----------------------------------------------------------------------------
+ * SpMV y = A x.  Replaces the matvec_* routines of src/Matvec.f:21-300 and
+ * src/Matvec_omp.f (matvec_csrd[_ilu2]_omp, matvec_csrd_sym_omp family,
+ * matvec_ellpack_omp).  Each row is summed in the reference's sequential
+ * order without FMA contraction, so results are bit-identical to the
+ * reference's sequential routines. */
+int fvc_matvec(int32_t matriz, int32_t unsym, int32_t neq, int32_t nad,
+               const double *ad, const double *au, const double *al,
+               const int32_t *iax, const int32_t *ja, const double *x, double *y);
+int fvc_matvec_dev(int32_t matriz, int32_t unsym, int32_t neq, int32_t nad,
+                   const double *ad, const double *au, const double *al,
+                   const int32_t *iax, const int32_t *ja, const double *x, double *y,
+                   void *stream);
+
+/* Inner product a.b.  Replaces dot / dot_ompL4 (src/Matvec.f:383-402,
+ * src/Matvec_omp.f:1247-1296).  Deterministic fixed-tree reduction: repeated
+ * calls give bit-identical results.  *result is written on the host. */
+int fvc_dot(const double *a, const double *b, int32_t n, double *result);
+int fvc_dot_dev(const double *a, const double *b, int32_t n, double *result, void *stream);
+
+/* getRes (src/Solver.f:368-391): flag != 0: x(num(i)) = u(i) (cells ->
+ * equations); flag == 0: u(i) = x(num(i)).  num holds 1-based equation ids. */
+int fvc_getres(double *u, double *x, const int32_t *num, int32_t neq, int32_t numel,
+               int32_t flag);
+int fvc_getres_dev(double *u, double *x, const int32_t *num, int32_t neq, int32_t numel,
+                   int32_t flag, void *stream);
+
+/* ----------------------------------------------------------------------------
+ * Device memory helpers, so that callers without a GPU framework can keep
+ * systems resident in HBM between solves. */
+int fvc_malloc(void **ptr, size_t bytes);
+int fvc_free(void *ptr);
+int fvc_memcpy_htod(void *dst, const void *src, size_t bytes);
+int fvc_memcpy_dtoh(void *dst, const void *src, size_t bytes);
+int fvc_memset_dev(void *dst, int value, size_t bytes);
+int fvc_synchronize(void);
+/* Number of compute units and the device name (name may be NULL). */
+int fvc_device_info(int32_t *compute_units, char *name, int32_t name_len);
+
+/* Timing helpers: record an event on `stream`, and elapsed ms between two. */
+int fvc_event_create(void **ev);
+int fvc_event_destroy(void *ev);
+int fvc_event_record(void *ev, void *stream);
+int fvc_event_elapsed_ms(void *start, void *stop, float *ms);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* FVCELL_H */
