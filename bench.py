@@ -1,0 +1,236 @@
+"""Benchmark: Krylov solves/s at 1M cells to relative residual 1e-10, and
+SpMV GB/s against the HBM peak, on one MI355X.
+
+Workload (one "step" = one solve, inputs resident in HBM): the nonsymmetric
+upwind advection-diffusion system on a 1024x1024 jittered quad mesh
+(1,048,576 cells, ELLPACK width 4, FP64), Jacobi-BiCGSTAB to
+||b - A x||_2 / ||b||_2 <= 1e-10.  The same line reports, for the same
+mesh size, Jacobi-PCG on the pinned pressure system and GMRES(30), the
+plain ELLPACK SpMV (warm and with the 256 MB last-level cache defeated by
+rotating buffers), the roofline of the dominant kernel, and the reference's
+own OpenMP Fortran solver (compiled from its sources, oracle/_ref) timed on
+the host cores as the CPU baseline.
+
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import re
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def spmv_bytes(neq: int, nnz_off: int, vbytes: int = 8) -> int:
+    """Algorithmic bytes of one SpMV: ad, x, y per row; value + int32 column
+    per stored off-diagonal (padding excluded)."""
+    return neq * 3 * vbytes + nnz_off * (vbytes + 4)
+
+
+def bicgstab_iter_bytes(neq: int, nnz_off: int) -> int:
+    """Per iteration: two SpMV plus 24 vector streams of 8 bytes per row."""
+    return 2 * spmv_bytes(neq, nnz_off) + 24 * 8 * neq
+
+
+def pcg_iter_bytes(neq: int, nnz_off: int) -> int:
+    """Per iteration: one SpMV plus 11 vector streams of 8 bytes per row."""
+    return spmv_bytes(neq, nnz_off) + 11 * 8 * neq
+
+
+def gmres_cycle_bytes(neq: int, nnz_off: int, steps: int) -> int:
+    """Arnoldi step j: one SpMV plus (5j + 6) vector streams (MGS)."""
+    return sum(spmv_bytes(neq, nnz_off) + (5 * j + 6) * 8 * neq for j in range(1, steps + 1))
+
+
+def read_traffic(name: str):
+    """HBM bytes per launch measured with rocprofv3 counters, if recorded
+    under profiles/ (line 'hbm_bytes_per_launch: N')."""
+    p = os.path.join(ROOT, "profiles", name)
+    if not os.path.exists(p):
+        return None
+    m = re.search(r"hbm_bytes_per_launch:\s*([0-9.eE+]+)", open(p).read())
+    return float(m.group(1)) if m else None
+
+
+def time_spmv(F, ds, neq, nnz, reps=50, rotate=1):
+    """Average SpMV kernel time (HIP events on the launch stream) over reps."""
+    import ctypes as C
+    L = F.load()
+    systems = [ds] + [F.DeviceSystem(ds._fmt) for _ in range(rotate - 1)]
+    xs = [F.DeviceArray.from_host(np.linspace(-1, 1, neq)) for _ in range(rotate)]
+    ys = [F.DeviceArray.empty(neq) for _ in range(rotate)]
+    for i in range(rotate):
+        systems[i].matvec(xs[i], ys[i])
+    e0, e1 = C.c_void_p(), C.c_void_p()
+    L.fvc_event_create(C.byref(e0))
+    L.fvc_event_create(C.byref(e1))
+    L.fvc_synchronize()
+    L.fvc_event_record(e0, None)
+    for r in range(reps):
+        i = r % rotate
+        systems[i].matvec(xs[i], ys[i])
+    L.fvc_event_record(e1, None)
+    ms = C.c_float()
+    L.fvc_event_elapsed_ms(e0, e1, C.byref(ms))
+    L.fvc_event_destroy(e0)
+    L.fvc_event_destroy(e1)
+    for s in systems[1:]:
+        del s
+    t = ms.value * 1e-3 / reps
+    return t, spmv_bytes(neq, nnz) / t / 1e9
+
+
+def cpu_baseline(fmt, method, tol, sample_solves=1):
+    """The reference Fortran (oracle/_ref) with OpenMP on the host cores: the
+    solver's own OpenMP routines (pbicgstab_omp + matvec_ellpack_omp +
+    dot_ompL4).  Falls back to the sequential C restatement."""
+    import oracle
+    threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or min(os.cpu_count() or 1, 16)
+    if oracle.have_ref():
+        r = oracle.ref()
+        oracle.ref_set_omp(1, threads)
+        x0 = np.zeros(fmt["neq"])
+        t0 = time.perf_counter()
+        for _ in range(sample_solves):
+            oracle.krylov(method, fmt, tol=tol, use_ref=True, omp=True, nthreads=threads, x0=x0, init=True)
+        dt = (time.perf_counter() - t0) / sample_solves
+        return {"value": 1.0 / dt, "unit": "solves/s", "cores": threads, "kind": "reference",
+                "sample": f"{sample_solves} full {method} solve(s) of the same system, reference "
+                          f"Fortran (pbicgstab_omp/matvec_ellpack_omp/dot_ompL4) built from its "
+                          f"sources, {threads} OpenMP threads, reference stopping rule tol={tol:g}",
+                "seconds_per_solve": dt}
+    t0 = time.perf_counter()
+    oracle.krylov(method, fmt, tol=tol)
+    dt = time.perf_counter() - t0
+    return {"value": 1.0 / dt, "unit": "solves/s", "cores": 1, "kind": "port",
+            "sample": f"1 full {method} solve, sequential C restatement", "seconds_per_solve": dt}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--n", type=int, default=1024, help="mesh side (n*n cells)")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-extra", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this build runs on one GPU only (asked for --gpus {args.gpus}); "
+              "the solver path is a single-GPU design, see the sharding notes", file=sys.stderr)
+        sys.exit(2)
+
+    import fortranfvcell_amd as F
+    from fortranfvcell_amd import problems as P
+
+    F.init(0)
+    info = F.device_info()
+    log("device", info)
+    n = args.n
+    t0 = time.perf_counter()
+    s = P.config_advdiff(n)
+    ell = P.to_ell(s)
+    log(f"generated advection-diffusion {n}x{n} in {time.perf_counter() - t0:.1f}s")
+    neq, nnz = s.neq, ell["nad"]
+    ds = F.DeviceSystem(ell)
+    ds._fmt = ell
+    tol = 1e-10
+    opts = dict(stop_rule=F.STOP_RELRES, x0_rule=F.X0_ZERO)
+
+    for _ in range(args.warmup):
+        st = ds.solve(F.BICGSTAB, tol=tol, **opts)
+    F.load().fvc_synchronize()
+    kern = []
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        st = ds.solve(F.BICGSTAB, tol=tol, **opts)
+        kern.append(st["seconds"])
+    F.load().fvc_synchronize()
+    wall = time.perf_counter() - t0
+    assert st["status"] == 0, st
+    x = ds.x.get()
+    relres = float(np.linalg.norm(s.b - P.scipy_csr(s) @ x) / np.linalg.norm(s.b))
+    its = st["iterations"]
+    solve_bytes = its * bicgstab_iter_bytes(neq, nnz) + 2 * spmv_bytes(neq, nnz)
+    kt = float(np.mean(kern))
+    achieved = solve_bytes / kt / 1e9
+    log(f"BiCGSTAB: {its} it, {kt * 1e3:.3f} ms/solve kernel, wall {wall / args.steps * 1e3:.3f} ms, relres {relres:.2e}")
+
+    result = {
+        "metric": "Krylov solves/s at 1M cells to 1e-10 resid; SpMV GB/s vs HBM peak",
+        "value": args.steps / wall,
+        "unit": "solves/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": wall / args.steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": f"advection-diffusion {n}x{n} jittered quads, ELLPACK w=4 FP64, "
+                               "Jacobi-BiCGSTAB to true relative residual 1e-10",
+                   "neq": neq, "nnz_offdiag": nnz, "method": "bicgstab", "format": "ellpack",
+                   "iterations": its, "true_relres": relres, "engine": st["engine"],
+                   "grid_blocks": st["grid_blocks"], "block_size": st["block_size"],
+                   "device": info["name"]},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS,
+                     "traffic": read_traffic("bicg_persistent_advdiff1024_traffic.txt"),
+                     "kernel": "fvc::bicg_persistent<0> (one launch per solve)",
+                     "algorithmic_bytes_per_launch": solve_bytes,
+                     "kernel_ms": kt * 1e3},
+    }
+
+    if not args.no_extra:
+        extra = {}
+        # plain SpMV, warm and cache-defeating rotation of 4 copies (> 256 MB)
+        t_w, gbs_w = time_spmv(F, ds, neq, nnz, reps=200, rotate=1)
+        t_f, gbs_f = time_spmv(F, ds, neq, nnz, reps=200, rotate=4)
+        extra["spmv_ell_fp64"] = {"us_warm": t_w * 1e6, "GBs_warm": gbs_w, "us_rotated": t_f * 1e6,
+                                  "GBs_rotated": gbs_f, "frac_rotated": gbs_f / HBM_PEAK_GBS,
+                                  "bytes": spmv_bytes(neq, nnz)}
+        log("spmv", extra["spmv_ell_fp64"])
+        # GMRES(30) on the same system
+        st_g = ds.solve(F.GMRES, tol=tol, gmres_k=30)
+        steps_total = st_g["inner_iterations"]
+        extra["gmres30"] = {"cycles": st_g["iterations"], "arnoldi_steps": steps_total,
+                            "ms_per_solve": st_g["seconds"] * 1e3, "solves_per_s": 1.0 / st_g["seconds"],
+                            "true_relres": st_g["true_relres"], "status": st_g["status"]}
+        log("gmres30", extra["gmres30"])
+        # Jacobi-PCG on the 1M-cell pinned pressure system
+        sp = P.config_poisson(n)
+        ellp = P.to_ell(sp)
+        dsp = F.DeviceSystem(ellp)
+        st_p = dsp.solve(F.PCG, tol=tol, **opts)
+        st_p = dsp.solve(F.PCG, tol=tol, **opts)
+        pb = st_p["iterations"] * pcg_iter_bytes(sp.neq, ellp["nad"]) + 2 * spmv_bytes(sp.neq, ellp["nad"])
+        extra["pcg_pressure"] = {"iterations": st_p["iterations"], "ms_per_solve": st_p["seconds"] * 1e3,
+                                 "solves_per_s": 1.0 / st_p["seconds"], "true_relres": st_p["true_relres"],
+                                 "GBs_algorithmic": pb / st_p["seconds"] / 1e9,
+                                 "us_per_iteration": st_p["seconds"] / st_p["iterations"] * 1e6}
+        log("pcg", extra["pcg_pressure"])
+        result["extra"] = extra
+
+    if not args.no_cpu:
+        result["cpu_baseline"] = cpu_baseline(ell, "bicgstab", tol)
+        log("cpu", result["cpu_baseline"])
+    print(json.dumps(result), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -59,9 +59,25 @@ def build_fortran() -> str | None:
     return out
 
 
+def build_fortran_driver() -> str | None:
+    """tests/fortran/drive_solver.exe: a Fortran caller of solver(...) linked
+    against the drop-in, used by the GPU tests."""
+    fc = shutil.which("amdflang") or (os.path.exists("/opt/rocm/bin/amdflang") and "/opt/rocm/bin/amdflang")
+    if not fc:
+        return None
+    tdir = os.path.join(ROOT, "tests", "fortran")
+    out = os.path.join(tdir, "drive_solver.exe")
+    objdir = os.path.join(HERE, "fortran", "_obj")
+    os.makedirs(objdir, exist_ok=True)
+    _run([fc, "-O2", "-module-dir", objdir, os.path.join(tdir, "drive_solver.f90"), "-o", out,
+          f"-L{HERE}", "-lfvcell_fortran", f"-Wl,-rpath,{HERE}", "-Wl,-rpath,$ORIGIN/../../fortranfvcell_amd"])
+    return out
+
+
 def build() -> None:
     build_hip()
     build_fortran()
+    build_fortran_driver()
 
 
 if __name__ == "__main__":

@@ -253,10 +253,14 @@ def stream_function(nodes: np.ndarray, seed: int, modes: int = 3) -> np.ndarray:
 
 def advdiff_system(mesh: Mesh, k: np.ndarray | None = None, peclet_max: float = 100.0,
                    seed: int = SEED_BASE, walls=("dirichlet",) * 4,
-                   wall_values=(0.0, 0.0, 0.0, 0.0), rhs: str = "random") -> System:
+                   wall_values=(0.0, 0.0, 0.0, 0.0), rhs: str = "random",
+                   dt: float | None = None, under_relax: float = 1.0) -> System:
     """First-order upwind advection-diffusion (momentum / energy form).  The
     velocity is u = curl psi of a seeded smooth stream function, scaled so the
-    largest cell Peclet number |F_f|/D_f equals peclet_max."""
+    largest cell Peclet number |F_f|/D_f equals peclet_max.  With ``dt`` the
+    transient mass term rho*area/dt joins the diagonal, and ``under_relax``
+    divides it as SIMPLE does (src/CellibSimple.f:347-355:
+    a_P = (sum a_nb + sum F + m) / underU)."""
     k = np.ones(mesh.neq) if k is None else k
     D, _ = _face_diffusion(mesh, k)
     psi = stream_function(mesh.nodes, seed)
@@ -277,11 +281,14 @@ def advdiff_system(mesh: Mesh, k: np.ndarray | None = None, peclet_max: float = 
             ap = np.where(on, k[:, None] * g["slen"] / g["dpf"], 0.0)
             ad += ap.sum(axis=1)
             b += ap.sum(axis=1) * wall_values[side]
+    if dt is not None:
+        ad = ad + g["area"] / dt
+    ad = ad / under_relax
     if rhs == "random":
         b = b + (2.0 * uniform(seed, 31, mesh.neq) - 1.0)
     return System(mesh.neq, mesh.w, ad, -anb, mesh.nbr.copy(), b, symmetric=False,
                   meta=dict(kind="advdiff", peclet_max=peclet_max, flux_scale=scale,
-                            face_flux=F))
+                            face_flux=F, dt=dt, under_relax=under_relax))
 
 
 # ------------------------------------------------------------------ layouts --
@@ -383,10 +390,14 @@ def config_poisson(n: int = 64, seed: int = SEED_BASE + 1, pin: str = "penalty")
     return pressure_system(quad_mesh(n), pin=pin, seed=seed)
 
 
-def config_advdiff(n: int = 1024, seed: int = SEED_BASE + 2, peclet_max: float = 100.0) -> System:
-    """Nonsymmetric upwind advection-diffusion on an n*n quad mesh with interior
-    nodes jittered by +-10% of h, Dirichlet walls, seeded RHS."""
-    return advdiff_system(quad_mesh(n, jitter=0.1, seed=seed), peclet_max=peclet_max, seed=seed)
+def config_advdiff(n: int = 1024, seed: int = SEED_BASE + 2, peclet_max: float = 100.0,
+                   dt: float | None = 0.005, under_relax: float = 0.8) -> System:
+    """Nonsymmetric upwind advection-diffusion in the momentum form of a SIMPLE
+    step on an n*n quad mesh with interior nodes jittered by +-10% of h:
+    Dirichlet walls, transient term area/dt and under-relaxation (the decks'
+    dt = 0.005, underU = 0.8), seeded RHS."""
+    return advdiff_system(quad_mesh(n, jitter=0.1, seed=seed), peclet_max=peclet_max, seed=seed,
+                          dt=dt, under_relax=under_relax)
 
 
 def config_pressure_lognormal(n: int = 1024, seed: int = SEED_BASE + 3, triangles: bool = False,
