@@ -20,6 +20,7 @@ CSRD, CSR, CSRC, ELLPACK = 1, 2, 3, 4
 X0_AS_OPENMP, X0_AS_SEQUENTIAL, X0_ZERO, X0_GIVEN = 0, 1, 2, 3
 STOP_REFERENCE, STOP_RELRES = 0, 1
 ENGINE_AUTO, ENGINE_KERNELS, ENGINE_PERSISTENT = 0, 1, 2
+PREC_FP64, PREC_MIXED = 0, 1
 
 # Every symbol include/fvcell.h declares.
 EXPORTS = [
@@ -34,7 +35,7 @@ EXPORTS = [
 class Options(C.Structure):
     _fields_ = [("x0_rule", C.c_int32), ("gmres_k", C.c_int32), ("stop_rule", C.c_int32),
                 ("true_residual", C.c_int32), ("engine", C.c_int32), ("block_size", C.c_int32),
-                ("grid_blocks", C.c_int32), ("reserved", C.c_int32 * 9)]
+                ("grid_blocks", C.c_int32), ("precision", C.c_int32), ("reserved", C.c_int32 * 8)]
 
 
 class Stats(C.Structure):

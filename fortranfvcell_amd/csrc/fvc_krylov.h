@@ -20,7 +20,7 @@ constexpr int S_OK = 0, S_MAXIT = 1, S_BREAKDOWN = 2, S_NONFINITE = 3;
 
 struct alignas(128) KState {
   double d, conv, alpha, beta, rr0, omega, energy, last;
-  double bnorm2, res2, econv, rnorm2, pad0, pad1, pad2, pad3;
+  double bnorm2, res2, econv, rnorm2, upd_est, d_prev, pad2, pad3;
   int iter, status, done, inner;      // inner: GMRES Arnoldi steps (nit)
   int step_i, mgs_j, inner_done, ni;  // GMRES position in the cycle
   int cycle, pad4, pad5, pad6;

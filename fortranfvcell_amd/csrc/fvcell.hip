@@ -14,6 +14,7 @@
 
 #include "../../include/fvcell.h"
 #include "fvc_krylov.h"
+#include "fvc_mixed.h"
 
 using namespace fvc;
 
@@ -192,7 +193,7 @@ struct SolveIn {
   double *x;
   const double *b;
   double tol;
-  int maxit, x0_zero, stop_rule, gmres_k, engine, bs, G_req;
+  int maxit, x0_zero, stop_rule, gmres_k, engine, bs, G_req, precision;
 };
 
 template <class KArgs>
@@ -211,6 +212,9 @@ int read_state(DevState &d, KState *st, hipStream_t s) {
   HIPCHK(hipStreamSynchronize(s));
   return FVC_OK;
 }
+
+void fill_stats(fvc_stats *stats, const KState &hs, int code, bool persistent, float ms, int G, int bs);
+int finish(const KState &hs, unsigned *ctr);
 
 template <int F>
 int run_solve(DevState &d, const SolveIn &in, fvc_stats *stats, hipStream_t s) {
@@ -368,24 +372,30 @@ int run_solve(DevState &d, const SolveIn &in, fvc_stats *stats, hipStream_t s) {
     }
   }
   HIPCHK(hipGetLastError());
-  const KState &hs = *d.host_st;
   float ms = 0.f;
   HIPCHK(hipEventElapsedTime(&ms, d.ev0, d.ev1));
-  if (stats) {
-    std::memset(stats, 0, sizeof *stats);
-    stats->status = hs.status;
-    stats->iterations = hs.iter;
-    stats->inner_iterations = in.code == FVC_GMRES ? hs.inner : hs.iter;
-    stats->engine = persistent ? FVC_ENGINE_PERSISTENT : FVC_ENGINE_KERNELS;
-    stats->energy = hs.energy;
-    stats->conv = in.code == FVC_GMRES ? hs.econv : hs.conv;
-    stats->last = hs.last;
-    stats->true_relres = hs.bnorm2 > 0 ? std::sqrt(hs.res2 / hs.bnorm2) : std::sqrt(hs.res2);
-    stats->seconds = ms * 1e-3;
-    stats->grid_blocks = G;
-    stats->block_size = bs;
-  }
+  fill_stats(stats, *d.host_st, in.code, persistent, ms, G, bs);
   // A bounded grid wait that expired is a device error, never a result.
+  return finish(*d.host_st, ctr);
+}
+
+void fill_stats(fvc_stats *stats, const KState &hs, int code, bool persistent, float ms, int G, int bs) {
+  if (!stats) return;
+  std::memset(stats, 0, sizeof *stats);
+  stats->status = hs.status;
+  stats->iterations = hs.iter;
+  stats->inner_iterations = code == FVC_GMRES ? hs.inner : hs.iter;
+  stats->engine = persistent ? FVC_ENGINE_PERSISTENT : FVC_ENGINE_KERNELS;
+  stats->energy = hs.energy;
+  stats->conv = code == FVC_GMRES ? hs.econv : hs.conv;
+  stats->last = hs.last;
+  stats->true_relres = hs.bnorm2 > 0 ? std::sqrt(hs.res2 / hs.bnorm2) : std::sqrt(hs.res2);
+  stats->seconds = ms * 1e-3;
+  stats->grid_blocks = G;
+  stats->block_size = bs;
+}
+
+int finish(const KState &hs, unsigned *ctr) {
   unsigned err = 0;
   HIPCHK(hipMemcpy(&err, ctr + kCtrErr, sizeof err, hipMemcpyDeviceToHost));
   if (err) return fail(FVC_DEVICE_ERROR, "persistent solve: a grid barrier wait expired");
@@ -396,7 +406,111 @@ int run_solve(DevState &d, const SolveIn &in, fvc_stats *stats, hipStream_t s) {
   return FVC_OK;
 }
 
+// Mixed-precision PCG (fvc_mixed.h): FP32 scaled matrix built per solve.
+template <int F>
+int run_mixed(DevState &d, const SolveIn &in, fvc_stats *stats, hipStream_t s) {
+  if constexpr (F != F_ELL4 && F != F_ELL3 && F != F_ELLW && F != F_CSRD) {
+    return fail(FVC_UNSUPPORTED, "mixed precision supports ELLPACK and unsymmetric CSRD storage");
+  } else {
+    const int neq = in.A.neq;
+    const int bs = in.bs > 0 ? in.bs : kDefaultBlock;
+    const void *pfn = (const void *)&mixed_persistent<F>;
+    const void *kfn = (const void *)&mixed_kernel<F>;
+    int occ = 0;
+    int rc = occupancy(pfn, bs, &occ);
+    if (rc) return rc;
+    const int G = choose_blocks(neq, d.cus, occ, in.G_req);
+    bool persistent = in.engine != FVC_ENGINE_KERNELS && G <= occ * d.cus;
+    const size_t n = (size_t)neq;
+    const size_t nval = (F == F_CSRD) ? (size_t)0 : (size_t)in.A.w * n;
+    size_t nv = nval;
+    if (F == F_CSRD) {
+      int last = 0;
+      HIPCHK(hipMemcpy(&last, in.A.ia + neq, sizeof(int), hipMemcpyDeviceToHost));
+      nv = (size_t)std::max(last - 1, 0);
+    }
+    size_t bytes = align_up(kCtrWords * sizeof(unsigned)) + align_up(sizeof(KState)) +
+                   align_up(kMaxK * (size_t)G * sizeof(double)) + align_up(nv * sizeof(float)) +
+                   align_up(n * sizeof(double)) + 6 * align_up(n * sizeof(float)) + 4096;
+    if ((rc = ensure_arena(d, bytes))) return rc;
+    Carve cv{d.arena};
+    unsigned *ctr = cv.take<unsigned>(kCtrWords);
+    KState *st = cv.take<KState>(1);
+    double *partials = cv.take<double>(kMaxK * (size_t)G);
+    float *av32 = cv.take<float>(std::max<size_t>(nv, 1));
+    double *sc = cv.take<double>(n);
+    float *dsq = cv.take<float>(n);
+    MixedArgs a{};
+    a.y = cv.take<float>(n);
+    a.r = cv.take<float>(n);
+    a.p0 = cv.take<float>(n);
+    a.p1 = cv.take<float>(n);
+    a.q = cv.take<float>(n);
+    a.A = in.A;
+    a.A32.neq = neq;
+    a.A32.w = in.A.w;
+    a.A32.av = av32;
+    a.A32.ia = in.A.ia;
+    a.A32.ja = in.A.ja;
+    a.g = make_grid(neq, G, ctr, partials, persistent);
+    a.st = st;
+    a.m = in.m;
+    a.s = sc;
+    a.dsq = dsq;
+    a.x = in.x;
+    a.b = in.b;
+    a.tol = in.tol;
+    a.delta = 0.1;
+    a.maxit = in.maxit;
+    a.x0_zero = in.x0_zero;
+    a.max_inner = 1000;
+    HIPCHK(hipEventRecord(d.ev0, s));
+    hipLaunchKernelGGL(reset_kernel, dim3(1), dim3(256), 0, s, ctr, (int)kCtrWords, st);
+    hipLaunchKernelGGL(scale_to_f32_kernel<F>, dim3((neq + 255) / 256), dim3(256), 0, s, in.A, in.m,
+                       av32, sc, dsq);
+    HIPCHK(hipGetLastError());
+    if (persistent) {
+      a.phase = -1;
+      if ((rc = launch(pfn, true, G, bs, a, s))) return rc;
+    } else {
+      a.phase = MX_SETUP;
+      if ((rc = launch(kfn, false, G, bs, a, s))) return rc;
+      int batch = 8, launched = 0;
+      for (;;) {
+        for (int it = 0; it < batch && launched < in.maxit; ++it, ++launched)
+          for (int ph : {MX_A, MX_B, MX_X, MX_R}) {
+            a.phase = ph;
+            if ((rc = launch(kfn, false, G, bs, a, s))) return rc;
+          }
+        if ((rc = read_state(d, st, s))) return rc;
+        if (d.host_st->done || launched >= in.maxit) break;
+        batch = std::min(batch * 2, 256);
+      }
+      a.phase = MX_FINAL;
+      if ((rc = launch(kfn, false, G, bs, a, s))) return rc;
+    }
+    HIPCHK(hipEventRecord(d.ev1, s));
+    if ((rc = read_state(d, st, s))) return rc;
+    float ms = 0.f;
+    HIPCHK(hipEventElapsedTime(&ms, d.ev0, d.ev1));
+    fill_stats(stats, *d.host_st, FVC_PCG, persistent, ms, G, bs);
+    if (stats) stats->inner_iterations = d.host_st->cycle;   // FP64 residual recomputations
+    return finish(*d.host_st, ctr);
+  }
+}
+
 int dispatch_solve(DevState &d, const SolveIn &in, fvc_stats *stats, hipStream_t s) {
+  if (in.precision == FVC_PREC_MIXED) {
+    if (in.code != FVC_PCG) return fail(FVC_UNSUPPORTED, "mixed precision is available for PCG");
+    switch (in.F) {
+      case F_ELL4: return run_mixed<F_ELL4>(d, in, stats, s);
+      case F_ELL3: return run_mixed<F_ELL3>(d, in, stats, s);
+      case F_ELLW: return run_mixed<F_ELLW>(d, in, stats, s);
+      case F_CSRD: return run_mixed<F_CSRD>(d, in, stats, s);
+    }
+    return fail(FVC_UNSUPPORTED, "mixed precision supports ELLPACK and unsymmetric CSRD storage");
+  }
+  if (in.precision != FVC_PREC_FP64) return fail(FVC_BAD_ARGUMENT, "precision %d", in.precision);
   switch (in.F) {
     case F_ELL4: return run_solve<F_ELL4>(d, in, stats, s);
     case F_ELL3: return run_solve<F_ELL3>(d, in, stats, s);
@@ -597,6 +711,7 @@ int solve_impl(int code, int matriz, int unsym, int neq, int nad, const double *
   in.engine = o.engine;
   in.bs = o.block_size;
   in.G_req = o.grid_blocks;
+  in.precision = o.precision;
   if (code == FVC_GMRES && o.stop_rule != FVC_STOP_REFERENCE)
     return fail(FVC_UNSUPPORTED, "GMRES supports only the reference stopping rule");
   rc = dispatch_solve(*d, in, stats, s);

@@ -69,6 +69,12 @@ extern "C" {
 #define FVC_ENGINE_KERNELS 1     /* one kernel per phase, reductions by the last block */
 #define FVC_ENGINE_PERSISTENT 2  /* one cooperative kernel per solve, grid barriers     */
 
+/* Arithmetic of the iteration. */
+#define FVC_PREC_FP64 0         /* everything in FP64 (the reference's real*8)              */
+#define FVC_PREC_MIXED 1        /* FP32 matrix + FP32 inner PCG on the Jacobi-scaled system,
+                                   FP64 x and FP64 residual recomputation ("reliable
+                                   updates"); stops on ||b - A x||_2 <= tol ||b||_2       */
+
 typedef struct fvc_options {
   int32_t x0_rule;        /* FVC_X0_*; default FVC_X0_AS_OPENMP                      */
   int32_t gmres_k;        /* restart length; 0 = 15 as src/Solver.f:63               */
@@ -77,7 +83,8 @@ typedef struct fvc_options {
   int32_t engine;         /* FVC_ENGINE_*                                            */
   int32_t block_size;     /* 0 = tuned default                                       */
   int32_t grid_blocks;    /* 0 = tuned default (number of row chunks)                */
-  int32_t reserved[9];
+  int32_t precision;      /* FVC_PREC_*; mixed applies to PCG on ELLPACK / CSRD      */
+  int32_t reserved[8];
 } fvc_options;
 
 typedef struct fvc_stats {

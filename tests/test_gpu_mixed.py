@@ -1,0 +1,59 @@
+"""Mixed precision: FP32 matrix and inner PCG on the Jacobi-scaled system,
+FP64 solution and FP64 residual recomputation.  It must reach the FP64
+true relative residual 1e-10, including on the reference's 1e60-pinned
+pressure systems (which overflow FP32 unless scaled first), and its error
+is stated against the FP64 solution."""
+import numpy as np
+import pytest
+import scipy.sparse.linalg as sla
+
+from fortranfvcell_amd import problems as P
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.mark.parametrize("name,build,lay", [
+    ("poisson64_ell", lambda: P.config_poisson(64), "ell"),
+    ("poisson64_csrd", lambda: P.config_poisson(64), "csrd"),
+    ("lognormal_quad48_ell", lambda: P.config_pressure_lognormal(48), "ell"),
+    ("lognormal_tri24_ell3", lambda: P.config_pressure_lognormal(24, triangles=True), "ell"),
+])
+def test_mixed_pcg_reaches_fp64_residual(fvc, name, build, lay):
+    s = build()
+    f = P.to_ell(s) if lay == "ell" else P.to_csrd(s)
+    print(f"mixed {name} neq={s.neq}", flush=True)
+    A = P.scipy_csr(s)
+    xd = sla.spsolve(A.tocsc(), s.b)
+    xm, sm = fvc.solve(f, fvc.PCG, tol=1e-10, precision=fvc.PREC_MIXED)
+    x64, s64 = fvc.solve(f, fvc.PCG, tol=1e-10, stop_rule=fvc.STOP_RELRES)
+    assert sm["status"] == 0, sm
+    r = np.linalg.norm(s.b - A @ xm) / np.linalg.norm(s.b)
+    assert r <= 1e-10, r
+    assert abs(sm["true_relres"] - r) <= 0.05 * r
+    e_m = np.linalg.norm(xm - xd) / np.linalg.norm(xd)
+    e_64 = np.linalg.norm(x64 - xd) / np.linalg.norm(xd)
+    print(f"  iterations mixed={sm['iterations']} (updates {sm['inner_iterations']}) fp64={s64['iterations']}"
+          f" err mixed={e_m:.2e} fp64={e_64:.2e} vs_fp64={np.linalg.norm(xm - x64) / np.linalg.norm(x64):.2e}",
+          flush=True)
+    # same residual level => error of the same order as the FP64 solve's
+    assert e_m <= 10 * e_64 + 1e-12
+    # reliable updates keep the FP32 recursion close to FP64 CG
+    assert sm["iterations"] <= 1.5 * s64["iterations"] + 20
+
+
+def test_mixed_engines_bit_identical(fvc):
+    s = P.config_pressure_lognormal(40)
+    f = P.to_ell(s)
+    xp, sp = fvc.solve(f, fvc.PCG, precision=fvc.PREC_MIXED, engine=fvc.ENGINE_PERSISTENT)
+    xk, sk = fvc.solve(f, fvc.PCG, precision=fvc.PREC_MIXED, engine=fvc.ENGINE_KERNELS)
+    assert sp["status"] == 0 and sk["status"] == 0
+    assert np.array_equal(xp, xk) and sp["iterations"] == sk["iterations"]
+
+
+def test_mixed_unsupported_combinations(fvc):
+    s = P.config_advdiff(16)
+    with pytest.raises(fvc.FvcError):
+        fvc.solve(P.to_ell(s), fvc.BICGSTAB, precision=fvc.PREC_MIXED, raise_on_failure=True)
+    sp = P.config_poisson(16)
+    with pytest.raises(fvc.FvcError):
+        fvc.solve(P.to_csrd(sp, sym=True), fvc.PCG, precision=fvc.PREC_MIXED, raise_on_failure=True)
