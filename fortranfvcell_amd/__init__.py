@@ -251,3 +251,62 @@ def dot_dev(a: DeviceArray, b: DeviceArray, n: int | None = None, stream=None) -
     n = a.shape[0] if n is None else n
     check(load().fvc_dot_dev(_ptr(a), _ptr(b), int(n), C.byref(r), stream))
     return r.value
+
+
+# ---------------------------------------------------------------- batches --
+
+def concat_ell(fmts: list) -> dict:
+    """Concatenate ELLPACK systems of one width (each in the reference layout,
+    local 1-based ja) into the batch layout of fvc_solve_batch."""
+    w = int(fmts[0]["iax"][0])
+    if any(int(f["iax"][0]) != w for f in fmts):
+        raise ValueError("all systems of a batch must share the ELLPACK width")
+    sizes = np.array([int(f["neq"]) for f in fmts], dtype=np.int64)
+    row_off = np.concatenate([[0], np.cumsum(sizes)]).astype(np.int32)
+    return dict(w=w, nsys=len(fmts), row_offsets=row_off,
+                ad=np.ascontiguousarray(np.concatenate([f["ad"] for f in fmts]), dtype=np.float64),
+                al=np.ascontiguousarray(np.concatenate([f["al"] for f in fmts]), dtype=np.float64),
+                ja=np.ascontiguousarray(np.concatenate([f["ja"] for f in fmts]), dtype=np.int32),
+                b=np.ascontiguousarray(np.concatenate([f["b"] for f in fmts]), dtype=np.float64))
+
+
+def _batch_stats(sts, n):
+    return [sts[i].as_dict() for i in range(n)]
+
+
+def solve_batch(batch: dict, tol: float = 1e-10, maxit: int = 50000, x0=None, options=None, **opt):
+    """Jacobi-PCG on every system of a batch (concat_ell) in one persistent
+    launch.  Returns (x concatenated, list of per-system stats)."""
+    n = int(batch["row_offsets"][-1])
+    x = np.zeros(n) if x0 is None else np.array(x0, dtype=np.float64, copy=True)
+    o = options if options is not None else default_options(**opt)
+    sts = (Stats * batch["nsys"])()
+    rc = load().fvc_solve_batch(PCG, int(batch["nsys"]), _ptr(batch["row_offsets"]), int(batch["w"]),
+                                _ptr(batch["ad"]), _ptr(batch["al"]), _ptr(x), _ptr(batch["b"]),
+                                _ptr(batch["ja"]), float(tol), int(maxit), C.byref(o), sts)
+    check(rc, _NONFATAL)
+    return x, _batch_stats(sts, batch["nsys"])
+
+
+class DeviceBatch:
+    """A batch of ELLPACK systems resident in HBM."""
+
+    def __init__(self, batch: dict):
+        self.nsys = int(batch["nsys"])
+        self.w = int(batch["w"])
+        self.n = int(batch["row_offsets"][-1])
+        self.row_offsets = DeviceArray.from_host(batch["row_offsets"])
+        self.ad = DeviceArray.from_host(batch["ad"])
+        self.al = DeviceArray.from_host(batch["al"])
+        self.ja = DeviceArray.from_host(batch["ja"])
+        self.b = DeviceArray.from_host(batch["b"])
+        self.x = DeviceArray.empty(self.n)
+
+    def solve(self, tol: float = 1e-10, maxit: int = 50000, options=None, stream=None, **opt):
+        o = options if options is not None else default_options(**opt)
+        sts = (Stats * self.nsys)()
+        rc = load().fvc_solve_batch_dev(PCG, self.nsys, _ptr(self.row_offsets), self.w, _ptr(self.ad),
+                                        _ptr(self.al), _ptr(self.x), _ptr(self.b), _ptr(self.ja),
+                                        float(tol), int(maxit), C.byref(o), sts, stream)
+        check(rc, _NONFATAL)
+        return _batch_stats(sts, self.nsys)

@@ -1,0 +1,166 @@
+// fvc_batch.h -- many independent systems in one persistent launch.
+//
+// The systems are concatenated: system s owns rows [row_off[s], row_off[s+1])
+// of the concatenated ELLPACK arrays, with its own 1-based local column
+// indices (each system in the reference's layout, back to back).  Every
+// system gets its own group of consecutive blocks (sized by the host from a
+// work estimate), its own barrier counters, reduction partials and solver
+// state, and runs the unchanged single-system phase functions on pointers
+// offset to its rows.  Groups never wait on each other, so a small system
+// finishes in a few hundred fast block-level iterations while large ones
+// keep their blocks busy; all blocks are co-resident (cooperative launch).
+#pragma once
+#include "fvc_mixed.h"
+
+namespace fvc {
+
+struct SysDesc {
+  int row_off;   // first row of the system in the concatenated arrays
+  int neq;
+  int g_first;   // first block of its group
+  int g_count;   // blocks in its group
+};
+
+struct BatchArgs {
+  int w;                    // ELLPACK width
+  const double *ad;         // concatenated ad (Jacobi diagonal as well)
+  const double *av;         // concatenated a(w, .)
+  const int *ja;            // concatenated ja(w, .), local 1-based indices
+  const SysDesc *desc;
+  const int *block_sys;     // system of each block
+  KState *st;               // per system
+  unsigned *ctr;            // kCtrWords per system
+  double *partials;         // kMaxK per block
+  double *x;
+  const double *b;
+  double *v0, *v1, *v2, *v3, *v4;   // FP64 work vectors (concatenated)
+  float *f0, *f1, *f2, *f3, *f4;    // FP32 work vectors (mixed)
+  float *av32;                      // FP32 scaled values (mixed)
+  double *s;                        // 1/sqrt(a_ii) (mixed)
+  float *dsq;                       // sqrt(a_ii) (mixed)
+  double tol;
+  int maxit, x0_zero, stop_rule, mixed;
+};
+
+__device__ __forceinline__ void batch_grid(const BatchArgs &B, int sys, const SysDesc &D, Grid &g) {
+  g.G = D.g_count;
+  g.R = (D.neq + D.g_count - 1) / D.g_count;
+  g.neq = D.neq;
+  g.ctr = B.ctr + (size_t)sys * kCtrWords;
+  g.partials = B.partials + (size_t)kMaxK * D.g_first;
+  g.persistent = true;
+  g.base = D.g_first;
+}
+
+__device__ __forceinline__ void batch_mat(const BatchArgs &B, const SysDesc &D, Mat &A) {
+  const size_t off = (size_t)D.row_off;
+  A.neq = D.neq;
+  A.w = B.w;
+  A.ad = B.ad + off;
+  A.av = B.av + off * B.w;
+  A.au = nullptr;
+  A.ia = nullptr;
+  A.ja = B.ja + off * B.w;
+  A.tptr = A.tsrc = A.tpos = nullptr;
+}
+
+template <int F>
+__global__ __launch_bounds__(1024) void pcg_batch_persistent(BatchArgs B) {
+  __shared__ double lds[kMaxK * kMaxWaves];
+  const int sys = B.block_sys[blockIdx.x];
+  const SysDesc D = B.desc[sys];
+  const size_t off = (size_t)D.row_off;
+  PcgArgs a{};
+  batch_mat(B, D, a.A);
+  batch_grid(B, sys, D, a.g);
+  a.st = B.st + sys;
+  a.m = a.A.ad;
+  a.x = B.x + off;
+  a.b = B.b + off;
+  a.r = B.v0 + off;
+  a.z = B.v1 + off;
+  a.p0 = B.v2 + off;
+  a.p1 = B.v3 + off;
+  a.q = B.v4 + off;
+  a.tol = B.tol;
+  a.maxit = B.maxit;
+  a.x0_zero = B.x0_zero;
+  a.stop_rule = B.stop_rule;
+  if (!pcg_phase<F>(a, PCG_SETUP, lds)) return;
+  while (!ld_int(&a.st->done)) {
+    if (!pcg_phase<F>(a, PCG_A, lds)) return;
+    if (ld_int(&a.st->done)) break;
+    if (!pcg_phase<F>(a, PCG_B, lds)) return;
+  }
+  pcg_phase<F>(a, PCG_FINAL, lds);
+}
+
+template <int F>
+__global__ __launch_bounds__(1024) void mixed_batch_persistent(BatchArgs B) {
+  __shared__ double lds[kMaxK * kMaxWaves];
+  const int sys = B.block_sys[blockIdx.x];
+  const SysDesc D = B.desc[sys];
+  const size_t off = (size_t)D.row_off;
+  MixedArgs a{};
+  batch_mat(B, D, a.A);
+  a.A32.neq = D.neq;
+  a.A32.w = B.w;
+  a.A32.av = B.av32 + off * B.w;
+  a.A32.ia = nullptr;
+  a.A32.ja = a.A.ja;
+  batch_grid(B, sys, D, a.g);
+  a.st = B.st + sys;
+  a.m = a.A.ad;
+  a.s = B.s + off;
+  a.dsq = B.dsq + off;
+  a.x = B.x + off;
+  a.b = B.b + off;
+  a.y = B.f0 + off;
+  a.r = B.f1 + off;
+  a.p0 = B.f2 + off;
+  a.p1 = B.f3 + off;
+  a.q = B.f4 + off;
+  a.tol = B.tol;
+  a.delta = 0.1;
+  a.maxit = B.maxit;
+  a.x0_zero = B.x0_zero;
+  a.max_inner = 1000;
+  if (!mixed_phase<F>(a, MX_SETUP, lds)) return;
+  while (!ld_int(&a.st->done)) {
+    if (!mixed_phase<F>(a, MX_A, lds)) return;
+    if (ld_int(&a.st->done)) break;
+    if (!mixed_phase<F>(a, MX_B, lds)) return;
+    if (ld_int(&a.st->step_i)) {
+      if (!mixed_phase<F>(a, MX_X, lds)) return;
+      if (!mixed_phase<F>(a, MX_R, lds)) return;
+    }
+  }
+  mixed_phase<F>(a, MX_FINAL, lds);
+}
+
+// S A S in FP32 for every system of the batch (rows are independent; the
+// column indices are local to each system).
+template <int F>
+__global__ void batch_scale_kernel(BatchArgs B, int nsys, const int *row_off) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  // system of row i: binary search in row_off[0..nsys]
+  int lo = 0, hi = nsys;
+  const int total = row_off[nsys];
+  if (i >= total) return;
+  while (hi - lo > 1) {
+    const int mid = (lo + hi) >> 1;
+    if (row_off[mid] <= i) lo = mid; else hi = mid;
+  }
+  const size_t off = (size_t)row_off[lo];
+  const double si = 1.0 / sqrt(B.ad[i]);
+  B.s[i] = si;
+  B.dsq[i] = (float)sqrt(B.ad[i]);
+  const int w = (F == F_ELL4) ? 4 : (F == F_ELL3) ? 3 : B.w;
+  for (int k = 0; k < w; ++k) {
+    const size_t e = (size_t)w * i + k;
+    const int j = B.ja[e] - 1;
+    B.av32[e] = (float)(B.av[e] * si * (1.0 / sqrt(B.ad[off + j])));
+  }
+}
+
+}  // namespace fvc

@@ -183,11 +183,13 @@ struct Grid {
   unsigned *ctr;
   double *partials;      // kMaxK * G
   bool persistent;
+  int base;              // first block of this grid in the launch (batches)
 
+  __device__ __forceinline__ int bid() const { return (int)blockIdx.x - base; }
   // XCD-aware chunk: consecutive block ids go to different XCDs, so give each
   // XCD a contiguous 1/8 of the rows (neighbour gathers stay in its L2).
   __device__ __forceinline__ int chunk() const {
-    const int b = blockIdx.x;
+    const int b = bid();
     return ((G & 7) == 0) ? (b & 7) * (G >> 3) + (b >> 3) : b;
   }
   __device__ __forceinline__ int row_begin() const { return chunk() * R; }
@@ -216,7 +218,7 @@ __device__ bool grid_reduce(const Grid &g, double *v, double *lds, Epi &&epi) {
     for (int k = 0; k < K; ++k) g.partials[(size_t)k * g.G + c] = v[k];
     const unsigned gen = g.persistent ? ld_relaxed(g.ctr + kCtrGen) : 0u;
     fence_release();
-    const int grp = blockIdx.x & 7;
+    const int grp = g.bid() & 7;
     const int ngroups = g.G < 8 ? g.G : 8;
     int last = 0;
     const unsigned t = add_acq_rel(g.ctr + grp * kCtrStride, 1u);

@@ -659,12 +659,12 @@ __global__ __launch_bounds__(1024) void dot_kernel(Grid g, const double *a, cons
 
 // Resets the barrier counters and the solver state (a kernel rather than a
 // memset, so that the sequence can be captured in a graph).
-__global__ void reset_kernel(unsigned *ctr, int nctr, KState *st) {
+__global__ void reset_kernel(unsigned *ctr, int nctr, KState *st, int nst = 1) {
   for (int i = threadIdx.x; i < nctr; i += blockDim.x) ctr[i] = 0u;
-  if (threadIdx.x == 0) {
+  for (int i = threadIdx.x; i < nst; i += blockDim.x) {
     KState z{};
     z.status = S_MAXIT;
-    *st = z;
+    st[i] = z;
   }
 }
 

@@ -15,6 +15,7 @@
 #include "../../include/fvcell.h"
 #include "fvc_krylov.h"
 #include "fvc_mixed.h"
+#include "fvc_batch.h"
 
 using namespace fvc;
 
@@ -174,6 +175,7 @@ Grid make_grid(int neq, int G, unsigned *ctr, double *partials, bool persistent)
   g.ctr = ctr;
   g.partials = partials;
   g.persistent = persistent;
+  g.base = 0;
   return g;
 }
 
@@ -798,6 +800,183 @@ int dot_impl(const double *a, const double *b, int n, double *result, bool on_de
   return FVC_OK;
 }
 
+// ------------------------------------------------------------- batches --
+
+// Block groups per system: g_s proportional to neq_s^exp (PCG work ~ rows x
+// iterations, iterations ~ sqrt(rows) on these meshes), at least one, at most
+// one per 64 rows, the total within the co-resident capacity.
+std::vector<int> allocate_blocks(const std::vector<int> &neq, int cap, double expo) {
+  const int n = (int)neq.size();
+  std::vector<double> w(n);
+  double tot = 0;
+  for (int i = 0; i < n; ++i) tot += (w[i] = std::pow((double)std::max(neq[i], 1), expo));
+  std::vector<int> g(n, 1);
+  int used = n;
+  const int spare = cap - n;
+  if (spare > 0) {
+    for (int i = 0; i < n; ++i) {
+      const int lim = std::max(1, (neq[i] + 63) / 64);
+      const int extra = (int)std::floor(spare * w[i] / tot);
+      const int add = std::min(extra, lim - 1);
+      g[i] += add;
+      used += add;
+    }
+  }
+  return g;
+}
+
+template <int F>
+int run_batch(DevState &d, int nsys, const std::vector<int> &row_off, int w, const double *ad,
+              const double *av, const int *ja, double *x, const double *b, const int *d_row_off,
+              double tol, int maxit, const fvc_options &o, fvc_stats *stats, hipStream_t s) {
+  const bool mixed = o.precision == FVC_PREC_MIXED;
+  const int bs = o.block_size > 0 ? o.block_size : kDefaultBlock;
+  if (bs % 64 != 0 || bs > 1024) return fail(FVC_BAD_ARGUMENT, "block size %d", bs);
+  const void *fn = mixed ? (const void *)&mixed_batch_persistent<F> : (const void *)&pcg_batch_persistent<F>;
+  int occ = 0;
+  int rc = occupancy(fn, bs, &occ);
+  if (rc) return rc;
+  const int cap = std::max(1, o.grid_blocks > 0 ? std::min(o.grid_blocks, occ * d.cus)
+                                               : d.cus * std::max(1, std::min(occ, 2)));
+  const int total = row_off[nsys];
+  // systems go in waves of at most cap/2 systems (each needs a block)
+  const int wave = std::max(1, std::min(nsys, cap / 2 > 0 ? cap / 2 : 1));
+  const double expo = o.batch_work_exp10 > 0 ? o.batch_work_exp10 / 10.0 : 1.5;
+  const size_t n = (size_t)total;
+  size_t bytes = align_up((size_t)wave * kCtrWords * sizeof(unsigned)) + align_up((size_t)wave * sizeof(KState)) +
+                 align_up((size_t)kMaxK * cap * sizeof(double)) + align_up((size_t)wave * sizeof(SysDesc)) +
+                 align_up((size_t)cap * sizeof(int)) + 4096;
+  if (mixed) bytes += 5 * align_up(n * sizeof(float)) + align_up(n * w * sizeof(float)) +
+                      align_up(n * sizeof(double)) + align_up(n * sizeof(float));
+  else bytes += 5 * align_up(n * sizeof(double));
+  if ((rc = ensure_arena(d, bytes))) return rc;
+  Carve cv{d.arena};
+  unsigned *ctr = cv.take<unsigned>((size_t)wave * kCtrWords);
+  KState *st = cv.take<KState>(wave);
+  double *partials = cv.take<double>((size_t)kMaxK * cap);
+  SysDesc *ddesc = cv.take<SysDesc>(wave);
+  int *dbsys = cv.take<int>(cap);
+  BatchArgs B{};
+  B.w = w;
+  B.ad = ad;
+  B.av = av;
+  B.ja = ja;
+  B.x = x;
+  B.b = b;
+  B.tol = tol;
+  B.maxit = maxit;
+  B.x0_zero = o.x0_rule == FVC_X0_GIVEN ? 0 : 1;
+  B.stop_rule = o.stop_rule;
+  B.mixed = mixed;
+  if (mixed) {
+    B.f0 = cv.take<float>(n); B.f1 = cv.take<float>(n); B.f2 = cv.take<float>(n);
+    B.f3 = cv.take<float>(n); B.f4 = cv.take<float>(n);
+    B.av32 = cv.take<float>(n * w);
+    B.s = cv.take<double>(n);
+    B.dsq = cv.take<float>(n);
+  } else {
+    B.v0 = cv.take<double>(n); B.v1 = cv.take<double>(n); B.v2 = cv.take<double>(n);
+    B.v3 = cv.take<double>(n); B.v4 = cv.take<double>(n);
+  }
+  B.ctr = ctr;
+  B.st = st;
+  B.partials = partials;
+  B.desc = ddesc;
+  B.block_sys = dbsys;
+  HIPCHK(hipEventRecord(d.ev0, s));
+  if (mixed) {
+    hipLaunchKernelGGL(batch_scale_kernel<F>, dim3((total + 255) / 256), dim3(256), 0, s, B, nsys, d_row_off);
+    HIPCHK(hipGetLastError());
+  }
+  std::vector<KState> host_st(nsys);
+  for (int s0 = 0; s0 < nsys; s0 += wave) {
+    const int m = std::min(wave, nsys - s0);
+    std::vector<int> sizes(m);
+    for (int i = 0; i < m; ++i) sizes[i] = row_off[s0 + i + 1] - row_off[s0 + i];
+    std::vector<int> g = allocate_blocks(sizes, cap, expo);
+    std::vector<SysDesc> desc(m);
+    std::vector<int> bsys;
+    int first = 0;
+    for (int i = 0; i < m; ++i) {
+      desc[i] = SysDesc{row_off[s0 + i], sizes[i], first, g[i]};
+      for (int k = 0; k < g[i]; ++k) bsys.push_back(i);
+      first += g[i];
+    }
+    const int G = first;
+    HIPCHK(hipMemcpyAsync(ddesc, desc.data(), m * sizeof(SysDesc), hipMemcpyHostToDevice, s));
+    HIPCHK(hipMemcpyAsync(dbsys, bsys.data(), G * sizeof(int), hipMemcpyHostToDevice, s));
+    hipLaunchKernelGGL(reset_kernel, dim3(1), dim3(256), 0, s, ctr, m * kCtrWords, st, m);
+    HIPCHK(hipGetLastError());
+    if ((rc = launch(fn, true, G, bs, B, s))) return rc;
+    HIPCHK(hipMemcpyAsync(host_st.data() + s0, st, m * sizeof(KState), hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));   // desc/bsys host buffers are reused next wave
+    unsigned err = 0;
+    for (int i = 0; i < m; ++i) {
+      HIPCHK(hipMemcpy(&err, ctr + (size_t)i * kCtrWords + kCtrErr, sizeof err, hipMemcpyDeviceToHost));
+      if (err) return fail(FVC_DEVICE_ERROR, "batched solve: a grid barrier wait expired");
+    }
+  }
+  HIPCHK(hipEventRecord(d.ev1, s));
+  HIPCHK(hipEventSynchronize(d.ev1));
+  float ms = 0.f;
+  HIPCHK(hipEventElapsedTime(&ms, d.ev0, d.ev1));
+  int worst = S_OK;
+  for (int i = 0; i < nsys; ++i) {
+    if (stats) {
+      fill_stats(&stats[i], host_st[i], FVC_PCG, true, ms, 0, bs);
+      if (mixed) stats[i].inner_iterations = host_st[i].cycle;
+    }
+    if (host_st[i].status != S_OK && worst == S_OK) worst = host_st[i].status;
+  }
+  if (worst != S_OK) return fail(worst, "a system of the batch did not converge");
+  return FVC_OK;
+}
+
+int batch_impl(int code, int nsys, const int32_t *row_offsets, int w, const double *ad,
+               const double *al, double *x, const double *b, const int32_t *ja, double tol,
+               int maxit, const fvc_options *opt, fvc_stats *stats, bool on_device, hipStream_t s) {
+  if (code != FVC_PCG) return fail(FVC_UNSUPPORTED, "batched solves run Jacobi-PCG (code 1)");
+  if (nsys <= 0 || !row_offsets || !ad || !al || !x || !b || !ja) return fail(FVC_BAD_ARGUMENT, "bad batch arguments");
+  if (w < 1 || w > 64) return fail(FVC_BAD_ARGUMENT, "ELLPACK width %d", w);
+  fvc_options o = opt ? *opt : defaults();
+  DevState *d = nullptr;
+  int rc = current_device(&d);
+  if (rc) return rc;
+  std::lock_guard<std::mutex> lk(d->mu);
+  std::vector<int> row_off(nsys + 1);
+  if (on_device) HIPCHK(hipMemcpy(row_off.data(), row_offsets, (nsys + 1) * sizeof(int), hipMemcpyDeviceToHost));
+  else std::memcpy(row_off.data(), row_offsets, (nsys + 1) * sizeof(int));
+  if (row_off[0] != 0) return fail(FVC_BAD_ARGUMENT, "row_offsets[0] must be 0");
+  for (int i = 0; i < nsys; ++i)
+    if (row_off[i + 1] <= row_off[i]) return fail(FVC_BAD_ARGUMENT, "system %d is empty", i);
+  const size_t n = (size_t)row_off[nsys];
+  Owned own;
+  const double *dad = ad, *dav = al, *db = b;
+  const int *dja = ja, *droff = row_offsets;
+  double *dx = x;
+  if (!on_device) {
+    double *t1, *t2, *t3, *t4;
+    int *i1, *i2;
+    if ((rc = own.up(&t1, ad, n)) || (rc = own.up(&t2, al, n * w)) || (rc = own.up(&t3, b, n)) ||
+        (rc = own.up(&t4, x, n)) || (rc = own.up(&i1, ja, n * w)) ||
+        (rc = own.up(&i2, (const int *)row_off.data(), (size_t)nsys + 1)))
+      return rc;
+    dad = t1; dav = t2; db = t3; dx = t4; dja = i1; droff = i2;
+  } else if (o.precision == FVC_PREC_MIXED) {
+    int *i2;
+    if ((rc = own.up(&i2, (const int *)row_off.data(), (size_t)nsys + 1))) return rc;
+    droff = i2;
+  }
+  switch (w) {
+    case 4: rc = run_batch<F_ELL4>(*d, nsys, row_off, w, dad, dav, dja, dx, db, droff, tol, maxit, o, stats, s); break;
+    case 3: rc = run_batch<F_ELL3>(*d, nsys, row_off, w, dad, dav, dja, dx, db, droff, tol, maxit, o, stats, s); break;
+    default: rc = run_batch<F_ELLW>(*d, nsys, row_off, w, dad, dav, dja, dx, db, droff, tol, maxit, o, stats, s); break;
+  }
+  if (!on_device && (rc == FVC_OK || rc == FVC_MAXIT || rc == FVC_BREAKDOWN || rc == FVC_NONFINITE))
+    HIPCHK(hipMemcpy(x, dx, n * sizeof(double), hipMemcpyDeviceToHost));
+  return rc;
+}
+
 }  // namespace
 
 // =================================================================== C ABI ==
@@ -845,6 +1024,21 @@ int fvc_solve_dev(int32_t code, int32_t matriz, int32_t unsym, int32_t neq, int3
                   int32_t maxit, const fvc_options *opt, fvc_stats *stats, void *stream) {
   return solve_impl(code, matriz, unsym, neq, nad, ad, au, al, x, b, iax, ja, tol, maxit, opt,
                     stats, true, (hipStream_t)stream);
+}
+
+int fvc_solve_batch(int32_t code, int32_t nsys, const int32_t *row_offsets, int32_t w,
+                    const double *ad, const double *al, double *x, const double *b,
+                    const int32_t *ja, double tol, int32_t maxit, const fvc_options *opt,
+                    fvc_stats *stats) {
+  return batch_impl(code, nsys, row_offsets, w, ad, al, x, b, ja, tol, maxit, opt, stats, false, nullptr);
+}
+
+int fvc_solve_batch_dev(int32_t code, int32_t nsys, const int32_t *row_offsets, int32_t w,
+                        const double *ad, const double *al, double *x, const double *b,
+                        const int32_t *ja, double tol, int32_t maxit, const fvc_options *opt,
+                        fvc_stats *stats, void *stream) {
+  return batch_impl(code, nsys, row_offsets, w, ad, al, x, b, ja, tol, maxit, opt, stats, true,
+                    (hipStream_t)stream);
 }
 
 int fvc_matvec(int32_t matriz, int32_t unsym, int32_t neq, int32_t nad, const double *ad,

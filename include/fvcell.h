@@ -84,7 +84,8 @@ typedef struct fvc_options {
   int32_t block_size;     /* 0 = tuned default                                       */
   int32_t grid_blocks;    /* 0 = tuned default (number of row chunks)                */
   int32_t precision;      /* FVC_PREC_*; mixed applies to PCG on ELLPACK / CSRD      */
-  int32_t reserved[8];
+  int32_t batch_work_exp10; /* batches: blocks per system ~ neq^(this/10); 0 = 15     */
+  int32_t reserved[7];
 } fvc_options;
 
 typedef struct fvc_stats {
@@ -141,6 +142,24 @@ int fvc_solve_dev(int32_t code, int32_t matriz, int32_t unsym, int32_t neq, int3
                   const double *ad, const double *au, const double *al, double *x,
                   const double *b, const int32_t *iax, const int32_t *ja, double tol,
                   int32_t maxit, const fvc_options *opt, fvc_stats *stats, void *stream);
+
+/* ----------------------------------------------------------------------------
+ * Batched Jacobi-PCG: nsys independent ELLPACK systems of width w, each in
+ * the reference's layout (local 1-based ja), concatenated back to back:
+ * system s owns rows row_offsets[s] .. row_offsets[s+1]-1 (row_offsets[0] = 0)
+ * of ad, b, x and entries w*row_offsets[s] .. of al and ja.  All systems are
+ * solved in one persistent launch, each by its own block group with its own
+ * scalars and stopping test; stats receives nsys entries.  x0 is zero unless
+ * opt->x0_rule == FVC_X0_GIVEN.  opt->precision selects FP64 or mixed.  No
+ * counterpart in the reference (its solver takes one system per call). */
+int fvc_solve_batch(int32_t code, int32_t nsys, const int32_t *row_offsets, int32_t w,
+                    const double *ad, const double *al, double *x, const double *b,
+                    const int32_t *ja, double tol, int32_t maxit, const fvc_options *opt,
+                    fvc_stats *stats);
+int fvc_solve_batch_dev(int32_t code, int32_t nsys, const int32_t *row_offsets, int32_t w,
+                        const double *ad, const double *al, double *x, const double *b,
+                        const int32_t *ja, double tol, int32_t maxit, const fvc_options *opt,
+                        fvc_stats *stats, void *stream);
 
 /* ----------------------------------------------------------------------------
  * SpMV y = A x.  Replaces the matvec_* routines of src/Matvec.f:21-300 and

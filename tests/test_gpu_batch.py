@@ -1,0 +1,65 @@
+"""Batched Jacobi-PCG: many uneven pressure systems in one persistent launch,
+each converging to its own tolerance, in FP64 and mixed precision; each
+system's solution matches the single-system solve and the oracle."""
+import numpy as np
+import pytest
+
+from fortranfvcell_amd import problems as P
+
+pytestmark = pytest.mark.gpu
+
+
+def _systems(count, lo, hi, seed):
+    sides = P.config_batch_sides(count, lo, hi, seed)
+    out = []
+    for i, n in enumerate(sides):
+        sd = seed * 1000 + i
+        mesh = P.quad_mesh(int(n))
+        out.append(P.pressure_system(mesh, k=P.lognormal_k(sd, mesh.neq), seed=sd))
+    return out
+
+
+@pytest.mark.parametrize("precision", [0, 1])
+def test_batch_matches_single_solves(fvc, orc, precision):
+    systems = _systems(24, 8, 48, 7)
+    fmts = [P.to_ell(s) for s in systems]
+    batch = fvc.concat_ell(fmts)
+    print(f"batch of {len(fmts)} systems, {batch['row_offsets'][-1]} rows, precision={precision}", flush=True)
+    x, sts = fvc.solve_batch(batch, tol=1e-10, precision=precision, stop_rule=fvc.STOP_RELRES)
+    ro = batch["row_offsets"]
+    for i, (s, f) in enumerate(zip(systems, fmts)):
+        assert sts[i]["status"] == 0, (i, sts[i])
+        xi = x[ro[i]:ro[i + 1]]
+        r = np.linalg.norm(s.b - P.scipy_csr(s) @ xi) / np.linalg.norm(s.b)
+        assert r <= 1e-10 * 1.01, (i, r)
+        xs, ss = fvc.solve(f, fvc.PCG, tol=1e-10, precision=precision, stop_rule=fvc.STOP_RELRES,
+                           x0_rule=fvc.X0_ZERO)
+        if precision == 0:
+            # the same phases on the same rows with a smaller block group:
+            # same algorithm, different reduction tree
+            assert np.linalg.norm(xi - xs) <= 1e-8 * np.linalg.norm(xs)
+        xo, _, so = orc.krylov("pcg", f, tol=1e-12)
+        assert np.linalg.norm(xi - xo) <= 1e-6 * np.linalg.norm(xo)
+
+
+def test_batch_uneven_sizes_and_waves(fvc):
+    # more systems than half the co-resident blocks: solved in waves
+    systems = _systems(300, 4, 12, 11)
+    batch = fvc.concat_ell([P.to_ell(s) for s in systems])
+    x, sts = fvc.solve_batch(batch, tol=1e-10, stop_rule=fvc.STOP_RELRES)
+    assert all(st["status"] == 0 for st in sts)
+    ro = batch["row_offsets"]
+    for i in (0, 150, 299):
+        s = systems[i]
+        xi = x[ro[i]:ro[i + 1]]
+        assert np.linalg.norm(s.b - P.scipy_csr(s) @ xi) <= 1.01e-10 * np.linalg.norm(s.b)
+
+
+def test_batch_device_resident(fvc):
+    systems = _systems(16, 16, 64, 5)
+    batch = fvc.concat_ell([P.to_ell(s) for s in systems])
+    db = fvc.DeviceBatch(batch)
+    sts = db.solve(tol=1e-10, stop_rule=fvc.STOP_RELRES)
+    xh, sth = fvc.solve_batch(batch, tol=1e-10, stop_rule=fvc.STOP_RELRES)
+    assert np.array_equal(db.x.get(), xh)
+    assert [s["iterations"] for s in sts] == [s["iterations"] for s in sth]
