@@ -25,7 +25,7 @@ PREC_FP64, PREC_MIXED = 0, 1
 # Every symbol include/fvcell.h declares.
 EXPORTS = [
     "fvc_default_options", "fvc_init", "fvc_last_error", "fvc_solver", "fvc_solve", "fvc_solve_dev",
-    "fvc_solve_batch", "fvc_solve_batch_dev",
+    "fvc_solve_batch", "fvc_solve_batch_dev", "fvc_ell_transpose_dev", "fvc_grad_k_dev",
     "fvc_matvec", "fvc_matvec_dev", "fvc_dot", "fvc_dot_dev", "fvc_getres", "fvc_getres_dev",
     "fvc_malloc", "fvc_free", "fvc_memcpy_htod", "fvc_memcpy_dtoh", "fvc_memset_dev",
     "fvc_synchronize", "fvc_device_info", "fvc_event_create", "fvc_event_destroy",
@@ -79,6 +79,8 @@ def load(path: str = LIB_PATH):
                                 C.POINTER(Stats), P]
     L.fvc_solve_batch.argtypes = [I, I, P, I, P, P, P, P, P, D, I, C.POINTER(Options), P]
     L.fvc_solve_batch_dev.argtypes = [I, I, P, I, P, P, P, P, P, D, I, C.POINTER(Options), P, P]
+    L.fvc_ell_transpose_dev.argtypes = [I, I, P, P, P, P, P]
+    L.fvc_grad_k_dev.argtypes = [I, I, P, P, P, P, P, P, P, P, P, P]
     L.fvc_matvec.argtypes = [I, I, I, I, P, P, P, P, P, P, P]
     L.fvc_matvec_dev.argtypes = [I, I, I, I, P, P, P, P, P, P, P, P]
     L.fvc_dot.argtypes = [P, P, I, C.POINTER(D)]

@@ -1,0 +1,60 @@
+// fvc_adjoint.h -- kernels of the differentiable solve.
+//
+// The adjoint of A T = b is A^T lam = dL/dT.  For an ELLPACK matrix whose
+// pattern is symmetric (finite-volume adjacency), A^T has the same ja; only
+// the values move: A^T(i, s) = A(j, t) with j = ja(s,i) and t the slot of row
+// j that points back at i.  The adjoint solve is then the ordinary
+// BiCGSTAB on (ad, a^T, ja).
+//
+// The gradient of L with respect to the per-cell conductivity k follows from
+// dL/dk = -lam^T (dA/dk T - db/dk) and the reference's coefficient forms:
+//   interior face (c, N): D_f = k_f * geo, 1/k_f = a/k_c + (1-a)/k_N
+//     (src/CellibTrans.f:510-511,527); D_f enters row c as +D_f T_c - D_f T_N
+//     and row N symmetrically, so its term is -(lam_c - lam_N)(T_c - T_N)
+//     * geo * dk_f/dk_c with dk_f/dk_c = k_f^2 a / k_c^2;
+//   Dirichlet face: a_P += k_c |S|/d and b += k_c |S|/d T_wall
+//     (src/CellibTrans.f:488-491): term -lam_c (T_c - T_wall) |S|/d.
+// One lane per cell sums its own faces; no atomics.
+#pragma once
+#include <hip/hip_runtime.h>
+
+namespace fvc {
+
+__global__ void ell_transpose_kernel(int neq, int w, const int *__restrict__ ja,
+                                     const int *__restrict__ back, const double *__restrict__ a,
+                                     double *__restrict__ at) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= neq) return;
+  for (int s = 0; s < w; ++s) {
+    const size_t e = (size_t)w * i + s;
+    const int t = back[e];
+    at[e] = (t >= 0) ? a[(size_t)w * (ja[e] - 1) + t] : 0.0;
+  }
+}
+
+__global__ void grad_k_kernel(int neq, int w, const int *__restrict__ ja, const double *__restrict__ k,
+                              const double *__restrict__ T, const double *__restrict__ lam,
+                              const double *__restrict__ geo, const double *__restrict__ alpha,
+                              const double *__restrict__ wall_g, const double *__restrict__ wall_T,
+                              double *__restrict__ grad) {
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= neq) return;
+  const double kc = k[c], Tc = T[c], lc = lam[c];
+  double acc = 0.0;
+  for (int s = 0; s < w; ++s) {
+    const size_t e = (size_t)w * c + s;
+    const double gf = geo[e];
+    if (gf != 0.0) {
+      const int j = ja[e] - 1;
+      const double a = alpha[e];
+      const double kf = 1.0 / (a / kc + (1.0 - a) / k[j]);
+      const double dkf = kf * kf * a / (kc * kc);
+      acc += -(lc - lam[j]) * (Tc - T[j]) * gf * dkf;
+    }
+    const double wg = wall_g[e];
+    if (wg != 0.0) acc += -lc * (Tc - wall_T[e]) * wg;
+  }
+  grad[c] = acc;
+}
+
+}  // namespace fvc

@@ -16,6 +16,7 @@
 #include "fvc_krylov.h"
 #include "fvc_mixed.h"
 #include "fvc_batch.h"
+#include "fvc_adjoint.h"
 
 using namespace fvc;
 
@@ -1039,6 +1040,26 @@ int fvc_solve_batch_dev(int32_t code, int32_t nsys, const int32_t *row_offsets, 
                         fvc_stats *stats, void *stream) {
   return batch_impl(code, nsys, row_offsets, w, ad, al, x, b, ja, tol, maxit, opt, stats, true,
                     (hipStream_t)stream);
+}
+
+int fvc_ell_transpose_dev(int32_t neq, int32_t w, const int32_t *ja, const int32_t *back,
+                          const double *al, double *alt, void *stream) {
+  if (neq <= 0 || w < 1 || !ja || !back || !al || !alt) return fail(FVC_BAD_ARGUMENT, "bad transpose arguments");
+  hipLaunchKernelGGL(ell_transpose_kernel, dim3((neq + 255) / 256), dim3(256), 0, (hipStream_t)stream,
+                     neq, w, ja, back, al, alt);
+  HIPCHK(hipGetLastError());
+  return FVC_OK;
+}
+
+int fvc_grad_k_dev(int32_t neq, int32_t w, const int32_t *ja, const double *k, const double *t,
+                   const double *lam, const double *geo, const double *alpha, const double *wall_g,
+                   const double *wall_t, double *grad_k, void *stream) {
+  if (neq <= 0 || w < 1 || !ja || !k || !t || !lam || !geo || !alpha || !wall_g || !wall_t || !grad_k)
+    return fail(FVC_BAD_ARGUMENT, "bad gradient arguments");
+  hipLaunchKernelGGL(grad_k_kernel, dim3((neq + 255) / 256), dim3(256), 0, (hipStream_t)stream, neq, w,
+                     ja, k, t, lam, geo, alpha, wall_g, wall_t, grad_k);
+  HIPCHK(hipGetLastError());
+  return FVC_OK;
 }
 
 int fvc_matvec(int32_t matriz, int32_t unsym, int32_t neq, int32_t nad, const double *ad,

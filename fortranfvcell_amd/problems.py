@@ -425,13 +425,115 @@ def config_batch(count: int = 256, lo: int = 32, hi: int = 512, seed: int = SEED
     return systems
 
 
-def config_heat(n: int = 512, seed: int = SEED_BASE + 5, peclet_max: float = 100.0):
+HEAT_WALLS = ("neumann", "dirichlet", "neumann", "dirichlet")
+HEAT_WALL_T = (0.0, 0.0, 0.0, 1.0)
+
+
+def config_heat(n: int = 512, seed: int = SEED_BASE + 5, peclet_max: float = 10.0):
     """Steady advection-diffusion heat problem: n*n jittered quads, lognormal k,
-    hot (T=1) west wall, cold (T=0) east wall, adiabatic south/north walls."""
+    hot (T=1) west wall, cold (T=0) east wall, adiabatic south/north walls,
+    the stream-function velocity family of config_advdiff scaled to a largest
+    cell Peclet number of 10 (at 100 the steady Jacobi-BiCGSTAB solve needs
+    ~1e5 iterations at 256^2)."""
     mesh = quad_mesh(n, jitter=0.1, seed=seed)
     k = lognormal_k(seed, mesh.neq)
-    s = advdiff_system(mesh, k=k, peclet_max=peclet_max, seed=seed,
-                       walls=("neumann", "dirichlet", "neumann", "dirichlet"),
-                       wall_values=(0.0, 0.0, 0.0, 1.0), rhs="none")
+    s = advdiff_system(mesh, k=k, peclet_max=peclet_max, seed=seed, walls=HEAT_WALLS,
+                       wall_values=HEAT_WALL_T, rhs="none")
     s.meta.update(kind="heat", k=k, mesh=mesh)
     return s
+
+
+@dataclass
+class HeatOperator:
+    """The heat problem as a function of the per-cell conductivity k:
+    A(k) T = b(k) + q.  Only the diffusion coefficients and the Dirichlet wall
+    coefficients depend on k; the upwind convection part is fixed.
+
+    Per (cell c, slot s):
+      geo[c,s]    = |S| / ((n . xi^) |xi|)   (D_f = k_f * geo)
+      alpha[c,s]  = c's share a of the normal distance (1/k_f = a/k_c + (1-a)/k_N)
+      wall_g[c,s] = |S| / d_cf on Dirichlet faces, else 0 (a_P += k_c wall_g)
+      wall_T[c,s] = the wall temperature there
+      conv[c,s]   = max(-F_f, 0), convF[c] = sum_f F_f (fixed)
+    """
+    mesh: Mesh
+    geo: np.ndarray
+    alpha: np.ndarray
+    wall_g: np.ndarray
+    wall_T: np.ndarray
+    conv: np.ndarray
+    convF: np.ndarray
+
+    @classmethod
+    def build(cls, mesh: Mesh, k_for_flux_scale: np.ndarray, peclet_max: float, seed: int,
+              walls=HEAT_WALLS, wall_values=HEAT_WALL_T) -> "HeatOperator":
+        g = mesh.geometry()
+        D, aux = _face_diffusion(mesh, k_for_flux_scale)
+        inner = mesh.nbr >= 0
+        geo = np.where(inner, g["slen"] / (aux["nk"] * aux["mxi"]), 0.0)
+        psi = stream_function(mesh.nodes, seed)
+        cn = mesh.cell_nodes
+        F = np.where(inner, psi[np.roll(cn, -1, axis=1)] - psi[cn], 0.0)
+        ratio = np.abs(F) / np.where(inner, D, np.inf)
+        scale = peclet_max / ratio.max() if ratio.max() > 0 else 0.0
+        F = F * scale
+        wall_g = np.zeros_like(geo)
+        wall_T = np.zeros_like(geo)
+        for side in range(4):
+            on = mesh.boundary_tag == side
+            if walls[side] == "dirichlet":
+                wall_g = np.where(on, g["slen"] / g["dpf"], wall_g)
+                wall_T = np.where(on, wall_values[side], wall_T)
+        return cls(mesh, geo, aux["alpha"], wall_g, wall_T, np.where(inner, np.maximum(-F, 0.0), 0.0),
+                   F.sum(axis=1))
+
+    def face_k(self, k: np.ndarray) -> np.ndarray:
+        nb = np.where(self.mesh.nbr >= 0, self.mesh.nbr, 0)
+        a = self.alpha
+        return 1.0 / (a / k[:, None] + (1.0 - a) / k[nb])
+
+    def assemble(self, k: np.ndarray, q: np.ndarray | None = None) -> System:
+        inner = self.mesh.nbr >= 0
+        D = np.where(inner, self.face_k(k) * self.geo, 0.0)
+        anb = np.where(inner, D + self.conv, 0.0)
+        ap_wall = k[:, None] * self.wall_g
+        ad = anb.sum(axis=1) + self.convF + ap_wall.sum(axis=1)
+        b = (ap_wall * self.wall_T).sum(axis=1)
+        if q is not None:
+            b = b + q
+        return System(self.mesh.neq, self.mesh.w, ad, -anb, self.mesh.nbr.copy(), b,
+                      meta=dict(kind="heat"))
+
+    def grad_k(self, k: np.ndarray, T: np.ndarray, lam: np.ndarray) -> np.ndarray:
+        """dL/dk = -lam^T (dA/dk T - db/dk) for the adjoint lam (A^T lam = dL/dT)."""
+        inner = self.mesh.nbr >= 0
+        nb = np.where(inner, self.mesh.nbr, 0)
+        kf = self.face_k(k)
+        dkf = kf * kf * self.alpha / (k[:, None] ** 2)
+        face = -(lam[:, None] - lam[nb]) * (T[:, None] - T[nb]) * self.geo * dkf
+        wall = -lam[:, None] * (T[:, None] - self.wall_T) * self.wall_g
+        return np.where(inner, face, 0.0).sum(axis=1) + wall.sum(axis=1)
+
+
+def ell_back_slots(nbr: np.ndarray) -> np.ndarray:
+    """For slot (i, s) pointing at j, the slot t of row j that points back at
+    i (int32; -1 for padding): A^T(i, s) = A(j, t)."""
+    neq, w = nbr.shape
+    nb = np.where(nbr >= 0, nbr, 0)
+    back = np.full((neq, w), -1, dtype=np.int32)
+    for t in range(w):
+        hit = (nbr >= 0) & (nbr[nb, t] == np.arange(neq)[:, None])
+        back = np.where(hit, t, back)
+    return back
+
+
+def heat_problem(n: int = 512, seed: int = SEED_BASE + 5, peclet_max: float = 10.0,
+                 subset_frac: float = 0.01):
+    """(operator, k, subset) of the differentiation config: the loss is
+    mean(T[subset]^2) over a seeded subset of subset_frac of the cells."""
+    mesh = quad_mesh(n, jitter=0.1, seed=seed)
+    k = lognormal_k(seed, mesh.neq)
+    op = HeatOperator.build(mesh, k, peclet_max, seed)
+    m = max(1, int(round(subset_frac * mesh.neq)))
+    subset = np.sort(np.argsort(uniform(seed, 51, mesh.neq))[:m])
+    return op, k, subset

@@ -162,6 +162,25 @@ int fvc_solve_batch_dev(int32_t code, int32_t nsys, const int32_t *row_offsets, 
                         fvc_stats *stats, void *stream);
 
 /* ----------------------------------------------------------------------------
+ * Differentiable solve (no counterpart in the reference).
+ *
+ * fvc_ell_transpose_dev: values of A^T for an ELLPACK matrix with a
+ * symmetric pattern, same ja: alt(s,i) = al(back(s,i), ja(s,i)), 0 where
+ * back(s,i) < 0 (padding).  back holds 0-based slots.
+ *
+ * fvc_grad_k_dev: dL/dk = -lam^T (dA/dk T - db/dk) for a finite-volume
+ * diffusion operator with harmonic face conductivity
+ * (1/k_f = alpha/k_c + (1-alpha)/k_N, D_f = k_f*geo; src/CellibTrans.f:
+ * 510-511,527) and Dirichlet faces (a_P += k_c*wall_g, b += k_c*wall_g*T_wall;
+ * src/CellibTrans.f:488-491).  Per (slot, cell) arrays geo, alpha, wall_g,
+ * wall_t in the ELLPACK layout; geo = 0 marks a non-interior slot. */
+int fvc_ell_transpose_dev(int32_t neq, int32_t w, const int32_t *ja, const int32_t *back,
+                          const double *al, double *alt, void *stream);
+int fvc_grad_k_dev(int32_t neq, int32_t w, const int32_t *ja, const double *k, const double *t,
+                   const double *lam, const double *geo, const double *alpha, const double *wall_g,
+                   const double *wall_t, double *grad_k, void *stream);
+
+/* ----------------------------------------------------------------------------
  * SpMV y = A x.  Replaces the matvec_* routines of src/Matvec.f:21-300 and
  * src/Matvec_omp.f (matvec_csrd[_ilu2]_omp, matvec_csrd_sym_omp family,
  * matvec_ellpack_omp).  Each row is summed in the reference's sequential
