@@ -119,6 +119,32 @@ def cpu_baseline(fmt, method, tol, sample_solves=1):
             "sample": f"1 full {method} solve, sequential C restatement", "seconds_per_solve": dt}
 
 
+def calibrate_copy(mib: int, reps: int) -> None:
+    """A plain device copy of a known size: rocprofv3's FETCH_SIZE and
+    WRITE_SIZE for it give the scale of those counters (bytes each reads per
+    byte moved), applied to the solver kernels' counts."""
+    import ctypes as C
+    import fortranfvcell_amd as F
+    F.init(0)
+    L = F.load()
+    nbytes = mib << 20
+    src = F.DeviceArray.empty(nbytes // 8)
+    dst = F.DeviceArray.empty(nbytes // 8)
+    L.fvc_memset_dev(C.c_void_p(src.ptr), 1, nbytes)
+    e0, e1 = C.c_void_p(), C.c_void_p()
+    L.fvc_event_create(C.byref(e0))
+    L.fvc_event_create(C.byref(e1))
+    L.fvc_event_record(e0, None)
+    for _ in range(reps):
+        F.check(L.fvc_memcpy_dtod(C.c_void_p(dst.ptr), C.c_void_p(src.ptr), nbytes, None))
+    L.fvc_event_record(e1, None)
+    ms = C.c_float()
+    L.fvc_event_elapsed_ms(e0, e1, C.byref(ms))
+    t = ms.value * 1e-3 / reps
+    print(json.dumps({"copy_bytes_each_way": nbytes, "reps": reps, "us_per_copy": t * 1e6,
+                      "GBs_read_plus_write": 2 * nbytes / t / 1e9}), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -127,7 +153,11 @@ def main():
     ap.add_argument("--n", type=int, default=1024, help="mesh side (n*n cells)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-extra", action="store_true")
+    ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
+                    help="only run the plain copy kernel on MIB MiB (hardware byte-counter reference)")
     args = ap.parse_args()
+    if args.calibrate_copy:
+        return calibrate_copy(args.calibrate_copy, max(args.steps, 1))
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU only (asked for --gpus {args.gpus}); "
               "the solver path is a single-GPU design, see the sharding notes", file=sys.stderr)

@@ -27,7 +27,7 @@ EXPORTS = [
     "fvc_default_options", "fvc_init", "fvc_last_error", "fvc_solver", "fvc_solve", "fvc_solve_dev",
     "fvc_solve_batch", "fvc_solve_batch_dev", "fvc_ell_transpose_dev", "fvc_grad_k_dev",
     "fvc_matvec", "fvc_matvec_dev", "fvc_dot", "fvc_dot_dev", "fvc_getres", "fvc_getres_dev",
-    "fvc_malloc", "fvc_free", "fvc_memcpy_htod", "fvc_memcpy_dtoh", "fvc_memset_dev",
+    "fvc_malloc", "fvc_free", "fvc_memcpy_htod", "fvc_memcpy_dtoh", "fvc_memcpy_dtod", "fvc_memset_dev",
     "fvc_synchronize", "fvc_device_info", "fvc_event_create", "fvc_event_destroy",
     "fvc_event_record", "fvc_event_elapsed_ms",
 ]
@@ -92,6 +92,7 @@ def load(path: str = LIB_PATH):
     L.fvc_memcpy_htod.argtypes = [P, P, SZ]
     L.fvc_memcpy_dtoh.argtypes = [P, P, SZ]
     L.fvc_memset_dev.argtypes = [P, C.c_int, SZ]
+    L.fvc_memcpy_dtod.argtypes = [P, P, SZ, P]
     L.fvc_synchronize.argtypes = []
     L.fvc_device_info.argtypes = [C.POINTER(I), C.c_char_p, I]
     L.fvc_event_create.argtypes = [C.POINTER(P)]

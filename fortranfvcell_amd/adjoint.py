@@ -1,0 +1,130 @@
+"""Differentiable finite-volume solve: T = A(k)^-1 (b(k) + q) with a
+hand-written backward.
+
+Forward: assemble the heat operator A(k), b(k) (problems.HeatOperator, the
+reference's coefficient forms), solve with Jacobi-BiCGSTAB on the GPU.
+Backward: the adjoint solve A^T lam = dL/dT, again Jacobi-BiCGSTAB on the GPU,
+on the same ELLPACK structure with transposed values (fvc_ell_transpose_dev),
+then the face-flux gradient kernel (fvc_grad_k_dev) gives dL/dk; dL/dq = lam.
+
+``FVSolve`` wraps this as a torch.autograd.Function on CPU float64 tensors;
+the arithmetic runs through the C ABI (libfvcell.so) on the GPU.
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from . import (BICGSTAB, ELLPACK, STOP_RELRES, X0_ZERO, DeviceArray, _ptr, check,
+               default_options, load)
+from . import problems as P
+
+
+class HeatSolver:
+    """Device-resident forward and adjoint solves of one HeatOperator."""
+
+    def __init__(self, op: P.HeatOperator, tol: float = 1e-13, maxit: int = 200000):
+        self.op = op
+        mesh = op.mesh
+        self.neq, self.w = mesh.neq, mesh.w
+        self.tol, self.maxit = tol, maxit
+        inner = mesh.nbr >= 0
+        self.nad = int(inner.sum())
+        ja = np.where(inner, mesh.nbr + 1, 1).astype(np.int32)
+        self.iax = DeviceArray.from_host(np.array([self.w, self.neq + 1], np.int32))
+        self.ja = DeviceArray.from_host(ja.ravel())
+        self.back = DeviceArray.from_host(P.ell_back_slots(mesh.nbr).ravel())
+        self.geo = DeviceArray.from_host(np.ascontiguousarray(op.geo.ravel()))
+        self.alpha = DeviceArray.from_host(np.ascontiguousarray(np.where(inner, op.alpha, 0.0).ravel()))
+        self.wall_g = DeviceArray.from_host(np.ascontiguousarray(op.wall_g.ravel()))
+        self.wall_T = DeviceArray.from_host(np.ascontiguousarray(op.wall_T.ravel()))
+        n, nw = self.neq, self.neq * self.w
+        self.ad = DeviceArray.empty(n)
+        self.al = DeviceArray.empty(nw)
+        self.alT = DeviceArray.empty(nw)
+        self.b = DeviceArray.empty(n)
+        self.T = DeviceArray.empty(n)
+        self.g = DeviceArray.empty(n)
+        self.lam = DeviceArray.empty(n)
+        self.k = DeviceArray.empty(n)
+        self.gk = DeviceArray.empty(n)
+        self.last_forward = None
+        self.last_adjoint = None
+
+    def _bicgstab(self, al: DeviceArray, x: DeviceArray, b: DeviceArray) -> dict:
+        from . import Stats
+        o = default_options(stop_rule=STOP_RELRES, x0_rule=X0_ZERO)
+        st = Stats()
+        rc = load().fvc_solve_dev(BICGSTAB, ELLPACK, 1, self.neq, self.nad, _ptr(self.ad), _ptr(al),
+                                  _ptr(al), _ptr(x), _ptr(b), _ptr(self.iax), _ptr(self.ja), self.tol,
+                                  self.maxit, C.byref(o), C.byref(st), None)
+        check(rc)
+        return st.as_dict()
+
+    def load(self, k: np.ndarray, q: np.ndarray | None = None) -> P.System:
+        """Assemble A(k), b(k) + q and put them in HBM."""
+        s = self.op.assemble(np.asarray(k, dtype=np.float64), q)
+        inner = s.nbr >= 0
+        self.ad.upload(s.ad)
+        self.al.upload(np.where(inner, s.off, 0.0).ravel())
+        self.b.upload(s.b)
+        self.k.upload(np.asarray(k, dtype=np.float64))
+        return s
+
+    def forward(self, k: np.ndarray, q: np.ndarray | None = None) -> np.ndarray:
+        self.load(k, q)
+        self.last_forward = self._bicgstab(self.al, self.T, self.b)
+        return self.T.get()
+
+    def solve_rhs(self, r: np.ndarray) -> np.ndarray:
+        """A x = r with the loaded matrix (x is returned; T is left alone)."""
+        self.g.upload(np.asarray(r, dtype=np.float64))
+        st = self._bicgstab(self.al, self.lam, self.g)
+        if st["status"] != 0:
+            raise RuntimeError(f"correction solve did not converge: {st}")
+        return self.lam.get()
+
+    def adjoint(self, gT: np.ndarray):
+        """(dL/dk, dL/dq) for the last forward solve, given dL/dT."""
+        L = load()
+        check(L.fvc_ell_transpose_dev(self.neq, self.w, _ptr(self.ja), _ptr(self.back), _ptr(self.al),
+                                      _ptr(self.alT), None))
+        self.g.upload(np.asarray(gT, dtype=np.float64))
+        self.last_adjoint = self._bicgstab(self.alT, self.lam, self.g)
+        check(L.fvc_grad_k_dev(self.neq, self.w, _ptr(self.ja), _ptr(self.k), _ptr(self.T), _ptr(self.lam),
+                               _ptr(self.geo), _ptr(self.alpha), _ptr(self.wall_g), _ptr(self.wall_T),
+                               _ptr(self.gk), None))
+        lam = self.lam.get()
+        return self.gk.get(), lam
+
+
+def make_fvsolve():
+    """torch.autograd.Function class (torch imported on first use)."""
+    import torch
+
+    class FVSolve(torch.autograd.Function):
+        """T = FVSolve.apply(k, q, solver): differentiable in k and q."""
+
+        @staticmethod
+        def forward(ctx, k, q, solver: HeatSolver):
+            T = solver.forward(k.detach().cpu().numpy(), q.detach().cpu().numpy())
+            ctx.solver = solver
+            return torch.from_numpy(T.copy())
+
+        @staticmethod
+        def backward(ctx, gT):
+            gk, gq = ctx.solver.adjoint(gT.detach().cpu().numpy())
+            return torch.from_numpy(gk), torch.from_numpy(gq), None
+
+    return FVSolve
+
+
+def loss_and_grad(solver: HeatSolver, k: np.ndarray, subset: np.ndarray, q: np.ndarray | None = None):
+    """L = mean(T[subset]^2), with dL/dk and dL/dq from one adjoint solve."""
+    T = solver.forward(k, q)
+    Ls = float(np.mean(T[subset] ** 2))
+    gT = np.zeros_like(T)
+    gT[subset] = 2.0 * T[subset] / len(subset)
+    gk, gq = solver.adjoint(gT)
+    return Ls, gk, gq, T

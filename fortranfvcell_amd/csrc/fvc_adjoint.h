@@ -57,4 +57,10 @@ __global__ void grad_k_kernel(int neq, int w, const int *__restrict__ ja, const 
   grad[c] = acc;
 }
 
+// Plain device copy, 16 bytes per lane per step (bandwidth reference).
+__global__ void copy_kernel(const uint4 *__restrict__ src, uint4 *__restrict__ dst, size_t n16) {
+  const size_t stride = (size_t)gridDim.x * blockDim.x;
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n16; i += stride) dst[i] = src[i];
+}
+
 }  // namespace fvc

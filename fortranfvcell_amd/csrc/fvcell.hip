@@ -1121,6 +1121,23 @@ int fvc_memcpy_dtoh(void *dst, const void *src, size_t bytes) {
   HIPCHK(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost));
   return FVC_OK;
 }
+int fvc_memcpy_dtod(void *dst, const void *src, size_t bytes, void *stream) {
+  if (!dst || !src) return fail(FVC_BAD_ARGUMENT, "null pointer");
+  if (bytes % 16 != 0 || ((uintptr_t)dst | (uintptr_t)src) % 16 != 0)
+    return fail(FVC_BAD_ARGUMENT, "fvc_memcpy_dtod needs 16-byte aligned pointers and sizes");
+  DevState *d = nullptr;
+  int rc = current_device(&d);
+  if (rc) return rc;
+  const size_t n16 = bytes / 16;
+  const int blocks = (int)std::min<size_t>((n16 + 255) / 256, (size_t)d->cus * 16);
+  if (n16) {
+    hipLaunchKernelGGL(copy_kernel, dim3(std::max(blocks, 1)), dim3(256), 0, (hipStream_t)stream,
+                       (const uint4 *)src, (uint4 *)dst, n16);
+    HIPCHK(hipGetLastError());
+  }
+  return FVC_OK;
+}
+
 int fvc_memset_dev(void *dst, int value, size_t bytes) {
   HIPCHK(hipMemset(dst, value, bytes));
   return FVC_OK;

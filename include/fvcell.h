@@ -215,6 +215,10 @@ int fvc_free(void *ptr);
 int fvc_memcpy_htod(void *dst, const void *src, size_t bytes);
 int fvc_memcpy_dtoh(void *dst, const void *src, size_t bytes);
 int fvc_memset_dev(void *dst, int value, size_t bytes);
+/* Device-to-device copy by a plain streaming kernel (16-byte aligned
+ * pointers, size a multiple of 16); also the bandwidth reference the
+ * benchmark calibrates hardware byte counters against. */
+int fvc_memcpy_dtod(void *dst, const void *src, size_t bytes, void *stream);
 int fvc_synchronize(void);
 /* Number of compute units and the device name (name may be NULL). */
 int fvc_device_info(int32_t *compute_units, char *name, int32_t name_len);
