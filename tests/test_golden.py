@@ -1,0 +1,64 @@
+"""The oracle replays the fixtures the reference's own solver wrote
+(tests/golden/make_golden.py): x and the overwritten b bit for bit, the
+iteration count and the energy norm of its unit-13 log line."""
+import os
+
+import numpy as np
+import pytest
+
+from fortranfvcell_amd import problems as P
+
+HERE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+SMALL = np.load(os.path.join(HERE, "ref_solver_small.npz"), allow_pickle=False)
+STATS = np.load(os.path.join(HERE, "ref_solver_1024_stats.npz"), allow_pickle=False)
+
+SYSTEMS = {"poisson16": lambda: P.config_poisson(16), "poisson32": lambda: P.config_poisson(32),
+           "advdiff16": lambda: P.config_advdiff(16),
+           "lognormal_tri8": lambda: P.config_pressure_lognormal(8, triangles=True)}
+LAY = {"ell": P.to_ell, "csrd": P.to_csrd}
+
+
+def _cases():
+    out = []
+    for i, key in enumerate(SMALL["keys"]):
+        key = str(key)
+        if "_code" in key:
+            out.append((i, key))
+    return out
+
+
+@pytest.mark.parametrize("i,key", _cases(), ids=[c[1] for c in _cases()])
+def test_oracle_replays_reference_fixture(orc, i, key):
+    sname, lay, code, omp = key.rsplit("_", 3)
+    s = SYSTEMS[sname]()
+    f = LAY[lay](s)
+    x0 = np.linspace(-1.0, 1.0, s.neq)
+    x0[0] = 0.0
+    x, b, st = orc.solver(f, int(code[4:]), unsym=True, omp=bool(int(omp[3:])), x0=x0, tol=1e-10)
+    assert np.array_equal(x, SMALL[key + "_x"])
+    assert np.array_equal(b, SMALL[key + "_b"])
+    assert st["iterations"] == SMALL["iterations"][i]
+    assert abs(st["energy"] - SMALL["energy"][i]) <= 1e-9 * abs(SMALL["energy"][i])
+
+
+def test_oracle_replays_gmres30_and_pinned_warm_start(orc):
+    keys = [str(k) for k in SMALL["keys"]]
+    s = P.config_advdiff(16)
+    x, _, st = orc.krylov("gmres", P.to_ell(s), k=30, tol=1e-10)
+    i = keys.index("advdiff16_ell_gmres30")
+    assert np.array_equal(x, SMALL["advdiff16_ell_gmres30_x"]) and st["iterations"] == SMALL["iterations"][i]
+    s = P.config_poisson(16)
+    x, _, st = orc.solver(P.to_ell(s), 1, omp=True, x0=np.linspace(-1.0, 1.0, s.neq), tol=1e-10)
+    i = keys.index("poisson16_ell_pinned_warmstart")
+    assert np.array_equal(x, SMALL["poisson16_ell_pinned_warmstart_x"])
+    assert st["iterations"] == SMALL["iterations"][i] == 1
+
+
+def test_oracle_replays_1024_bicgstab_statistics(orc):
+    name = "advdiff1024_ell_bicgstab"
+    s = P.config_advdiff(1024)
+    x, _, st = orc.solver(P.to_ell(s), 2, omp=False, tol=1e-10)
+    assert st["iterations"] == STATS[name + "_iterations"]
+    assert abs(st["energy"] - STATS[name + "_energy"]) <= 1e-9 * abs(STATS[name + "_energy"])
+    assert np.array_equal(x[STATS[name + "_idx"]], STATS[name + "_xsample"])
+    assert x.sum() == STATS[name + "_xsum"] and (x * x).sum() == STATS[name + "_xsq"]
