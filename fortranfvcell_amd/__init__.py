@@ -215,6 +215,9 @@ class DeviceSystem:
         self.al = DeviceArray.from_host(al) if al is not None else None
         self.au = DeviceArray.from_host(au) if (au is not None and au is not al) else self.al
         self.iax = DeviceArray.from_host(iax)
+        # ELLPACK's iax is {w, neq+1}: handed over from host memory, so that a
+        # solve or SpMV on resident data needs no device-to-host read
+        self._iax_arg = iax if self.matriz == ELLPACK else self.iax
         self.ja = DeviceArray.from_host(ja)
         self.b = DeviceArray.from_host(np.ascontiguousarray(fmt["b"], dtype=np.float64))
         self.x = DeviceArray.empty(self.neq)
@@ -227,14 +230,14 @@ class DeviceSystem:
         st = Stats()
         rc = load().fvc_solve_dev(code, self.matriz, int(self.unsym), self.neq, self.nad,
                                   _ptr(self.ad), _ptr(self.au or self.ad), _ptr(self.al or self.ad),
-                                  _ptr(self.x), _ptr(self.b), _ptr(self.iax), _ptr(self.ja),
+                                  _ptr(self.x), _ptr(self.b), _ptr(self._iax_arg), _ptr(self.ja),
                                   float(tol), int(maxit), C.byref(o), C.byref(st), stream)
         check(rc, _NONFATAL)
         return st.as_dict()
 
     def matvec(self, x: DeviceArray, y: DeviceArray, stream=None) -> None:
         check(load().fvc_matvec_dev(self.matriz, int(self.unsym), self.neq, self.nad, _ptr(self.ad),
-                                    _ptr(self.au or self.ad), _ptr(self.al or self.ad), _ptr(self.iax),
+                                    _ptr(self.au or self.ad), _ptr(self.al or self.ad), _ptr(self._iax_arg),
                                     _ptr(self.ja), _ptr(x), _ptr(y), stream))
 
 

@@ -32,7 +32,7 @@ class HeatSolver:
         inner = mesh.nbr >= 0
         self.nad = int(inner.sum())
         ja = np.where(inner, mesh.nbr + 1, 1).astype(np.int32)
-        self.iax = DeviceArray.from_host(np.array([self.w, self.neq + 1], np.int32))
+        self.iax = np.array([self.w, self.neq + 1], np.int32)   # host: read without a sync
         self.ja = DeviceArray.from_host(ja.ravel())
         self.back = DeviceArray.from_host(P.ell_back_slots(mesh.nbr).ravel())
         self.geo = DeviceArray.from_host(np.ascontiguousarray(op.geo.ravel()))

@@ -154,27 +154,22 @@ constexpr int kCtrTop = 8 * kCtrStride;
 constexpr int kCtrGen = 9 * kCtrStride;
 constexpr int kCtrErr = 10 * kCtrStride;
 constexpr int kCtrWords = 11 * kCtrStride;
-// Bounded wait: ~2^22 polls with s_sleep in between (well over a second);
+// Bounded wait: ~2^23 polls with s_sleep in between (about a second);
 // a barrier that never completes sets the error flag and every block leaves.
-constexpr unsigned kSpinLimit = 1u << 22;
+constexpr unsigned kSpinLimit = 1u << 23;
 
-__device__ __forceinline__ unsigned ld_acquire(unsigned *p) {
-  return __hip_atomic_load(p, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
-}
 __device__ __forceinline__ unsigned ld_relaxed(unsigned *p) {
   return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-__device__ __forceinline__ void st_release(unsigned *p, unsigned v) {
-  __hip_atomic_store(p, v, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
 }
 __device__ __forceinline__ void st_relaxed(unsigned *p, unsigned v) {
   __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
-__device__ __forceinline__ unsigned add_acq_rel(unsigned *p, unsigned v) {
-  return __hip_atomic_fetch_add(p, v, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+__device__ __forceinline__ unsigned add_relaxed(unsigned *p, unsigned v) {
+  return __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 __device__ __forceinline__ void fence_acquire() { __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent"); }
 __device__ __forceinline__ void fence_release() { __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent"); }
+__device__ __forceinline__ void fence_acq_rel() { __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "agent"); }
 
 struct Grid {
   int G;                 // blocks = row chunks
@@ -206,6 +201,13 @@ __device__ __forceinline__ int group_size(int G, int g) { return (G - g + 7) / 8
 // waits for the epilogue to be published; in kernel-per-phase mode the other
 // blocks return at once and the kernel boundary publishes it.
 // Returns false if a bounded wait expired (the caller must leave).
+//
+// Memory ordering: each block releases once (its vector writes and its
+// partial) before a relaxed arrival; group leaders and the last block
+// acquire once; waiters poll with relaxed loads and acquire once when the
+// generation moves.  On gfx950 an agent-scope acquire invalidates the L2 of
+// the XCD, so it happens once per block per barrier, by thread 0 only; the
+// block's other waves order behind it with the workgroup barrier.
 template <int K, class Epi>
 __device__ bool grid_reduce(const Grid &g, double *v, double *lds, Epi &&epi) {
   __shared__ int s_last;
@@ -221,12 +223,14 @@ __device__ bool grid_reduce(const Grid &g, double *v, double *lds, Epi &&epi) {
     const int grp = g.bid() & 7;
     const int ngroups = g.G < 8 ? g.G : 8;
     int last = 0;
-    const unsigned t = add_acq_rel(g.ctr + grp * kCtrStride, 1u);
+    const unsigned t = add_relaxed(g.ctr + grp * kCtrStride, 1u);
     if ((int)t + 1 == group_size(g.G, grp)) {
       st_relaxed(g.ctr + grp * kCtrStride, 0u);
-      const unsigned t2 = add_acq_rel(g.ctr + kCtrTop, 1u);
+      fence_acq_rel();
+      const unsigned t2 = add_relaxed(g.ctr + kCtrTop, 1u);
       if ((int)t2 + 1 == ngroups) {
         st_relaxed(g.ctr + kCtrTop, 0u);
+        fence_acquire();
         last = 1;
       }
     }
@@ -236,7 +240,6 @@ __device__ bool grid_reduce(const Grid &g, double *v, double *lds, Epi &&epi) {
   }
   __syncthreads();
   if (s_last) {
-    fence_acquire();
     double tot[K > 0 ? K : 1];
     if constexpr (K > 0) {
       double w[K];
@@ -248,24 +251,24 @@ __device__ bool grid_reduce(const Grid &g, double *v, double *lds, Epi &&epi) {
     __syncthreads();
     if (threadIdx.x == 0 && g.persistent) {
       fence_release();
-      st_release(g.ctr + kCtrGen, s_gen + 1u);
+      st_relaxed(g.ctr + kCtrGen, s_gen + 1u);
     }
   } else if (g.persistent) {
     if (threadIdx.x == 0) {
       unsigned spins = 0;
-      while (ld_acquire(g.ctr + kCtrGen) == s_gen) {
-        __builtin_amdgcn_s_sleep(2);
+      while (ld_relaxed(g.ctr + kCtrGen) == s_gen) {
+        __builtin_amdgcn_s_sleep(1);
         if (++spins > kSpinLimit || ld_relaxed(g.ctr + kCtrErr) != 0u) {
           st_relaxed(g.ctr + kCtrErr, 1u);
           s_ok = 0;
           break;
         }
       }
+      fence_acquire();
     }
   }
   if (g.persistent) {
     __syncthreads();
-    fence_acquire();
     return s_ok != 0;
   }
   return true;

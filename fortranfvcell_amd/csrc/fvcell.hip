@@ -199,15 +199,24 @@ struct SolveIn {
   int maxit, x0_zero, stop_rule, gmres_k, engine, bs, G_req, precision;
 };
 
+// Persistent launches are ordinary launches whose grid fits the device at
+// once (G <= co-resident blocks, checked with the occupancy API before every
+// persistent launch); the runtime's cooperative-launch queue is not used (it
+// is not shareable between processes).  Should blocks nevertheless not be
+// co-resident, the kernels' bounded waits expire, every block leaves, and the
+// solve reports it (the AUTO engine then reruns it one kernel per phase).
 template <class KArgs>
-int launch(const void *fn, bool coop, int G, int bs, KArgs &args, hipStream_t s) {
+int launch(const void *fn, bool persistent, int G, int bs, KArgs &args, hipStream_t s) {
+  (void)persistent;
   void *params[] = {&args};
-  if (coop) {
-    HIPCHK(hipLaunchCooperativeKernel(fn, dim3(G), dim3(bs), params, 0, s));
-  } else {
-    HIPCHK(hipLaunchKernel(fn, dim3(G), dim3(bs), params, 0, s));
-  }
+  HIPCHK(hipLaunchKernel(fn, dim3(G), dim3(bs), params, 0, s));
   return FVC_OK;
+}
+
+bool barrier_expired(unsigned *ctr) {
+  unsigned err = 0;
+  if (hipMemcpy(&err, ctr + kCtrErr, sizeof err, hipMemcpyDeviceToHost) != hipSuccess) return true;
+  return err != 0;
 }
 
 int read_state(DevState &d, KState *st, hipStream_t s) {
@@ -255,7 +264,7 @@ int run_solve(DevState &d, const SolveIn &in, fvc_stats *stats, hipStream_t s) {
   int nvec = in.code == FVC_PCG ? 5 : in.code == FVC_BICGSTAB ? 6 : (k + 1);
   size_t bytes = align_up(kCtrWords * sizeof(unsigned)) + align_up(sizeof(KState)) +
                  align_up(kMaxK * (size_t)G * sizeof(double)) + (size_t)nvec * align_up(n * sizeof(double)) +
-                 align_up((size_t)(k + 2) * (k + 2) * sizeof(double)) * 2 + 4096;
+                 align_up((size_t)(k + 2) * (k + 2) * sizeof(double)) * 2 + align_up(n * sizeof(double)) + 4096;
   rc = ensure_arena(d, bytes);
   if (rc) return rc;
   Carve cv{d.arena};
@@ -265,6 +274,10 @@ int run_solve(DevState &d, const SolveIn &in, fvc_stats *stats, hipStream_t s) {
   std::vector<double *> vec(nvec);
   for (int i = 0; i < nvec; ++i) vec[i] = cv.take<double>(n);
   double *gsmall = cv.take<double>((size_t)(k + 2) * (k + 2) * 2);
+  double *xbackup = cv.take<double>(n);
+  const bool may_fall_back = persistent && in.engine == FVC_ENGINE_AUTO;
+  if (may_fall_back && !in.x0_zero)
+    HIPCHK(hipMemcpyAsync(xbackup, in.x, n * sizeof(double), hipMemcpyDeviceToDevice, s));
 
   Grid g = make_grid(neq, G, ctr, partials, persistent);
   HIPCHK(hipEventRecord(d.ev0, s));
@@ -375,6 +388,14 @@ int run_solve(DevState &d, const SolveIn &in, fvc_stats *stats, hipStream_t s) {
     }
   }
   HIPCHK(hipGetLastError());
+  if (may_fall_back && barrier_expired(ctr)) {
+    // blocks were not co-resident: rerun one kernel per phase from the same x0
+    if (!in.x0_zero)
+      HIPCHK(hipMemcpyAsync(in.x, xbackup, n * sizeof(double), hipMemcpyDeviceToDevice, s));
+    SolveIn again = in;
+    again.engine = FVC_ENGINE_KERNELS;
+    return run_solve<F>(d, again, stats, s);
+  }
   float ms = 0.f;
   HIPCHK(hipEventElapsedTime(&ms, d.ev0, d.ev1));
   fill_stats(stats, *d.host_st, in.code, persistent, ms, G, bs);
@@ -578,6 +599,17 @@ struct Owned {
   }
 };
 
+// Device (or managed) memory?  Plain host memory reports an error or an
+// unregistered type; either way it is read directly.
+bool is_device_pointer(const void *p) {
+  hipPointerAttribute_t attr;
+  if (hipPointerGetAttributes(&attr, p) != hipSuccess) {
+    (void)hipGetLastError();
+    return false;
+  }
+  return attr.type == hipMemoryTypeDevice || attr.type == hipMemoryTypeManaged;
+}
+
 int build_device_matrix(int matriz, int unsym, int neq, int nad, const double *ad,
                         const double *au, const double *al, const int32_t *iax,
                         const int32_t *ja, bool on_device, Owned &own, MatDesc &md,
@@ -587,7 +619,7 @@ int build_device_matrix(int matriz, int unsym, int neq, int nad, const double *a
   const int *ia_host = nullptr;
   if (matriz == FVC_ELLPACK) {
     int nl[2] = {0, 0};
-    if (on_device) HIPCHK(hipMemcpy(nl, iax, sizeof nl, hipMemcpyDefault));
+    if (on_device && is_device_pointer(iax)) HIPCHK(hipMemcpy(nl, iax, sizeof nl, hipMemcpyDeviceToHost));
     else std::memcpy(nl, iax, sizeof nl);
     w = nl[0];
     if (w < 1 || w > 64) return fail(FVC_BAD_ARGUMENT, "ELLPACK width iax(1) = %d", w);
