@@ -219,8 +219,19 @@ __global__ __launch_bounds__(1024) void pcg_kernel(PcgArgs a) {
 }
 
 // ------------------------------------------------------------- BiCGSTAB --
+//
+// src/Iterativos.f:140-234 / src/Iterativos_omp.f:161-274 in three phases per
+// iteration, one per reduction the recurrence needs (alpha, omega, beta):
+//   F1  p = b + beta (p - w v), z = p / m    (the previous iteration's tail)
+//       v = A z, with z of the neighbours formed on the fly;   -> v.r
+//   F2  b' = b - alpha v, z2 = b' / m, t = A z2 (z2 on the fly); -> t.b', t.t
+//   F3  x = (x + alpha z) + w z2, b' = b' - w t;   -> b'.z2, r.b', b'.b'
+// The vectors the on-the-fly forms read (b, p, v) alternate between two
+// buffers so that no block overwrites a value another block still gathers.
+// Every value is the reference's expression in the reference's order, so the
+// iterates are those of the five-loop form.
 
-enum { BI_SETUP = 0, BI_P1 = 1, BI_P2 = 2, BI_P3 = 3, BI_P4 = 4, BI_P5 = 5, BI_FINAL = 6 };
+enum { BI_SETUP = 0, BI_F1 = 1, BI_F2 = 2, BI_F3 = 3, BI_FINAL = 4 };
 
 struct BicgArgs {
   Mat A;
@@ -229,13 +240,30 @@ struct BicgArgs {
   const double *m;
   double *x;
   const double *b;
-  double *bb;   // running residual (the reference's overwritten b)
-  double *r, *p, *z, *v, *t;
+  double *bb0, *bb1;   // running residual (the reference's overwritten b), two buffers
+  double *p0, *p1, *v0, *v1;
+  double *r, *z, *z2, *t;
   double tol;
   int maxit, x0_zero, stop_rule, phase;
 };
 
-// src/Iterativos.f:140-234 / src/Iterativos_omp.f:161-274.
+// z of the F1 gather: the new direction over m, or p/m in the first iteration.
+struct XBiDir {
+  const double *bb, *p, *v, *m;
+  double beta, w;
+  int first;
+  __device__ __forceinline__ double operator()(int j) const {
+    if (first) return p[j] / m[j];
+    return (bb[j] + beta * (p[j] - w * v[j])) / m[j];
+  }
+};
+// z2 of the F2 gather: (b - alpha v) / m.
+struct XBiRes {
+  const double *bb, *v, *m;
+  double alpha;
+  __device__ __forceinline__ double operator()(int j) const { return (bb[j] - alpha * v[j]) / m[j]; }
+};
+
 template <int F>
 __device__ bool bicg_phase(const BicgArgs &a, int ph, double *lds) {
   const int i0 = a.g.row_begin(), i1 = a.g.row_end();
@@ -255,9 +283,8 @@ __device__ bool bicg_phase(const BicgArgs &a, int ph, double *lds) {
       const double ri = bi - ax;
       const double zi = ri / a.m[i];
       a.r[i] = ri;
-      a.p[i] = ri;
-      a.bb[i] = ri;
-      a.z[i] = zi;
+      a.p0[i] = ri;
+      a.bb0[i] = ri;
       v[0] += ri * zi;
       v[1] += ri * ri;   // = b.r of the first iteration (b = p = r)
       v[2] += bi * bi;
@@ -283,65 +310,65 @@ __device__ bool bicg_phase(const BicgArgs &a, int ph, double *lds) {
       if (a.maxit <= 0 && !st->done) st->done = 1;
     });
   }
-  if (ph == BI_P1) {
-    // v = A z; alpha = rr0 / v.r
+  const int j = ld_int(&st->iter) + 1;   // iteration in progress
+  const bool odd = (j & 1) != 0;         // buffers: odd j reads set 0, writes set 1
+  const double *bbc = odd ? a.bb0 : a.bb1;
+  double *bbn = odd ? a.bb1 : a.bb0;
+  const double *pc = odd ? a.p0 : a.p1;
+  double *pn = odd ? a.p1 : a.p0;
+  const double *vc = odd ? a.v0 : a.v1;
+  double *vn = odd ? a.v1 : a.v0;
+  if (ph == BI_F1) {
+    const XBiDir zf{bbc, pc, vc, a.m, ld_dbl(&st->beta), ld_dbl(&st->omega), j == 1};
     double v[1] = {0.0};
     for (int i = i0 + (int)threadIdx.x; i < i1; i += blockDim.x) {
-      const double vi = row_apply<F>(a.A, i, XVec{a.z});
-      a.v[i] = vi;
+      const double pi = (j == 1) ? pc[i] : bbc[i] + zf.beta * (pc[i] - zf.w * vc[i]);
+      const double zi = pi / a.m[i];
+      const double vi = row_apply<F>(a.A, i, zf);
+      pn[i] = pi;
+      a.z[i] = zi;
+      vn[i] = vi;
       v[0] += vi * a.r[i];
     }
     return grid_reduce<1>(a.g, v, lds, [&](const double *t) {
       const double alpha = st->rr0 / t[0];
       st->alpha = alpha;
-      if (t[0] == 0.0) { st->status = S_BREAKDOWN; st->done = 1; st->iter = st->iter + 1; }
-      else if (!finite(alpha)) { st->status = S_NONFINITE; st->done = 1; st->iter = st->iter + 1; }
+      if (t[0] == 0.0) { st->status = S_BREAKDOWN; st->done = 1; st->iter = j; }
+      else if (!finite(alpha)) { st->status = S_NONFINITE; st->done = 1; st->iter = j; }
     });
   }
-  if (ph == BI_P2) {
-    // x += alpha z; b -= alpha v; z = b/m
-    const double alpha = ld_dbl(&st->alpha);
-    for (int i = i0 + (int)threadIdx.x; i < i1; i += blockDim.x) {
-      const double zi = a.z[i];
-      a.x[i] = a.x[i] + alpha * zi;
-      const double bi = a.bb[i] - alpha * a.v[i];
-      a.bb[i] = bi;
-      a.z[i] = bi / a.m[i];
-    }
-    double v[1] = {0.0};
-    return grid_reduce<0>(a.g, v, lds, [&](const double *) {});
-  }
-  if (ph == BI_P3) {
-    // t = A z; w = t.b / t.t
+  if (ph == BI_F2) {
+    const XBiRes zf{bbc, vn, a.m, ld_dbl(&st->alpha)};
     double v[2] = {0.0, 0.0};
     for (int i = i0 + (int)threadIdx.x; i < i1; i += blockDim.x) {
-      const double ti = row_apply<F>(a.A, i, XVec{a.z});
+      const double bi = bbc[i] - zf.alpha * vn[i];
+      const double ti = row_apply<F>(a.A, i, zf);
+      bbn[i] = bi;
+      a.z2[i] = bi / a.m[i];
       a.t[i] = ti;
-      v[0] += ti * a.bb[i];
+      v[0] += ti * bi;
       v[1] += ti * ti;
     }
     return grid_reduce<2>(a.g, v, lds, [&](const double *t) {
       const double w = t[0] / t[1];
       st->omega = w;
-      if (t[1] == 0.0 || w == 0.0) { st->status = S_BREAKDOWN; st->done = 1; st->iter = st->iter + 1; }
-      else if (!finite(w)) { st->status = S_NONFINITE; st->done = 1; st->iter = st->iter + 1; }
+      if (t[1] == 0.0 || w == 0.0) { st->status = S_BREAKDOWN; st->done = 1; st->iter = j; }
+      else if (!finite(w)) { st->status = S_NONFINITE; st->done = 1; st->iter = j; }
     });
   }
-  if (ph == BI_P4) {
-    // x += w z; b -= w t; beta = b.z / d (stop test); beta = (r.b/rr0)(alpha/w)
-    const double w = ld_dbl(&st->omega);
+  if (ph == BI_F3) {
+    const double alpha = ld_dbl(&st->alpha), w = ld_dbl(&st->omega);
     double v[3] = {0.0, 0.0, 0.0};
     for (int i = i0 + (int)threadIdx.x; i < i1; i += blockDim.x) {
-      const double zi = a.z[i];
-      a.x[i] = a.x[i] + w * zi;
-      const double bi = a.bb[i] - w * a.t[i];
-      a.bb[i] = bi;
-      v[0] += bi * zi;
+      const double z2i = a.z2[i];
+      a.x[i] = (a.x[i] + alpha * a.z[i]) + w * z2i;
+      const double bi = bbn[i] - w * a.t[i];
+      bbn[i] = bi;
+      v[0] += bi * z2i;
       v[1] += a.r[i] * bi;
       v[2] += bi * bi;
     }
     return grid_reduce<3>(a.g, v, lds, [&](const double *t) {
-      const int j = st->iter + 1;
       st->iter = j;
       st->rnorm2 = t[2];
       const double beta1 = t[0] / st->d;
@@ -360,17 +387,6 @@ __device__ bool bicg_phase(const BicgArgs &a, int ph, double *lds) {
       else if (!finite(beta1) || !finite(beta)) { st->status = S_NONFINITE; st->done = 1; }
       else if (j >= a.maxit) { st->status = S_MAXIT; st->done = 1; }
     });
-  }
-  if (ph == BI_P5) {
-    // p = b + beta (p - w v); z = p/m
-    const double beta = ld_dbl(&st->beta), w = ld_dbl(&st->omega);
-    for (int i = i0 + (int)threadIdx.x; i < i1; i += blockDim.x) {
-      const double pi = a.bb[i] + beta * (a.p[i] - w * a.v[i]);
-      a.p[i] = pi;
-      a.z[i] = pi / a.m[i];
-    }
-    double v[1] = {0.0};
-    return grid_reduce<0>(a.g, v, lds, [&](const double *) {});
   }
   // BI_FINAL: energy x.Ax (src/Iterativos.f:204-205) and ||b - Ax||^2
   double v[2] = {0.0, 0.0};
@@ -391,14 +407,11 @@ __global__ __launch_bounds__(1024) void bicg_persistent(BicgArgs a) {
   __shared__ double lds[kMaxK * kMaxWaves];
   if (!bicg_phase<F>(a, BI_SETUP, lds)) return;
   while (!ld_int(&a.st->done)) {
-    if (!bicg_phase<F>(a, BI_P1, lds)) return;
+    if (!bicg_phase<F>(a, BI_F1, lds)) return;
     if (ld_int(&a.st->done)) break;
-    if (!bicg_phase<F>(a, BI_P2, lds)) return;
-    if (!bicg_phase<F>(a, BI_P3, lds)) return;
+    if (!bicg_phase<F>(a, BI_F2, lds)) return;
     if (ld_int(&a.st->done)) break;
-    if (!bicg_phase<F>(a, BI_P4, lds)) return;
-    if (ld_int(&a.st->done)) break;
-    if (!bicg_phase<F>(a, BI_P5, lds)) return;
+    if (!bicg_phase<F>(a, BI_F3, lds)) return;
   }
   bicg_phase<F>(a, BI_FINAL, lds);
 }

@@ -261,7 +261,7 @@ int run_solve(DevState &d, const SolveIn &in, fvc_stats *stats, hipStream_t s) {
 
   // workspace
   const size_t n = (size_t)neq;
-  int nvec = in.code == FVC_PCG ? 5 : in.code == FVC_BICGSTAB ? 6 : (k + 1);
+  int nvec = in.code == FVC_PCG ? 5 : in.code == FVC_BICGSTAB ? 10 : (k + 1);
   size_t bytes = align_up(kCtrWords * sizeof(unsigned)) + align_up(sizeof(KState)) +
                  align_up(kMaxK * (size_t)G * sizeof(double)) + (size_t)nvec * align_up(n * sizeof(double)) +
                  align_up((size_t)(k + 2) * (k + 2) * sizeof(double)) * 2 + align_up(n * sizeof(double)) + 4096;
@@ -319,7 +319,8 @@ int run_solve(DevState &d, const SolveIn &in, fvc_stats *stats, hipStream_t s) {
   } else if (in.code == FVC_BICGSTAB) {
     BicgArgs a{};
     a.A = in.A; a.g = g; a.st = st; a.m = in.m; a.x = in.x; a.b = in.b;
-    a.bb = vec[0]; a.r = vec[1]; a.p = vec[2]; a.z = vec[3]; a.v = vec[4]; a.t = vec[5];
+    a.bb0 = vec[0]; a.bb1 = vec[1]; a.p0 = vec[2]; a.p1 = vec[3]; a.v0 = vec[4]; a.v1 = vec[5];
+    a.r = vec[6]; a.z = vec[7]; a.z2 = vec[8]; a.t = vec[9];
     a.tol = in.tol; a.maxit = in.maxit; a.x0_zero = in.x0_zero; a.stop_rule = in.stop_rule;
     if (persistent) {
       a.phase = -1;
@@ -332,7 +333,7 @@ int run_solve(DevState &d, const SolveIn &in, fvc_stats *stats, hipStream_t s) {
       int batch = 8, launched = 0;
       for (;;) {
         for (int it = 0; it < batch && launched < in.maxit; ++it, ++launched)
-          for (int ph = BI_P1; ph <= BI_P5; ++ph) {
+          for (int ph = BI_F1; ph <= BI_F3; ++ph) {
             a.phase = ph;
             if ((rc = launch(kfn, false, G, bs, a, s))) return rc;
           }
