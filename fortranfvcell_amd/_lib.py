@@ -10,7 +10,7 @@ import os
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libfvcell.so")
-FORTRAN_LIB_PATH = os.path.join(HERE, "libfvcell_fortran.so")
+FORTRAN_LIB_PATH = os.path.join(HERE, "libfvcell_fortran.a")
 
 FVC_OK, FVC_MAXIT, FVC_BREAKDOWN, FVC_NONFINITE, FVC_UNSUPPORTED, FVC_DEVICE_ERROR, FVC_BAD_ARGUMENT = range(7)
 STATUS_NAMES = {0: "converged", 1: "maxit", 2: "breakdown", 3: "nonfinite", 4: "unsupported",

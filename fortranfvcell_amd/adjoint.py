@@ -16,7 +16,7 @@ import ctypes as C
 
 import numpy as np
 
-from . import (BICGSTAB, ELLPACK, STOP_RELRES, X0_ZERO, DeviceArray, _ptr, check,
+from . import (BICGSTAB, ELLPACK, STOP_RELRES, X0_GIVEN, X0_ZERO, DeviceArray, _ptr, check,
                default_options, load)
 from . import problems as P
 
@@ -52,14 +52,15 @@ class HeatSolver:
         self.last_forward = None
         self.last_adjoint = None
 
-    def _bicgstab(self, al: DeviceArray, x: DeviceArray, b: DeviceArray) -> dict:
-        from . import Stats
-        o = default_options(stop_rule=STOP_RELRES, x0_rule=X0_ZERO)
+    def _bicgstab(self, al: DeviceArray, x: DeviceArray, b: DeviceArray, warm: bool = False,
+                  strict: bool = True) -> dict:
+        from . import _NONFATAL, Stats
+        o = default_options(stop_rule=STOP_RELRES, x0_rule=X0_GIVEN if warm else X0_ZERO)
         st = Stats()
         rc = load().fvc_solve_dev(BICGSTAB, ELLPACK, 1, self.neq, self.nad, _ptr(self.ad), _ptr(al),
                                   _ptr(al), _ptr(x), _ptr(b), _ptr(self.iax), _ptr(self.ja), self.tol,
                                   self.maxit, C.byref(o), C.byref(st), None)
-        check(rc)
+        check(rc, (0,) if strict else _NONFATAL)
         return st.as_dict()
 
     def load(self, k: np.ndarray, q: np.ndarray | None = None) -> P.System:
@@ -77,10 +78,20 @@ class HeatSolver:
         self.last_forward = self._bicgstab(self.al, self.T, self.b)
         return self.T.get()
 
-    def solve_rhs(self, r: np.ndarray) -> np.ndarray:
-        """A x = r with the loaded matrix (x is returned; T is left alone)."""
+    def solve_rhs(self, r: np.ndarray, restarts: int = 8) -> np.ndarray:
+        """A x = r with the loaded matrix (x is returned; T is left alone).
+        BiCGSTAB's shadow residual is the initial residual; for a very
+        localised r it can become orthogonal to the iterates (breakdown).
+        The solve then restarts from its last finite iterate, whose residual
+        is spread out."""
         self.g.upload(np.asarray(r, dtype=np.float64))
-        st = self._bicgstab(self.al, self.lam, self.g)
+        st = self._bicgstab(self.al, self.lam, self.g, strict=False)
+        for _ in range(restarts):
+            if st["status"] == 0:
+                break
+            if st["status"] not in (2, 3):
+                raise RuntimeError(f"correction solve did not converge: {st}")
+            st = self._bicgstab(self.al, self.lam, self.g, warm=True, strict=False)
         if st["status"] != 0:
             raise RuntimeError(f"correction solve did not converge: {st}")
         return self.lam.get()

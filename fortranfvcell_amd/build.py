@@ -1,8 +1,10 @@
 """Builds the native parts in-tree:
 
   libfvcell.so          hipcc, gfx950: kernels + C ABI (include/fvcell.h)
-  libfvcell_fortran.so  amdflang: the ISO_C_BINDING drop-in for the
-                        reference's Solver.o (subroutines solver, getRes)
+  libfvcell_fortran.a   amdflang: the ISO_C_BINDING drop-in for the
+                        reference's Solver.o (subroutines solver, getRes),
+                        linked into the calling program together with
+                        -lfvcell
 
 Run ``python -m fortranfvcell_amd.build``.
 """
@@ -47,15 +49,22 @@ def build_fortran() -> str | None:
     fc = shutil.which("amdflang") or (os.path.exists("/opt/rocm/bin/amdflang") and "/opt/rocm/bin/amdflang")
     if not fc:
         return None
-    out = os.path.join(HERE, "libfvcell_fortran.so")
+    out = os.path.join(HERE, "libfvcell_fortran.a")
     objdir = os.path.join(HERE, "fortran", "_obj")
     os.makedirs(objdir, exist_ok=True)
     src = os.path.join(HERE, "fortran", "fvc_solver.f90")
     obj = os.path.join(objdir, "fvc_solver.o")
     _run([fc, "-O2", "-fPIC", "-module-dir", objdir, "-c", src, "-o", obj])
+    # a static archive: linked into the calling program it shares that
+    # program's Fortran runtime (unit 13, STOP), exactly like Solver.o did
     tmp = out + ".tmp"
-    _run([fc, "-shared", "-o", tmp, obj, f"-L{HERE}", "-lfvcell", "-Wl,-rpath,$ORIGIN", "-Wl,--exclude-libs,ALL"])
+    if os.path.exists(tmp):
+        os.remove(tmp)
+    _run(["ar", "rcs", tmp, obj])
     os.replace(tmp, out)
+    stale = os.path.join(HERE, "libfvcell_fortran.so")
+    if os.path.exists(stale):
+        os.remove(stale)
     return out
 
 
@@ -70,7 +79,8 @@ def build_fortran_driver() -> str | None:
     objdir = os.path.join(HERE, "fortran", "_obj")
     os.makedirs(objdir, exist_ok=True)
     _run([fc, "-O2", "-module-dir", objdir, os.path.join(tdir, "drive_solver.f90"), "-o", out,
-          f"-L{HERE}", "-lfvcell_fortran", f"-Wl,-rpath,{HERE}", "-Wl,-rpath,$ORIGIN/../../fortranfvcell_amd"])
+          os.path.join(HERE, "libfvcell_fortran.a"), f"-L{HERE}", "-lfvcell", f"-Wl,-rpath,{HERE}",
+          "-Wl,-rpath,$ORIGIN/../../fortranfvcell_amd"])
     return out
 
 

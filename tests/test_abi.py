@@ -11,7 +11,7 @@ import pytest
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 HEADER = os.path.join(ROOT, "include", "fvcell.h")
 LIB = os.path.join(ROOT, "fortranfvcell_amd", "libfvcell.so")
-FLIB = os.path.join(ROOT, "fortranfvcell_amd", "libfvcell_fortran.so")
+FLIB = os.path.join(ROOT, "fortranfvcell_amd", "libfvcell_fortran.a")
 
 
 def declared():
@@ -38,7 +38,7 @@ def test_python_binding_covers_the_header():
 
 
 def test_fortran_dropin_exports_solver_and_getres():
-    out = subprocess.run(["nm", "-D", "--defined-only", FLIB], capture_output=True, text=True, check=True).stdout
+    out = subprocess.run(["nm", "--defined-only", FLIB], capture_output=True, text=True, check=True).stdout
     syms = {line.split()[-1] for line in out.splitlines() if " T " in line}
     # gfortran/flang external names of src/Solver.f:28 and :368
     assert "solver_" in syms and "getres_" in syms
