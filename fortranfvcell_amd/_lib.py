@@ -37,7 +37,7 @@ class Options(C.Structure):
     _fields_ = [("x0_rule", C.c_int32), ("gmres_k", C.c_int32), ("stop_rule", C.c_int32),
                 ("true_residual", C.c_int32), ("engine", C.c_int32), ("block_size", C.c_int32),
                 ("grid_blocks", C.c_int32), ("precision", C.c_int32), ("batch_work_exp10", C.c_int32),
-                ("reserved", C.c_int32 * 7)]
+                ("barrier", C.c_int32), ("reserved", C.c_int32 * 6)]
 
 
 class Stats(C.Structure):

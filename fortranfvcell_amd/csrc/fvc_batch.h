@@ -8,7 +8,8 @@
 // state, and runs the unchanged single-system phase functions on pointers
 // offset to its rows.  Groups never wait on each other, so a small system
 // finishes in a few hundred fast block-level iterations while large ones
-// keep their blocks busy; all blocks are co-resident (cooperative launch).
+// keep their blocks busy; all blocks are co-resident (the grid fits the
+// device at once).
 #pragma once
 #include "fvc_mixed.h"
 
@@ -30,6 +31,8 @@ struct BatchArgs {
   const int *block_sys;     // system of each block
   KState *st;               // per system
   unsigned *ctr;            // kCtrWords per system
+  unsigned *flags;          // one arrival flag per block
+  int flagbar;
   double *partials;         // kMaxK per block
   double *x;
   const double *b;
@@ -50,6 +53,8 @@ __device__ __forceinline__ void batch_grid(const BatchArgs &B, int sys, const Sy
   g.partials = B.partials + (size_t)kMaxK * D.g_first;
   g.persistent = true;
   g.base = D.g_first;
+  g.flagbar = B.flagbar;
+  g.flags = B.flags + D.g_first;
 }
 
 __device__ __forceinline__ void batch_mat(const BatchArgs &B, const SysDesc &D, Mat &A) {
@@ -67,12 +72,16 @@ __device__ __forceinline__ void batch_mat(const BatchArgs &B, const SysDesc &D, 
 template <int F>
 __global__ __launch_bounds__(1024) void pcg_batch_persistent(BatchArgs B) {
   __shared__ double lds[kMaxK * kMaxWaves];
+  __shared__ unsigned s_ep;
+  if (threadIdx.x == 0) s_ep = 0u;
+  __syncthreads();
   const int sys = B.block_sys[blockIdx.x];
   const SysDesc D = B.desc[sys];
   const size_t off = (size_t)D.row_off;
   PcgArgs a{};
   batch_mat(B, D, a.A);
   batch_grid(B, sys, D, a.g);
+  a.g.ep = &s_ep;
   a.st = B.st + sys;
   a.m = a.A.ad;
   a.x = B.x + off;
@@ -98,6 +107,9 @@ __global__ __launch_bounds__(1024) void pcg_batch_persistent(BatchArgs B) {
 template <int F>
 __global__ __launch_bounds__(1024) void mixed_batch_persistent(BatchArgs B) {
   __shared__ double lds[kMaxK * kMaxWaves];
+  __shared__ unsigned s_ep;
+  if (threadIdx.x == 0) s_ep = 0u;
+  __syncthreads();
   const int sys = B.block_sys[blockIdx.x];
   const SysDesc D = B.desc[sys];
   const size_t off = (size_t)D.row_off;
@@ -109,6 +121,7 @@ __global__ __launch_bounds__(1024) void mixed_batch_persistent(BatchArgs B) {
   a.A32.ia = nullptr;
   a.A32.ja = a.A.ja;
   batch_grid(B, sys, D, a.g);
+  a.g.ep = &s_ep;
   a.st = B.st + sys;
   a.m = a.A.ad;
   a.s = B.s + off;

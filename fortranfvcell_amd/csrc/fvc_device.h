@@ -179,6 +179,9 @@ struct Grid {
   double *partials;      // kMaxK * G
   bool persistent;
   int base;              // first block of this grid in the launch (batches)
+  int flagbar;           // persistent barrier: 1 = per-block flags, 0 = counters
+  unsigned *flags;       // G arrival flags (flag barrier)
+  unsigned *ep;          // this block's barrier count (LDS), persistent kernels
 
   __device__ __forceinline__ int bid() const { return (int)blockIdx.x - base; }
   // XCD-aware chunk: consecutive block ids go to different XCDs, so give each
@@ -272,6 +275,83 @@ __device__ bool grid_reduce(const Grid &g, double *v, double *lds, Epi &&epi) {
     return s_ok != 0;
   }
   return true;
+}
+
+// Persistent grid reduction with per-block arrival flags: every block stores
+// its partial, releases, and writes its own flag (no atomic read-modify-write
+// to serialise on); block 0 polls all G flags in parallel, acquires, sums the
+// partials in fixed order, runs the epilogue, releases and publishes the
+// generation the others poll.  Same partials, same summation order as
+// grid_reduce, so the results are identical.
+template <int K, class Epi>
+__device__ bool grid_reduce_flags(const Grid &g, double *v, double *lds, Epi &&epi) {
+  __shared__ int s_ok;
+  if constexpr (K > 0) block_sum<K>(v, lds);
+  const unsigned e = *g.ep + 1u;          // barriers passed so far by every block, plus one
+  const int b = g.bid();
+  if (threadIdx.x == 0) {
+    const int c = g.chunk();
+#pragma unroll
+    for (int k = 0; k < K; ++k) g.partials[(size_t)k * g.G + c] = v[k];
+    fence_release();
+    st_relaxed(g.flags + b, e);
+    s_ok = 1;
+  }
+  if (b == 0) {
+    __syncthreads();
+    int ok = 1;
+    for (int q = threadIdx.x; q < g.G; q += blockDim.x) {
+      unsigned spins = 0;
+      while (ld_relaxed(g.flags + q) < e) {
+        __builtin_amdgcn_s_sleep(1);
+        if (++spins > kSpinLimit || ld_relaxed(g.ctr + kCtrErr) != 0u) {
+          st_relaxed(g.ctr + kCtrErr, 1u);
+          ok = 0;
+          break;
+        }
+      }
+      if (!ok) break;
+    }
+    if (!ok) s_ok = 0;
+    __syncthreads();
+    if (threadIdx.x == 0) fence_acquire();
+    __syncthreads();
+    double tot[K > 0 ? K : 1];
+    if constexpr (K > 0) {
+      double w[K];
+      sum_partials<K>(g.partials, g.G, w, lds);
+#pragma unroll
+      for (int k = 0; k < K; ++k) tot[k] = w[k];
+    }
+    if (threadIdx.x == 0) {
+      if (s_ok) epi(tot);
+      fence_release();
+      st_relaxed(g.ctr + kCtrGen, e);
+    }
+  } else if (threadIdx.x == 0) {
+    unsigned spins = 0;
+    while (ld_relaxed(g.ctr + kCtrGen) < e) {
+      __builtin_amdgcn_s_sleep(1);
+      if (++spins > kSpinLimit || ld_relaxed(g.ctr + kCtrErr) != 0u) {
+        st_relaxed(g.ctr + kCtrErr, 1u);
+        s_ok = 0;
+        break;
+      }
+    }
+    fence_acquire();
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) *g.ep = e;
+  const bool ok = s_ok != 0;
+  __syncthreads();
+  return ok;
+}
+
+// The persistent engines' barrier: flags or counters as configured.
+template <int K, class Epi>
+__device__ __forceinline__ bool grid_sync(const Grid &g, double *v, double *lds, Epi &&epi) {
+  if (g.persistent && g.flagbar) return grid_reduce_flags<K>(g, v, lds, epi);
+  return grid_reduce<K>(g, v, lds, epi);
 }
 
 }  // namespace fvc

@@ -96,7 +96,7 @@ __device__ bool pcg_phase(const PcgArgs &a, int ph, double *lds) {
       v[1] += bi * bi;
       v[2] += ri * ri;
     }
-    return grid_reduce<3>(a.g, v, lds, [&](const double *t) {
+    return grid_sync<3>(a.g, v, lds, [&](const double *t) {
       st->d = t[0];
       st->bnorm2 = t[1];
       st->rnorm2 = t[2];
@@ -141,7 +141,7 @@ __device__ bool pcg_phase(const PcgArgs &a, int ph, double *lds) {
         v[0] += pi * qi;
       }
     }
-    return grid_reduce<1>(a.g, v, lds, [&](const double *t) {
+    return grid_sync<1>(a.g, v, lds, [&](const double *t) {
       const double alpha = st->d / t[0];
       st->alpha = alpha;
       if (t[0] == 0.0) { st->status = S_BREAKDOWN; st->done = 1; st->iter = j; }
@@ -164,7 +164,7 @@ __device__ bool pcg_phase(const PcgArgs &a, int ph, double *lds) {
       v[0] += ri * zi;
       v[1] += ri * ri;
     }
-    return grid_reduce<2>(a.g, v, lds, [&](const double *t) {
+    return grid_sync<2>(a.g, v, lds, [&](const double *t) {
       const double beta = t[0] / st->d;
       const double d = beta * st->d;
       st->beta = beta;
@@ -193,7 +193,7 @@ __device__ bool pcg_phase(const PcgArgs &a, int ph, double *lds) {
     v[0] += xi * ax;
     v[1] += ri * ri;
   }
-  return grid_reduce<2>(a.g, v, lds, [&](const double *t) {
+  return grid_sync<2>(a.g, v, lds, [&](const double *t) {
     st->energy = t[0];
     st->res2 = t[1];
   });
@@ -202,6 +202,10 @@ __device__ bool pcg_phase(const PcgArgs &a, int ph, double *lds) {
 template <int F>
 __global__ __launch_bounds__(1024) void pcg_persistent(PcgArgs a) {
   __shared__ double lds[kMaxK * kMaxWaves];
+  __shared__ unsigned s_ep;
+  if (threadIdx.x == 0) s_ep = 0u;
+  __syncthreads();
+  a.g.ep = &s_ep;
   if (!pcg_phase<F>(a, PCG_SETUP, lds)) return;
   while (!ld_int(&a.st->done)) {
     if (!pcg_phase<F>(a, PCG_A, lds)) return;
@@ -289,7 +293,7 @@ __device__ bool bicg_phase(const BicgArgs &a, int ph, double *lds) {
       v[1] += ri * ri;   // = b.r of the first iteration (b = p = r)
       v[2] += bi * bi;
     }
-    return grid_reduce<3>(a.g, v, lds, [&](const double *t) {
+    return grid_sync<3>(a.g, v, lds, [&](const double *t) {
       st->d = t[0];
       st->rr0 = t[1];
       st->rnorm2 = t[1];
@@ -330,7 +334,7 @@ __device__ bool bicg_phase(const BicgArgs &a, int ph, double *lds) {
       vn[i] = vi;
       v[0] += vi * a.r[i];
     }
-    return grid_reduce<1>(a.g, v, lds, [&](const double *t) {
+    return grid_sync<1>(a.g, v, lds, [&](const double *t) {
       const double alpha = st->rr0 / t[0];
       st->alpha = alpha;
       if (t[0] == 0.0) { st->status = S_BREAKDOWN; st->done = 1; st->iter = j; }
@@ -349,7 +353,7 @@ __device__ bool bicg_phase(const BicgArgs &a, int ph, double *lds) {
       v[0] += ti * bi;
       v[1] += ti * ti;
     }
-    return grid_reduce<2>(a.g, v, lds, [&](const double *t) {
+    return grid_sync<2>(a.g, v, lds, [&](const double *t) {
       const double w = t[0] / t[1];
       st->omega = w;
       if (t[1] == 0.0 || w == 0.0) { st->status = S_BREAKDOWN; st->done = 1; st->iter = j; }
@@ -368,7 +372,7 @@ __device__ bool bicg_phase(const BicgArgs &a, int ph, double *lds) {
       v[1] += a.r[i] * bi;
       v[2] += bi * bi;
     }
-    return grid_reduce<3>(a.g, v, lds, [&](const double *t) {
+    return grid_sync<3>(a.g, v, lds, [&](const double *t) {
       st->iter = j;
       st->rnorm2 = t[2];
       const double beta1 = t[0] / st->d;
@@ -396,7 +400,7 @@ __device__ bool bicg_phase(const BicgArgs &a, int ph, double *lds) {
     v[0] += a.x[i] * ax;
     v[1] += ri * ri;
   }
-  return grid_reduce<2>(a.g, v, lds, [&](const double *t) {
+  return grid_sync<2>(a.g, v, lds, [&](const double *t) {
     st->energy = t[0];
     st->res2 = t[1];
   });
@@ -405,6 +409,10 @@ __device__ bool bicg_phase(const BicgArgs &a, int ph, double *lds) {
 template <int F>
 __global__ __launch_bounds__(1024) void bicg_persistent(BicgArgs a) {
   __shared__ double lds[kMaxK * kMaxWaves];
+  __shared__ unsigned s_ep;
+  if (threadIdx.x == 0) s_ep = 0u;
+  __syncthreads();
+  a.g.ep = &s_ep;
   if (!bicg_phase<F>(a, BI_SETUP, lds)) return;
   while (!ld_int(&a.st->done)) {
     if (!bicg_phase<F>(a, BI_F1, lds)) return;
@@ -479,7 +487,7 @@ __device__ bool gmres_phase(const GmresArgs &a, int ph, double *lds) {
       g1[i] = gi;
       v[0] += gi * gi;
     }
-    return grid_reduce<2>(a.g, v, lds, [&](const double *t) {
+    return grid_sync<2>(a.g, v, lds, [&](const double *t) {
       if (first) {
         st->bnorm2 = t[1];
         st->econv = a.tol * sqrt(t[1]);
@@ -525,7 +533,7 @@ __device__ bool gmres_phase(const GmresArgs &a, int ph, double *lds) {
       const double g1r = (i == 1) ? g1[r] / n1 : g1[r];
       v[0] += w * g1r;
     }
-    return grid_reduce<1>(a.g, v, lds, [&](const double *t) {
+    return grid_sync<1>(a.g, v, lds, [&](const double *t) {
       H(1, i) = t[0];
       st->mgs_j = 1;
     });
@@ -547,7 +555,7 @@ __device__ bool gmres_phase(const GmresArgs &a, int ph, double *lds) {
       gn[r] = w;
       v[0] += (j < i) ? w * gj1[r] : w * w;
     }
-    return grid_reduce<1>(a.g, v, lds, [&](const double *t) {
+    return grid_sync<1>(a.g, v, lds, [&](const double *t) {
       if (j < i) {
         H(j + 1, i) = t[0];
         st->mgs_j = j + 1;
@@ -600,7 +608,7 @@ __device__ bool gmres_phase(const GmresArgs &a, int ph, double *lds) {
       for (int j = 1; j <= nI; ++j) xr = xr + ysh[j] * a.gv[(size_t)(j - 1) * a.ld + r];
       a.x[r] = xr;
     }
-    return grid_reduce<0>(a.g, v, lds, [&](const double *) {
+    return grid_sync<0>(a.g, v, lds, [&](const double *) {
       const double en = a.e[nI + 1];
       st->last = fabs(en);
       st->iter = st->cycle;
@@ -619,7 +627,7 @@ __device__ bool gmres_phase(const GmresArgs &a, int ph, double *lds) {
     v[0] += a.x[i] * bi;
     v[1] += ri * ri;
   }
-  return grid_reduce<2>(a.g, v, lds, [&](const double *t) {
+  return grid_sync<2>(a.g, v, lds, [&](const double *t) {
     st->energy = t[0];
     st->res2 = t[1];
   });
@@ -628,6 +636,10 @@ __device__ bool gmres_phase(const GmresArgs &a, int ph, double *lds) {
 template <int F>
 __global__ __launch_bounds__(1024) void gmres_persistent(GmresArgs a) {
   __shared__ double lds[kMaxK * kMaxWaves];
+  __shared__ unsigned s_ep;
+  if (threadIdx.x == 0) s_ep = 0u;
+  __syncthreads();
+  a.g.ep = &s_ep;
   while (!ld_int(&a.st->done)) {
     if (!gmres_phase<F>(a, GM_START, lds)) return;
     if (ld_int(&a.st->done)) break;

@@ -132,7 +132,7 @@ __device__ bool mixed_phase(const MixedArgs &a, int ph, double *lds) {
       v[1] += ri * ri;
       v[2] += bi * bi;
     }
-    return grid_reduce<3>(a.g, v, lds, [&](const double *t) {
+    return grid_sync<3>(a.g, v, lds, [&](const double *t) {
       if (ph == MX_SETUP) {
         st->bnorm2 = t[2];
         st->conv = a.tol * sqrt(t[2]);
@@ -178,7 +178,7 @@ __device__ bool mixed_phase(const MixedArgs &a, int ph, double *lds) {
       a.q[i] = qi;
       v[0] += (double)pi * (double)qi;
     }
-    return grid_reduce<1>(a.g, v, lds, [&](const double *t) {
+    return grid_sync<1>(a.g, v, lds, [&](const double *t) {
       st->alpha = st->d / t[0];
       if (t[0] == 0.0) { st->status = S_BREAKDOWN; st->done = 1; }
       else if (!finite(st->alpha)) { st->status = S_NONFINITE; st->done = 1; }
@@ -199,7 +199,7 @@ __device__ bool mixed_phase(const MixedArgs &a, int ph, double *lds) {
       const double ru = (double)ri * (double)a.dsq[i];
       v[1] += ru * ru;
     }
-    return grid_reduce<2>(a.g, v, lds, [&](const double *t) {
+    return grid_sync<2>(a.g, v, lds, [&](const double *t) {
       st->iter = j;
       st->inner = st->inner + 1;
       const double est = sqrt(t[1]);
@@ -217,7 +217,7 @@ __device__ bool mixed_phase(const MixedArgs &a, int ph, double *lds) {
     // fold the FP32 increment into the FP64 solution: x += S y
     for (int i = i0 + (int)threadIdx.x; i < i1; i += blockDim.x) a.x[i] = a.x[i] + a.s[i] * (double)a.y[i];
     double v[1] = {0.0};
-    return grid_reduce<0>(a.g, v, lds, [&](const double *) {});
+    return grid_sync<0>(a.g, v, lds, [&](const double *) {});
   }
   // MX_FINAL: energy x.Ax and ||b - Ax||^2 in FP64
   double v[2] = {0.0, 0.0};
@@ -227,7 +227,7 @@ __device__ bool mixed_phase(const MixedArgs &a, int ph, double *lds) {
     v[0] += a.x[i] * ax;
     v[1] += ri * ri;
   }
-  return grid_reduce<2>(a.g, v, lds, [&](const double *t) {
+  return grid_sync<2>(a.g, v, lds, [&](const double *t) {
     st->energy = t[0];
     st->res2 = t[1];
   });
@@ -236,6 +236,10 @@ __device__ bool mixed_phase(const MixedArgs &a, int ph, double *lds) {
 template <int F>
 __global__ __launch_bounds__(1024) void mixed_persistent(MixedArgs a) {
   __shared__ double lds[kMaxK * kMaxWaves];
+  __shared__ unsigned s_ep;
+  if (threadIdx.x == 0) s_ep = 0u;
+  __syncthreads();
+  a.g.ep = &s_ep;
   if (!mixed_phase<F>(a, MX_SETUP, lds)) return;
   while (!ld_int(&a.st->done)) {
     if (!mixed_phase<F>(a, MX_A, lds)) return;

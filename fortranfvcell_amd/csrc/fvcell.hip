@@ -168,7 +168,8 @@ int occupancy(K kernel, int bs, int *nb) {
   return FVC_OK;
 }
 
-Grid make_grid(int neq, int G, unsigned *ctr, double *partials, bool persistent) {
+// ctr must hold kCtrWords counters followed by G arrival flags.
+Grid make_grid(int neq, int G, unsigned *ctr, double *partials, bool persistent, int barrier = 0) {
   Grid g;
   g.G = G;
   g.neq = neq;
@@ -177,6 +178,9 @@ Grid make_grid(int neq, int G, unsigned *ctr, double *partials, bool persistent)
   g.partials = partials;
   g.persistent = persistent;
   g.base = 0;
+  g.flagbar = barrier == 0 ? 1 : 0;
+  g.flags = ctr + kCtrWords;
+  g.ep = nullptr;
   return g;
 }
 
@@ -196,7 +200,7 @@ struct SolveIn {
   double *x;
   const double *b;
   double tol;
-  int maxit, x0_zero, stop_rule, gmres_k, engine, bs, G_req, precision;
+  int maxit, x0_zero, stop_rule, gmres_k, engine, bs, G_req, precision, barrier;
 };
 
 // Persistent launches are ordinary launches whose grid fits the device at
@@ -262,13 +266,13 @@ int run_solve(DevState &d, const SolveIn &in, fvc_stats *stats, hipStream_t s) {
   // workspace
   const size_t n = (size_t)neq;
   int nvec = in.code == FVC_PCG ? 5 : in.code == FVC_BICGSTAB ? 10 : (k + 1);
-  size_t bytes = align_up(kCtrWords * sizeof(unsigned)) + align_up(sizeof(KState)) +
+  size_t bytes = align_up((kCtrWords + (size_t)G) * sizeof(unsigned)) + align_up(sizeof(KState)) +
                  align_up(kMaxK * (size_t)G * sizeof(double)) + (size_t)nvec * align_up(n * sizeof(double)) +
                  align_up((size_t)(k + 2) * (k + 2) * sizeof(double)) * 2 + align_up(n * sizeof(double)) + 4096;
   rc = ensure_arena(d, bytes);
   if (rc) return rc;
   Carve cv{d.arena};
-  unsigned *ctr = cv.take<unsigned>(kCtrWords);
+  unsigned *ctr = cv.take<unsigned>(kCtrWords + (size_t)G);
   KState *st = cv.take<KState>(1);
   double *partials = cv.take<double>(kMaxK * (size_t)G);
   std::vector<double *> vec(nvec);
@@ -279,9 +283,9 @@ int run_solve(DevState &d, const SolveIn &in, fvc_stats *stats, hipStream_t s) {
   if (may_fall_back && !in.x0_zero)
     HIPCHK(hipMemcpyAsync(xbackup, in.x, n * sizeof(double), hipMemcpyDeviceToDevice, s));
 
-  Grid g = make_grid(neq, G, ctr, partials, persistent);
+  Grid g = make_grid(neq, G, ctr, partials, persistent, in.barrier);
   HIPCHK(hipEventRecord(d.ev0, s));
-  hipLaunchKernelGGL(reset_kernel, dim3(1), dim3(256), 0, s, ctr, (int)kCtrWords, st);
+  hipLaunchKernelGGL(reset_kernel, dim3(1), dim3(256), 0, s, ctr, (int)kCtrWords + G, st, 1);
   HIPCHK(hipGetLastError());
 
   if (in.code == FVC_PCG) {
@@ -454,12 +458,12 @@ int run_mixed(DevState &d, const SolveIn &in, fvc_stats *stats, hipStream_t s) {
       HIPCHK(hipMemcpy(&last, in.A.ia + neq, sizeof(int), hipMemcpyDeviceToHost));
       nv = (size_t)std::max(last - 1, 0);
     }
-    size_t bytes = align_up(kCtrWords * sizeof(unsigned)) + align_up(sizeof(KState)) +
+    size_t bytes = align_up((kCtrWords + (size_t)G) * sizeof(unsigned)) + align_up(sizeof(KState)) +
                    align_up(kMaxK * (size_t)G * sizeof(double)) + align_up(nv * sizeof(float)) +
                    align_up(n * sizeof(double)) + 6 * align_up(n * sizeof(float)) + 4096;
     if ((rc = ensure_arena(d, bytes))) return rc;
     Carve cv{d.arena};
-    unsigned *ctr = cv.take<unsigned>(kCtrWords);
+    unsigned *ctr = cv.take<unsigned>(kCtrWords + (size_t)G);
     KState *st = cv.take<KState>(1);
     double *partials = cv.take<double>(kMaxK * (size_t)G);
     float *av32 = cv.take<float>(std::max<size_t>(nv, 1));
@@ -477,7 +481,7 @@ int run_mixed(DevState &d, const SolveIn &in, fvc_stats *stats, hipStream_t s) {
     a.A32.av = av32;
     a.A32.ia = in.A.ia;
     a.A32.ja = in.A.ja;
-    a.g = make_grid(neq, G, ctr, partials, persistent);
+    a.g = make_grid(neq, G, ctr, partials, persistent, in.barrier);
     a.st = st;
     a.m = in.m;
     a.s = sc;
@@ -490,7 +494,7 @@ int run_mixed(DevState &d, const SolveIn &in, fvc_stats *stats, hipStream_t s) {
     a.x0_zero = in.x0_zero;
     a.max_inner = 1000;
     HIPCHK(hipEventRecord(d.ev0, s));
-    hipLaunchKernelGGL(reset_kernel, dim3(1), dim3(256), 0, s, ctr, (int)kCtrWords, st);
+    hipLaunchKernelGGL(reset_kernel, dim3(1), dim3(256), 0, s, ctr, (int)kCtrWords + G, st, 1);
     hipLaunchKernelGGL(scale_to_f32_kernel<F>, dim3((neq + 255) / 256), dim3(256), 0, s, in.A, in.m,
                        av32, sc, dsq);
     HIPCHK(hipGetLastError());
@@ -748,6 +752,7 @@ int solve_impl(int code, int matriz, int unsym, int neq, int nad, const double *
   in.bs = o.block_size;
   in.G_req = o.grid_blocks;
   in.precision = o.precision;
+  in.barrier = o.barrier;
   if (code == FVC_GMRES && o.stop_rule != FVC_STOP_REFERENCE)
     return fail(FVC_UNSUPPORTED, "GMRES supports only the reference stopping rule");
   rc = dispatch_solve(*d, in, stats, s);
@@ -816,11 +821,11 @@ int dot_impl(const double *a, const double *b, int n, double *result, bool on_de
   }
   const int bs = kDefaultBlock;
   const int G = choose_blocks(n, d->cus, 2, 0);
-  size_t bytes = align_up(kCtrWords * sizeof(unsigned)) + align_up(sizeof(KState)) +
+  size_t bytes = align_up((kCtrWords + (size_t)G) * sizeof(unsigned)) + align_up(sizeof(KState)) +
                  align_up(kMaxK * (size_t)G * sizeof(double)) + 4096;
   if ((rc = ensure_arena(*d, bytes))) return rc;
   Carve cv{d->arena};
-  unsigned *ctr = cv.take<unsigned>(kCtrWords);
+  unsigned *ctr = cv.take<unsigned>(kCtrWords + (size_t)G);
   KState *st = cv.take<KState>(1);
   double *partials = cv.take<double>(kMaxK * (size_t)G);
   double *out = cv.take<double>(1);
@@ -877,7 +882,7 @@ int run_batch(DevState &d, int nsys, const std::vector<int> &row_off, int w, con
   const int wave = std::max(1, std::min(nsys, cap / 2 > 0 ? cap / 2 : 1));
   const double expo = o.batch_work_exp10 > 0 ? o.batch_work_exp10 / 10.0 : 1.5;
   const size_t n = (size_t)total;
-  size_t bytes = align_up((size_t)wave * kCtrWords * sizeof(unsigned)) + align_up((size_t)wave * sizeof(KState)) +
+  size_t bytes = align_up(((size_t)wave * kCtrWords + cap) * sizeof(unsigned)) + align_up((size_t)wave * sizeof(KState)) +
                  align_up((size_t)kMaxK * cap * sizeof(double)) + align_up((size_t)wave * sizeof(SysDesc)) +
                  align_up((size_t)cap * sizeof(int)) + 4096;
   if (mixed) bytes += 5 * align_up(n * sizeof(float)) + align_up(n * w * sizeof(float)) +
@@ -885,7 +890,7 @@ int run_batch(DevState &d, int nsys, const std::vector<int> &row_off, int w, con
   else bytes += 5 * align_up(n * sizeof(double));
   if ((rc = ensure_arena(d, bytes))) return rc;
   Carve cv{d.arena};
-  unsigned *ctr = cv.take<unsigned>((size_t)wave * kCtrWords);
+  unsigned *ctr = cv.take<unsigned>((size_t)wave * kCtrWords + (size_t)cap);
   KState *st = cv.take<KState>(wave);
   double *partials = cv.take<double>((size_t)kMaxK * cap);
   SysDesc *ddesc = cv.take<SysDesc>(wave);
@@ -913,6 +918,8 @@ int run_batch(DevState &d, int nsys, const std::vector<int> &row_off, int w, con
     B.v3 = cv.take<double>(n); B.v4 = cv.take<double>(n);
   }
   B.ctr = ctr;
+  B.flags = ctr + (size_t)wave * kCtrWords;
+  B.flagbar = o.barrier == 0 ? 1 : 0;
   B.st = st;
   B.partials = partials;
   B.desc = ddesc;
@@ -939,7 +946,7 @@ int run_batch(DevState &d, int nsys, const std::vector<int> &row_off, int w, con
     const int G = first;
     HIPCHK(hipMemcpyAsync(ddesc, desc.data(), m * sizeof(SysDesc), hipMemcpyHostToDevice, s));
     HIPCHK(hipMemcpyAsync(dbsys, bsys.data(), G * sizeof(int), hipMemcpyHostToDevice, s));
-    hipLaunchKernelGGL(reset_kernel, dim3(1), dim3(256), 0, s, ctr, m * kCtrWords, st, m);
+    hipLaunchKernelGGL(reset_kernel, dim3(1), dim3(256), 0, s, ctr, wave * kCtrWords + cap, st, m);
     HIPCHK(hipGetLastError());
     if ((rc = launch(fn, true, G, bs, B, s))) return rc;
     HIPCHK(hipMemcpyAsync(host_st.data() + s0, st, m * sizeof(KState), hipMemcpyDeviceToHost, s));

@@ -85,7 +85,9 @@ typedef struct fvc_options {
   int32_t grid_blocks;    /* 0 = tuned default (number of row chunks)                */
   int32_t precision;      /* FVC_PREC_*; mixed applies to PCG on ELLPACK / CSRD      */
   int32_t batch_work_exp10; /* batches: blocks per system ~ neq^(this/10); 0 = 15     */
-  int32_t reserved[7];
+  int32_t barrier;        /* persistent engine's grid barrier: 0 per-block flags,
+                             1 arrival counters                                     */
+  int32_t reserved[6];
 } fvc_options;
 
 typedef struct fvc_stats {
