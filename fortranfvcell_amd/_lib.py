@@ -21,6 +21,7 @@ X0_AS_OPENMP, X0_AS_SEQUENTIAL, X0_ZERO, X0_GIVEN = 0, 1, 2, 3
 STOP_REFERENCE, STOP_RELRES = 0, 1
 ENGINE_AUTO, ENGINE_KERNELS, ENGINE_PERSISTENT = 0, 1, 2
 PREC_FP64, PREC_MIXED = 0, 1
+REC_FUSED, REC_REFERENCE = 0, 1
 
 # Every symbol include/fvcell.h declares.
 EXPORTS = [
@@ -37,7 +38,7 @@ class Options(C.Structure):
     _fields_ = [("x0_rule", C.c_int32), ("gmres_k", C.c_int32), ("stop_rule", C.c_int32),
                 ("true_residual", C.c_int32), ("engine", C.c_int32), ("block_size", C.c_int32),
                 ("grid_blocks", C.c_int32), ("precision", C.c_int32), ("batch_work_exp10", C.c_int32),
-                ("barrier", C.c_int32), ("reserved", C.c_int32 * 6)]
+                ("barrier", C.c_int32), ("recurrence", C.c_int32), ("reserved", C.c_int32 * 5)]
 
 
 class Stats(C.Structure):

@@ -216,7 +216,10 @@ __device__ bool grid_reduce(const Grid &g, double *v, double *lds, Epi &&epi) {
   __shared__ int s_last;
   __shared__ unsigned s_gen;
   __shared__ int s_ok;
+  // every wave's vector writes precede thread 0's release (block_sum's
+  // workgroup barrier does it when there is something to sum)
   if constexpr (K > 0) block_sum<K>(v, lds);
+  else __syncthreads();
   if (threadIdx.x == 0) {
     const int c = g.chunk();
 #pragma unroll
@@ -287,6 +290,7 @@ template <int K, class Epi>
 __device__ bool grid_reduce_flags(const Grid &g, double *v, double *lds, Epi &&epi) {
   __shared__ int s_ok;
   if constexpr (K > 0) block_sum<K>(v, lds);
+  else __syncthreads();   // as in grid_reduce: all waves' writes before the release
   const unsigned e = *g.ep + 1u;          // barriers passed so far by every block, plus one
   const int b = g.bid();
   if (threadIdx.x == 0) {

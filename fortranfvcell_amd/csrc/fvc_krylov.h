@@ -222,6 +222,189 @@ __global__ __launch_bounds__(1024) void pcg_kernel(PcgArgs a) {
   pcg_phase<F>(a, a.phase, lds);
 }
 
+// ------------------------------------------------- PCG, one reduction --
+//
+// The same Jacobi-PCG iterates with the single-reduction recurrence of
+// Chronopoulos & Gear (1989).  With u = M^-1 r, w = A u and s = A p:
+//   p = u + beta p,  s = w + beta s,  x += alpha p,  r -= alpha s,
+//   u' = M^-1 r,  w' = A u',  gamma = r.u',  delta = w'.u',
+//   beta' = gamma / gamma_old,  alpha' = gamma / (delta - beta' gamma / alpha).
+// gamma is the reference's d = r.z (src/Iterativos.f:71-74), so the stop test
+// and iteration count are the reference's; p.Ap is never formed, so one grid
+// reduction per iteration suffices.  The SpMV operand u' at the neighbours is
+// formed on the fly from r, w and the previous s there; those three vectors
+// alternate between two buffers (odd iterations read set 0 and write set 1).
+enum { CG1_SETUP = 0, CG1_W0 = 1, CG1_ITER = 2, CG1_FINAL = 3 };
+
+struct Cg1Args {
+  Mat A;
+  Grid g;
+  KState *st;
+  const double *m;
+  double *x;
+  const double *b;
+  double *r0, *r1, *w0, *w1, *s0, *s1, *p;
+  double tol;
+  int maxit, x0_zero, stop_rule, phase;
+};
+
+// u' at row j: (r_j - alpha (w_j + beta s_j)) / m_j  (s = w in iteration 1)
+struct XCgNext {
+  const double *r, *w, *s, *m;
+  double alpha, beta;
+  int first;
+  __device__ __forceinline__ double operator()(int j) const {
+    const double sj = first ? w[j] : w[j] + beta * s[j];
+    return (r[j] - alpha * sj) / m[j];
+  }
+};
+struct XDiv {
+  const double *r, *m;
+  __device__ __forceinline__ double operator()(int j) const { return r[j] / m[j]; }
+};
+
+template <int F>
+__device__ bool cg1_phase(const Cg1Args &a, int ph, double *lds) {
+  const int i0 = a.g.row_begin(), i1 = a.g.row_end();
+  KState *st = a.st;
+  if (ph == CG1_SETUP) {
+    // r = b - A x (or r = b for x0 = 0); d = r.M^-1 r; conv as the reference
+    double v[3] = {0.0, 0.0, 0.0};
+    for (int i = i0 + (int)threadIdx.x; i < i1; i += blockDim.x) {
+      double ax;
+      if (a.x0_zero) {
+        a.x[i] = 0.0;
+        ax = 0.0;
+      } else {
+        ax = row_apply<F>(a.A, i, XVec{a.x});
+      }
+      const double bi = a.b[i];
+      const double ri = bi - ax;
+      a.r0[i] = ri;
+      v[0] += ri * (ri / a.m[i]);
+      v[1] += bi * bi;
+      v[2] += ri * ri;
+    }
+    return grid_sync<3>(a.g, v, lds, [&](const double *t) {
+      st->d = t[0];
+      st->bnorm2 = t[1];
+      st->rnorm2 = t[2];
+      st->iter = 0;
+      st->inner = 0;
+      st->status = S_MAXIT;
+      st->done = 0;
+      st->beta = 0.0;
+      if (a.stop_rule == 1) {
+        st->conv = a.tol * sqrt(t[1]);
+        st->last = sqrt(t[2]);
+        if (st->last <= st->conv) { st->status = S_OK; st->done = 1; }
+      } else {
+        st->conv = a.tol * sqrt(fabs(t[0]));
+        st->last = sqrt(fabs(t[0]));
+      }
+      if (!finite(t[0])) { st->status = S_NONFINITE; st->done = 1; }
+      if (a.maxit <= 0 && !st->done) st->done = 1;
+    });
+  }
+  if (ph == CG1_W0) {
+    // w = A u, u = r/m on the fly; alpha = d / w.u
+    const XDiv xf{a.r0, a.m};
+    double v[1] = {0.0};
+    for (int i = i0 + (int)threadIdx.x; i < i1; i += blockDim.x) {
+      const double wi = row_apply<F>(a.A, i, xf);
+      a.w0[i] = wi;
+      v[0] += wi * xf(i);
+    }
+    return grid_sync<1>(a.g, v, lds, [&](const double *t) {
+      st->alpha = st->d / t[0];
+      if (t[0] == 0.0) { st->status = S_BREAKDOWN; st->done = 1; }
+      else if (!finite(st->alpha)) { st->status = S_NONFINITE; st->done = 1; }
+    });
+  }
+  if (ph == CG1_ITER) {
+    const int j = ld_int(&st->iter) + 1;
+    const bool odd = (j & 1) != 0;
+    const double *rc = odd ? a.r0 : a.r1, *wc = odd ? a.w0 : a.w1, *sc = odd ? a.s0 : a.s1;
+    double *rn = odd ? a.r1 : a.r0, *wn = odd ? a.w1 : a.w0, *sn = odd ? a.s1 : a.s0;
+    const XCgNext xf{rc, wc, sc, a.m, ld_dbl(&st->alpha), ld_dbl(&st->beta), j == 1};
+    double v[3] = {0.0, 0.0, 0.0};
+    for (int i = i0 + (int)threadIdx.x; i < i1; i += blockDim.x) {
+      const double mi = a.m[i];
+      const double ui = rc[i] / mi;
+      const double pi = xf.first ? ui : ui + xf.beta * a.p[i];
+      const double si = xf.first ? wc[i] : wc[i] + xf.beta * sc[i];
+      a.x[i] = a.x[i] + xf.alpha * pi;
+      const double ri = rc[i] - xf.alpha * si;
+      const double un = ri / mi;
+      const double wi = row_apply<F>(a.A, i, xf);
+      a.p[i] = pi;
+      sn[i] = si;
+      rn[i] = ri;
+      wn[i] = wi;
+      v[0] += ri * un;
+      v[1] += wi * un;
+      v[2] += ri * ri;
+    }
+    return grid_sync<3>(a.g, v, lds, [&](const double *t) {
+      const double gamma = t[0];
+      const double beta = gamma / st->d;
+      const double den = t[1] - beta * gamma / st->alpha;
+      const double alpha = gamma / den;
+      st->beta = beta;
+      st->alpha = alpha;
+      st->d = gamma;
+      st->rnorm2 = t[2];
+      st->iter = j;
+      bool conv;
+      if (a.stop_rule == 1) {
+        st->last = sqrt(t[2]);
+        conv = st->last <= st->conv;
+      } else {
+        st->last = sqrt(fabs(gamma));
+        conv = st->last < st->conv;   // src/Iterativos.f:73
+      }
+      if (conv) { st->status = S_OK; st->done = 1; }
+      else if (!finite(gamma) || !finite(beta)) { st->status = S_NONFINITE; st->done = 1; }
+      else if (den == 0.0) { st->status = S_BREAKDOWN; st->done = 1; }
+      else if (!finite(alpha)) { st->status = S_NONFINITE; st->done = 1; }
+      else if (j >= a.maxit) { st->status = S_MAXIT; st->done = 1; }
+    });
+  }
+  // CG1_FINAL: energy norm x.Ax (src/Iterativos.f:82-83) and ||b - Ax||^2
+  double v[2] = {0.0, 0.0};
+  for (int i = i0 + (int)threadIdx.x; i < i1; i += blockDim.x) {
+    const double ax = row_apply<F>(a.A, i, XVec{a.x});
+    const double ri = a.b[i] - ax;
+    v[0] += a.x[i] * ax;
+    v[1] += ri * ri;
+  }
+  return grid_sync<2>(a.g, v, lds, [&](const double *t) {
+    st->energy = t[0];
+    st->res2 = t[1];
+  });
+}
+
+template <int F>
+__global__ __launch_bounds__(1024) void cg1_persistent(Cg1Args a) {
+  __shared__ double lds[kMaxK * kMaxWaves];
+  __shared__ unsigned s_ep;
+  if (threadIdx.x == 0) s_ep = 0u;
+  __syncthreads();
+  a.g.ep = &s_ep;
+  if (!cg1_phase<F>(a, CG1_SETUP, lds)) return;
+  if (!ld_int(&a.st->done) && !cg1_phase<F>(a, CG1_W0, lds)) return;
+  while (!ld_int(&a.st->done))
+    if (!cg1_phase<F>(a, CG1_ITER, lds)) return;
+  cg1_phase<F>(a, CG1_FINAL, lds);
+}
+
+template <int F>
+__global__ __launch_bounds__(1024) void cg1_kernel(Cg1Args a) {
+  __shared__ double lds[kMaxK * kMaxWaves];
+  if ((a.phase == CG1_W0 || a.phase == CG1_ITER) && ld_int(&a.st->done)) return;
+  cg1_phase<F>(a, a.phase, lds);
+}
+
 // ------------------------------------------------------------- BiCGSTAB --
 //
 // src/Iterativos.f:140-234 / src/Iterativos_omp.f:161-274 in three phases per
@@ -429,6 +612,212 @@ __global__ __launch_bounds__(1024) void bicg_kernel(BicgArgs a) {
   __shared__ double lds[kMaxK * kMaxWaves];
   if (a.phase != BI_SETUP && a.phase != BI_FINAL && ld_int(&a.st->done)) return;
   bicg_phase<F>(a, a.phase, lds);
+}
+
+// ------------------------------------------------ BiCGSTAB, two phases --
+//
+// The reference's BiCGSTAB (src/Iterativos.f:140-234) with two grid
+// reductions per iteration instead of three.  Iteration j:
+//   P(j)  completes iteration j-1 for the own rows -- b = s - w t and
+//         x = (x + alpha p/m) + w s/m -- then forms p = b + beta (p - w v)
+//         and v = A (p/m), gathering p/m of the neighbours from s, t, p, v;
+//         reduces rhat.v (alpha of j) and, exactly, the quantities of
+//         iteration j-1's stop test: b.(s/m) (src/Iterativos.f:188) and b.b.
+//   Q(j)  s = b - alpha v, t = A (s/m) (s/m of the neighbours on the fly);
+//         reduces t.s, t.t (omega, src/Iterativos.f:180-181) and rhat.s,
+//         rhat.t, which give rhat.b_new = rhat.s - omega rhat.t: the
+//         reference's dot(r, b) (src/Iterativos.f:192) without a third pass.
+// The stop test of iteration j-1 is decided in P(j)'s epilogue, one phase
+// later than the reference, with the same exact dot products; x is then
+// x_(j-1) and the extra direction P(j) built is dropped.  p and v alternate
+// between two buffers (odd iterations read set 0 and write set 1).
+enum { B2_SETUP = 0, B2_P = 1, B2_Q = 2, B2_FINAL = 3 };
+
+struct Bicg2Args {
+  Mat A;
+  Grid g;
+  KState *st;
+  const double *m;
+  double *x;
+  const double *b;
+  double *bb;               // the reference's running residual b (own rows)
+  double *p0, *p1, *v0, *v1;
+  double *s, *t, *r;        // half-step residual, A M^-1 s, shadow residual
+  double tol;
+  int maxit, x0_zero, stop_rule, phase;
+};
+
+// p/m at row j for P(j), j > 1: ((s - w t) + beta (p - w v)) / m
+struct XBi2Dir {
+  const double *s, *t, *p, *v, *m;
+  double beta, w;
+  __device__ __forceinline__ double operator()(int j) const {
+    return ((s[j] - w * t[j]) + beta * (p[j] - w * v[j])) / m[j];
+  }
+};
+
+template <int F>
+__device__ bool bicg2_phase(const Bicg2Args &a, int ph, double *lds) {
+  const int i0 = a.g.row_begin(), i1 = a.g.row_end();
+  KState *st = a.st;
+  if (ph == B2_SETUP) {
+    // r = b - Ax; rhat = r; p = r; b = r; d = r.(r/m); rho = b.rhat = r.r
+    double v[3] = {0.0, 0.0, 0.0};
+    for (int i = i0 + (int)threadIdx.x; i < i1; i += blockDim.x) {
+      double ax;
+      if (a.x0_zero) {
+        a.x[i] = 0.0;
+        ax = 0.0;
+      } else {
+        ax = row_apply<F>(a.A, i, XVec{a.x});
+      }
+      const double bi = a.b[i];
+      const double ri = bi - ax;
+      a.r[i] = ri;
+      a.p0[i] = ri;
+      a.bb[i] = ri;
+      v[0] += ri * (ri / a.m[i]);
+      v[1] += ri * ri;
+      v[2] += bi * bi;
+    }
+    return grid_sync<3>(a.g, v, lds, [&](const double *t) {
+      st->d = t[0];
+      st->rr0 = t[1];
+      st->rnorm2 = t[1];
+      st->bnorm2 = t[2];
+      st->iter = 0;
+      st->inner = 0;
+      st->status = S_MAXIT;
+      st->done = 0;
+      if (a.stop_rule == 1) {
+        st->conv = a.tol * sqrt(t[2]);
+        st->last = sqrt(t[1]);
+        if (st->last <= st->conv) { st->status = S_OK; st->done = 1; }
+      } else {
+        st->conv = a.tol * sqrt(fabs(t[0]));
+        st->last = sqrt(fabs(t[0]));
+      }
+      if (!finite(t[0])) { st->status = S_NONFINITE; st->done = 1; }
+      if (a.maxit <= 0 && !st->done) st->done = 1;
+    });
+  }
+  const int j = ld_int(&st->iter) + 1;
+  const bool odd = (j & 1) != 0;
+  const double *pc = odd ? a.p0 : a.p1, *vc = odd ? a.v0 : a.v1;
+  double *pn = odd ? a.p1 : a.p0, *vn = odd ? a.v1 : a.v0;
+  if (ph == B2_P) {
+    double v[3] = {0.0, 0.0, 0.0};
+    if (j == 1) {
+      const XDiv xf{pc, a.m};
+      for (int i = i0 + (int)threadIdx.x; i < i1; i += blockDim.x) {
+        const double vi = row_apply<F>(a.A, i, xf);
+        pn[i] = pc[i];
+        vn[i] = vi;
+        v[0] += vi * a.r[i];
+      }
+    } else {
+      const double alpha = ld_dbl(&st->alpha);
+      const XBi2Dir xf{a.s, a.t, pc, vc, a.m, ld_dbl(&st->beta), ld_dbl(&st->omega)};
+      for (int i = i0 + (int)threadIdx.x; i < i1; i += blockDim.x) {
+        const double mi = a.m[i];
+        const double si = a.s[i];
+        const double z2i = si / mi;
+        const double bi = si - xf.w * a.t[i];
+        a.x[i] = (a.x[i] + alpha * (pc[i] / mi)) + xf.w * z2i;
+        a.bb[i] = bi;
+        const double pi = bi + xf.beta * (pc[i] - xf.w * vc[i]);
+        const double vi = row_apply<F>(a.A, i, xf);
+        pn[i] = pi;
+        vn[i] = vi;
+        v[0] += vi * a.r[i];
+        v[1] += bi * z2i;
+        v[2] += bi * bi;
+      }
+    }
+    return grid_sync<3>(a.g, v, lds, [&](const double *t) {
+      if (j > 1) {
+        // the stop test of iteration j-1 (src/Iterativos.f:188-190)
+        const double beta1 = t[1] / st->d;
+        st->rnorm2 = t[2];
+        bool conv;
+        if (a.stop_rule == 1) {
+          st->last = sqrt(t[2]);
+          conv = st->last <= st->conv;
+        } else {
+          st->last = sqrt(fabs(beta1));
+          conv = st->last < st->conv;
+        }
+        if (conv) { st->status = S_OK; st->done = 1; return; }
+        if (!finite(beta1)) { st->status = S_NONFINITE; st->done = 1; return; }
+        if (j - 1 >= a.maxit) { st->status = S_MAXIT; st->done = 1; return; }
+      }
+      const double alpha = st->rr0 / t[0];
+      st->alpha = alpha;
+      if (t[0] == 0.0) { st->status = S_BREAKDOWN; st->done = 1; }
+      else if (!finite(alpha)) { st->status = S_NONFINITE; st->done = 1; }
+    });
+  }
+  if (ph == B2_Q) {
+    const XBiRes xf{a.bb, vn, a.m, ld_dbl(&st->alpha)};
+    double v[4] = {0.0, 0.0, 0.0, 0.0};
+    for (int i = i0 + (int)threadIdx.x; i < i1; i += blockDim.x) {
+      const double si = a.bb[i] - xf.alpha * vn[i];
+      const double ti = row_apply<F>(a.A, i, xf);
+      const double ri = a.r[i];
+      a.s[i] = si;
+      a.t[i] = ti;
+      v[0] += ti * si;
+      v[1] += ti * ti;
+      v[2] += ri * si;
+      v[3] += ri * ti;
+    }
+    return grid_sync<4>(a.g, v, lds, [&](const double *t) {
+      const double w = t[0] / t[1];
+      const double rho = t[2] - w * t[3];
+      const double beta = (rho / st->rr0) * (st->alpha / w);
+      st->omega = w;
+      st->beta = beta;
+      st->rr0 = rho;
+      st->iter = j;
+      if (t[1] == 0.0 || w == 0.0) { st->status = S_BREAKDOWN; st->done = 1; }
+      else if (!finite(w) || !finite(beta)) { st->status = S_NONFINITE; st->done = 1; }
+    });
+  }
+  // B2_FINAL: energy x.Ax (src/Iterativos.f:204-205) and ||b - Ax||^2
+  double v[2] = {0.0, 0.0};
+  for (int i = i0 + (int)threadIdx.x; i < i1; i += blockDim.x) {
+    const double ax = row_apply<F>(a.A, i, XVec{a.x});
+    const double ri = a.b[i] - ax;
+    v[0] += a.x[i] * ax;
+    v[1] += ri * ri;
+  }
+  return grid_sync<2>(a.g, v, lds, [&](const double *t) {
+    st->energy = t[0];
+    st->res2 = t[1];
+  });
+}
+
+template <int F>
+__global__ __launch_bounds__(1024) void bicg2_persistent(Bicg2Args a) {
+  __shared__ double lds[kMaxK * kMaxWaves];
+  __shared__ unsigned s_ep;
+  if (threadIdx.x == 0) s_ep = 0u;
+  __syncthreads();
+  a.g.ep = &s_ep;
+  if (!bicg2_phase<F>(a, B2_SETUP, lds)) return;
+  while (!ld_int(&a.st->done)) {
+    if (!bicg2_phase<F>(a, B2_P, lds)) return;
+    if (ld_int(&a.st->done)) break;
+    if (!bicg2_phase<F>(a, B2_Q, lds)) return;
+  }
+  bicg2_phase<F>(a, B2_FINAL, lds);
+}
+
+template <int F>
+__global__ __launch_bounds__(1024) void bicg2_kernel(Bicg2Args a) {
+  __shared__ double lds[kMaxK * kMaxWaves];
+  if ((a.phase == B2_P || a.phase == B2_Q) && ld_int(&a.st->done)) return;
+  bicg2_phase<F>(a, a.phase, lds);
 }
 
 // ---------------------------------------------------------------- GMRES --

@@ -200,7 +200,7 @@ struct SolveIn {
   double *x;
   const double *b;
   double tol;
-  int maxit, x0_zero, stop_rule, gmres_k, engine, bs, G_req, precision, barrier;
+  int maxit, x0_zero, stop_rule, gmres_k, engine, bs, G_req, precision, barrier, recurrence;
 };
 
 // Persistent launches are ordinary launches whose grid fits the device at
@@ -238,9 +238,16 @@ int run_solve(DevState &d, const SolveIn &in, fvc_stats *stats, hipStream_t s) {
   const int bs = in.bs > 0 ? in.bs : kDefaultBlock;
   if (bs % 64 != 0 || bs > 1024) return fail(FVC_BAD_ARGUMENT, "block size %d must be a multiple of 64 <= 1024", bs);
   const void *pfn = nullptr, *kfn = nullptr;
-  if (in.code == FVC_PCG) {
+  const bool fused = in.recurrence == FVC_REC_FUSED;
+  if (in.code == FVC_PCG && fused) {
+    pfn = (const void *)&cg1_persistent<F>;
+    kfn = (const void *)&cg1_kernel<F>;
+  } else if (in.code == FVC_PCG) {
     pfn = (const void *)&pcg_persistent<F>;
     kfn = (const void *)&pcg_kernel<F>;
+  } else if (in.code == FVC_BICGSTAB && fused) {
+    pfn = (const void *)&bicg2_persistent<F>;
+    kfn = (const void *)&bicg2_kernel<F>;
   } else if (in.code == FVC_BICGSTAB) {
     pfn = (const void *)&bicg_persistent<F>;
     kfn = (const void *)&bicg_kernel<F>;
@@ -265,7 +272,7 @@ int run_solve(DevState &d, const SolveIn &in, fvc_stats *stats, hipStream_t s) {
 
   // workspace
   const size_t n = (size_t)neq;
-  int nvec = in.code == FVC_PCG ? 5 : in.code == FVC_BICGSTAB ? 10 : (k + 1);
+  int nvec = in.code == FVC_PCG ? (fused ? 7 : 5) : in.code == FVC_BICGSTAB ? (fused ? 8 : 10) : (k + 1);
   size_t bytes = align_up((kCtrWords + (size_t)G) * sizeof(unsigned)) + align_up(sizeof(KState)) +
                  align_up(kMaxK * (size_t)G * sizeof(double)) + (size_t)nvec * align_up(n * sizeof(double)) +
                  align_up((size_t)(k + 2) * (k + 2) * sizeof(double)) * 2 + align_up(n * sizeof(double)) + 4096;
@@ -288,7 +295,69 @@ int run_solve(DevState &d, const SolveIn &in, fvc_stats *stats, hipStream_t s) {
   hipLaunchKernelGGL(reset_kernel, dim3(1), dim3(256), 0, s, ctr, (int)kCtrWords + G, st, 1);
   HIPCHK(hipGetLastError());
 
-  if (in.code == FVC_PCG) {
+  if (in.code == FVC_PCG && fused) {
+    Cg1Args a{};
+    a.A = in.A; a.g = g; a.st = st; a.m = in.m; a.x = in.x; a.b = in.b;
+    a.r0 = vec[0]; a.r1 = vec[1]; a.w0 = vec[2]; a.w1 = vec[3]; a.s0 = vec[4]; a.s1 = vec[5]; a.p = vec[6];
+    a.tol = in.tol; a.maxit = in.maxit; a.x0_zero = in.x0_zero; a.stop_rule = in.stop_rule;
+    if (persistent) {
+      a.phase = -1;
+      if ((rc = launch(pfn, true, G, bs, a, s))) return rc;
+      HIPCHK(hipEventRecord(d.ev1, s));
+      if ((rc = read_state(d, st, s))) return rc;
+    } else {
+      a.phase = CG1_SETUP;
+      if ((rc = launch(kfn, false, G, bs, a, s))) return rc;
+      a.phase = CG1_W0;
+      if ((rc = launch(kfn, false, G, bs, a, s))) return rc;
+      int batch = 16, launched = 0;
+      a.phase = CG1_ITER;
+      for (;;) {
+        for (int it = 0; it < batch && launched < in.maxit; ++it, ++launched)
+          if ((rc = launch(kfn, false, G, bs, a, s))) return rc;
+        if ((rc = read_state(d, st, s))) return rc;
+        if (d.host_st->done || launched >= in.maxit) break;
+        batch = std::min(batch * 2, 512);
+      }
+      a.phase = CG1_FINAL;
+      if ((rc = launch(kfn, false, G, bs, a, s))) return rc;
+      HIPCHK(hipEventRecord(d.ev1, s));
+      if ((rc = read_state(d, st, s))) return rc;
+    }
+  } else if (in.code == FVC_BICGSTAB && fused) {
+    Bicg2Args a{};
+    a.A = in.A; a.g = g; a.st = st; a.m = in.m; a.x = in.x; a.b = in.b;
+    a.bb = vec[0]; a.p0 = vec[1]; a.p1 = vec[2]; a.v0 = vec[3]; a.v1 = vec[4];
+    a.s = vec[5]; a.t = vec[6]; a.r = vec[7];
+    a.tol = in.tol; a.maxit = in.maxit; a.x0_zero = in.x0_zero; a.stop_rule = in.stop_rule;
+    if (persistent) {
+      a.phase = -1;
+      if ((rc = launch(pfn, true, G, bs, a, s))) return rc;
+      HIPCHK(hipEventRecord(d.ev1, s));
+      if ((rc = read_state(d, st, s))) return rc;
+    } else {
+      a.phase = B2_SETUP;
+      if ((rc = launch(kfn, false, G, bs, a, s))) return rc;
+      int batch = 8, launched = 0;
+      for (;;) {
+        for (int it = 0; it < batch && launched < in.maxit; ++it, ++launched)
+          for (int ph = B2_P; ph <= B2_Q; ++ph) {
+            a.phase = ph;
+            if ((rc = launch(kfn, false, G, bs, a, s))) return rc;
+          }
+        if ((rc = read_state(d, st, s))) return rc;
+        if (d.host_st->done || launched >= in.maxit) break;
+        batch = std::min(batch * 2, 256);
+      }
+      // the P phase that completes the last iteration and decides its stop test
+      a.phase = B2_P;
+      if ((rc = launch(kfn, false, G, bs, a, s))) return rc;
+      a.phase = B2_FINAL;
+      if ((rc = launch(kfn, false, G, bs, a, s))) return rc;
+      HIPCHK(hipEventRecord(d.ev1, s));
+      if ((rc = read_state(d, st, s))) return rc;
+    }
+  } else if (in.code == FVC_PCG) {
     PcgArgs a{};
     a.A = in.A; a.g = g; a.st = st; a.m = in.m; a.x = in.x; a.b = in.b;
     a.r = vec[0]; a.z = vec[1]; a.p0 = vec[2]; a.p1 = vec[3]; a.q = vec[4];
@@ -753,6 +822,9 @@ int solve_impl(int code, int matriz, int unsym, int neq, int nad, const double *
   in.G_req = o.grid_blocks;
   in.precision = o.precision;
   in.barrier = o.barrier;
+  in.recurrence = o.recurrence;
+  if (o.recurrence != FVC_REC_FUSED && o.recurrence != FVC_REC_REFERENCE)
+    return fail(FVC_BAD_ARGUMENT, "recurrence %d (0 fused, 1 reference order)", o.recurrence);
   if (code == FVC_GMRES && o.stop_rule != FVC_STOP_REFERENCE)
     return fail(FVC_UNSUPPORTED, "GMRES supports only the reference stopping rule");
   rc = dispatch_solve(*d, in, stats, s);

@@ -75,6 +75,12 @@ extern "C" {
                                    FP64 x and FP64 residual recomputation ("reliable
                                    updates"); stops on ||b - A x||_2 <= tol ||b||_2       */
 
+/* Recurrences of PCG and BiCGSTAB (same iterates in exact arithmetic). */
+#define FVC_REC_FUSED 0         /* fewest grid reductions: PCG one per iteration
+                                   (Chronopoulos-Gear), BiCGSTAB two                       */
+#define FVC_REC_REFERENCE 1     /* the reference's scalar expressions and order: PCG two,
+                                   BiCGSTAB three reductions per iteration                 */
+
 typedef struct fvc_options {
   int32_t x0_rule;        /* FVC_X0_*; default FVC_X0_AS_OPENMP                      */
   int32_t gmres_k;        /* restart length; 0 = 15 as src/Solver.f:63               */
@@ -87,7 +93,8 @@ typedef struct fvc_options {
   int32_t batch_work_exp10; /* batches: blocks per system ~ neq^(this/10); 0 = 15     */
   int32_t barrier;        /* persistent engine's grid barrier: 0 per-block flags,
                              1 arrival counters                                     */
-  int32_t reserved[6];
+  int32_t recurrence;     /* FVC_REC_*; PCG and BiCGSTAB                              */
+  int32_t reserved[5];
 } fvc_options;
 
 typedef struct fvc_stats {

@@ -21,26 +21,68 @@ def _loss_grad(solver, k, subset):
     return loss_and_grad(solver, k, subset)
 
 
+def _perturbation_rhs(op, k, T, c, dk):
+    """r = (b(k + dk e_c) - b(k)) - (A(k + dk e_c) - A(k)) T, one face at a time.
+
+    A face's diffusion coefficient is D = geo / u with u = a/k_P + (1-a)/k_N;
+    raising k_c by dk changes 1/u by a dk / (k_c (k_c + dk) u u') on c's own
+    faces (c is P, share a) and by (1-a) dk / (...) on the neighbours' faces
+    that point back at c (c is N there).  Forming the change this way instead
+    of subtracting assembled coefficients avoids the eps/dk relative rounding
+    of that subtraction, which otherwise leaves a non-conservative residual
+    that swamps the finite difference on cells whose gradient is a near
+    cancellation (lambda_c ~ lambda_N far from the loss cells)."""
+    nbr = op.mesh.nbr
+    back = P.ell_back_slots(nbr)
+    r = np.zeros(op.mesh.neq)
+    kc, kp = k[c], k[c] + dk
+    for s in range(nbr.shape[1]):
+        n = nbr[c, s]
+        if n >= 0:
+            a = op.alpha[c, s]
+            u, up = a / kc + (1 - a) / k[n], a / kp + (1 - a) / k[n]
+            r[c] -= a * dk / (kc * kp * u * up) * op.geo[c, s] * (T[c] - T[n])
+            t = back[c, s]
+            a2 = op.alpha[n, t]
+            u2, u2p = a2 / k[n] + (1 - a2) / kc, a2 / k[n] + (1 - a2) / kp
+            r[n] -= (1 - a2) * dk / (kc * kp * u2 * u2p) * op.geo[n, t] * (T[n] - T[c])
+        else:
+            r[c] += dk * op.wall_g[c, s] * (op.wall_T[c, s] - T[c])
+    return r
+
+
 def _fd_correction(op, solver, k, T, subset, c):
     """Central difference of L = mean(T[S]^2) in k_c, each side evaluated as
-    T(k +- h e_c) = T + A(k +- h e_c)^-1 [(b(k +- h) - b(k)) - (A(k +- h) - A(k)) T]:
+    T(k +- h e_c) = T + d+-,  d+- = A(k +- h e_c)^-1 r+-  (r from _perturbation_rhs):
     the exact difference of the two solutions (for the solution T of A(k) T = b),
-    solved for directly so that it does not cancel against T."""
+    solved for directly so that it does not cancel against T.  The loss
+    difference is formed as mean((d+ - d-) (2 T + d+ + d-)) over S, equal to
+    mean((T + d+)^2) - mean((T + d-)^2) but free of the cancellation against L
+    itself."""
     h = FD_REL_STEP * k[c]
-    s0 = op.assemble(k)
-    nbr = s0.nbr
-    nb = np.where(nbr >= 0, nbr, 0)
-    L = []
+    d = []
     for sgn in (+1.0, -1.0):
         kk = k.copy()
         kk[c] += sgn * h
-        s = solver.load(kk)
-        # (A(k+-h) - A(k)) T from the coefficient differences (zero away from c)
-        dA_T = (s.ad - s0.ad) * T + np.where(nbr >= 0, (s.off - s0.off) * T[nb], 0.0).sum(axis=1)
-        r = (s.b - s0.b) - dA_T
-        Tn = T + solver.solve_rhs(r)
-        L.append(np.mean(Tn[subset] ** 2))
-    return (L[0] - L[1]) / (2 * h)
+        solver.load(kk)
+        d.append(solver.solve_rhs(_perturbation_rhs(op, k, T, c, sgn * h))[subset])
+    dp, dm = d
+    return np.mean((dp - dm) * (2.0 * T[subset] + dp + dm)) / (2 * h)
+
+
+def test_perturbation_rhs_matches_assembled_difference():
+    op, k, subset = P.heat_problem(24, subset_frac=0.05)
+    s0 = op.assemble(k)
+    T = np.linspace(0.0, 1.0, s0.neq) ** 2
+    A0 = P.scipy_csr(s0)
+    for c in (0, 23, 300, 575):
+        for dk in (0.3 * k[c], -0.2 * k[c]):
+            kk = k.copy()
+            kk[c] += dk
+            s = op.assemble(kk)
+            want = (s.b - s0.b) - (P.scipy_csr(s) @ T - A0 @ T)
+            got = _perturbation_rhs(op, k, T, c, dk)
+            assert np.allclose(got, want, rtol=1e-12, atol=1e-13 * np.abs(want).max())
 
 
 def test_adjoint_vs_direct_fd_small(fvc):
@@ -116,7 +158,8 @@ def test_torch_autograd_function(fvc):
 
 def test_adjoint_full_config_512(fvc):
     """The differentiation config: 512x512 jittered quads, lognormal k, 1%
-    subset; 32 random cells against central finite differences."""
+    subset; 32 random cells against central finite differences, and all cells
+    against a sparse-LU FP64 adjoint."""
     from fortranfvcell_amd.adjoint import HeatSolver
     op, k, subset = P.heat_problem(512)
     solver = HeatSolver(op, tol=1e-15)
@@ -140,3 +183,20 @@ def test_adjoint_full_config_512(fvc):
     print(f"32 cells: worst relative error vs central FD {worst:.2e}", flush=True)
     for c, g, fd, err in rows:
         assert err <= 1e-5, (c, g, fd, err)
+    # every cell against an FP64 CPU adjoint by sparse LU (A^T lam = dL/dT,
+    # refined twice), with T solved the same way
+    s = op.assemble(k)
+    A = P.scipy_csr(s).tocsc()
+    lu = sla.splu(A)
+    Tl = lu.solve(s.b)
+    for _ in range(2):
+        Tl = Tl + lu.solve(s.b - A @ Tl)
+    g = np.zeros(s.neq)
+    g[subset] = 2.0 * Tl[subset] / len(subset)
+    lam = lu.solve(g, trans="T")
+    for _ in range(2):
+        lam = lam + lu.solve(g - A.T @ lam, trans="T")
+    gk_lu = op.grad_k(k, Tl, lam)
+    err_all = np.linalg.norm(gk - gk_lu) / np.linalg.norm(gk_lu)
+    print(f"all {s.neq} cells vs sparse-LU adjoint: relative 2-norm error {err_all:.2e}", flush=True)
+    assert err_all <= 1e-8

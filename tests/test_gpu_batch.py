@@ -63,3 +63,36 @@ def test_batch_device_resident(fvc):
     xh, sth = fvc.solve_batch(batch, tol=1e-10, stop_rule=fvc.STOP_RELRES)
     assert np.array_equal(db.x.get(), xh)
     assert [s["iterations"] for s in sts] == [s["iterations"] for s in sth]
+
+
+@pytest.fixture(scope="module")
+def full_batch():
+    """The batching config: 256 pressure systems, square quad meshes of side
+    log-uniform in [32, 512], lognormal(0.5) conductivity, seeded zero-mean
+    right-hand sides (10.2M rows in all)."""
+    systems = P.config_batch()
+    from fortranfvcell_amd import concat_ell
+    return systems, concat_ell([P.to_ell(s) for s in systems])
+
+
+@pytest.mark.parametrize("barrier", [0, 1], ids=["flags", "counters"])
+@pytest.mark.parametrize("precision", [0, 1], ids=["fp64", "mixed"])
+def test_full_batch_config_every_system_converges(fvc, full_batch, precision, barrier):
+    # Regression: a phase that reduces nothing (the mixed solve's x += S y)
+    # must still order every wave's writes before the block's arrival, or a
+    # neighbouring block's next SpMV reads stale x.
+    systems, batch = full_batch
+    print(f"full batch precision={precision} barrier={barrier} rows={batch['row_offsets'][-1]}", flush=True)
+    db = fvc.DeviceBatch(batch)
+    sts = db.solve(tol=1e-10, precision=precision, stop_rule=fvc.STOP_RELRES, barrier=barrier)
+    print(f"  {sts[0]['seconds'] * 1e3:.1f} ms, {len(sts) / sts[0]['seconds']:.1f} solves/s, "
+          f"max iterations {max(s['iterations'] for s in sts)}", flush=True)
+    x = db.x.get()
+    ro = batch["row_offsets"]
+    worst = 0.0
+    for i, s in enumerate(systems):
+        assert sts[i]["status"] == 0, (i, sts[i])
+        xi = x[ro[i]:ro[i + 1]]
+        worst = max(worst, np.linalg.norm(s.b - P.scipy_csr(s) @ xi) / np.linalg.norm(s.b))
+    print(f"  worst true relative residual {worst:.2e}", flush=True)
+    assert worst <= 1.01e-10

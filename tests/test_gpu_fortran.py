@@ -57,7 +57,8 @@ def test_fortran_dropin_matches_oracle(fvc, orc, tmp_path, code, lay, omp):
     label = {(1, True): "PCG OMP      : ", (1, False): "PCG:       ", (2, True): "PBICGSTAB OMP: ",
              (2, False): "PBICGSTAB: ", (3, False): "GMRES: "}[(code, omp)]
     # write(13,'(a,a,i9,a,d20.10,a,d20.10)') label,"it",j," energy norm ",energy," tol ",tol
-    pat = re.escape(label) + r"it[ 0-9]{9} energy norm [ -]0\.\d{10}D[+-]\d\d tol [ -]0\.1000000000D-09"
+    # (d20.10 right-justifies the value in 20 columns)
+    pat = re.escape(label) + r"it[ 0-9]{9} energy norm +-?0\.\d{10}D[+-]\d\d tol +0\.1000000000D-09"
     assert re.fullmatch(pat, logtxt.rstrip("\n")), logtxt
 
 

@@ -76,13 +76,38 @@ def test_solver_parity_vs_oracle(fvc, orc, case):
     assert abs(stg["energy"] - so["energy"]) <= 1e-8 * abs(so["energy"])
 
 
-@pytest.mark.parametrize("code,lay", [(1, "ell"), (2, "ell"), (2, "csrd"), (3, "ell"), (3, "csrd")])
-def test_engines_bit_identical(fvc, code, lay):
+KRYLOV2 = [c for c in CASES if c[2] in (1, 2)]
+
+
+@pytest.mark.parametrize("rec", [0, 1], ids=["fused", "reference_order"])
+@pytest.mark.parametrize("case", KRYLOV2, ids=[c[0] for c in KRYLOV2])
+def test_recurrences_vs_oracle(fvc, orc, case, rec):
+    """PCG and BiCGSTAB in both recurrences (one/two grid reductions per
+    iteration, or the reference's expressions in its order) against the
+    oracle at the same tolerance."""
+    name, build, code, lay = case
+    s = build()
+    f = LAYOUT[lay](s)
+    unsym = not lay.endswith("sym")
+    print(f"recurrence {rec} {name} neq={s.neq}", flush=True)
+    tol, xo, so = tol_for_relres(orc, s, f, code, unsym)
+    x, st = fvc.solve(f, code, tol=tol, maxit=20000, recurrence=rec)
+    assert st["status"] == 0
+    assert _rel(x, xo) <= X_TOL, (name, rec, _rel(x, xo))
+    assert _iters_close(st["iterations"], so["iterations"]), (st["iterations"], so["iterations"])
+    assert abs(st["energy"] - so["energy"]) <= 1e-8 * abs(so["energy"])
+
+
+@pytest.mark.parametrize("rec", [0, 1], ids=["fused", "reference_order"])
+@pytest.mark.parametrize("code,lay", [(1, "ell"), (1, "csrd"), (2, "ell"), (2, "csrd"), (3, "ell"), (3, "csrd")])
+def test_engines_bit_identical(fvc, code, lay, rec):
+    if code == 3 and rec == 1:
+        pytest.skip("GMRES has a single recurrence")
     s = P.config_poisson(40) if code == 1 else P.config_advdiff(40, peclet_max=60.0)
     f = LAYOUT[lay](s)
-    print(f"engines code={code} {lay}", flush=True)
-    xp, sp = fvc.solve(f, code, engine=fvc.ENGINE_PERSISTENT, gmres_k=20)
-    xk, sk = fvc.solve(f, code, engine=fvc.ENGINE_KERNELS, gmres_k=20)
+    print(f"engines code={code} {lay} recurrence={rec}", flush=True)
+    xp, sp = fvc.solve(f, code, engine=fvc.ENGINE_PERSISTENT, gmres_k=20, recurrence=rec)
+    xk, sk = fvc.solve(f, code, engine=fvc.ENGINE_KERNELS, gmres_k=20, recurrence=rec)
     assert sp["status"] == 0 and sk["status"] == 0
     assert sp["engine"] == fvc.ENGINE_PERSISTENT and sk["engine"] == fvc.ENGINE_KERNELS
     assert np.array_equal(xp, xk)
@@ -139,6 +164,22 @@ def test_true_residual_and_relres_rule(fvc):
     r = np.linalg.norm(s.b - P.scipy_csr(s) @ x) / np.linalg.norm(s.b)
     assert r <= 2e-10
     assert abs(st["true_relres"] - r) <= 1e-3 * r + 1e-16
+
+
+@pytest.mark.parametrize("rec", [0, 1], ids=["fused", "reference_order"])
+def test_maxit_counts(fvc, rec):
+    # maxit stops after exactly maxit iterations in both recurrences; the
+    # x returned is that iteration's (checked against a longer run's prefix
+    # through the residual decrease)
+    s = P.config_advdiff(24, peclet_max=60.0)
+    f = P.to_ell(s)
+    for code in (1, 2):
+        g = P.to_ell(P.config_poisson(24)) if code == 1 else f
+        for engine in (fvc.ENGINE_PERSISTENT, fvc.ENGINE_KERNELS):
+            x3, st3 = fvc.solve(g, code, maxit=3, recurrence=rec, engine=engine)
+            assert st3["status"] == 1 and st3["iterations"] == 3, (code, engine, st3)
+            x3r, _ = fvc.solve(g, code, maxit=3, recurrence=1 - rec, engine=engine)
+            assert _rel(x3, x3r) <= 1e-10, (code, engine)
 
 
 def test_status_codes(fvc):
