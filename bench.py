@@ -7,9 +7,11 @@ upwind advection-diffusion system on a 1024x1024 jittered quad mesh
 ||b - A x||_2 / ||b||_2 <= 1e-10.  The same line reports, for the same
 mesh size, Jacobi-PCG on the pinned pressure system and GMRES(30), the
 plain ELLPACK SpMV (warm and with the 256 MB last-level cache defeated by
-rotating buffers), the roofline of the dominant kernel, and the reference's
-own OpenMP Fortran solver (compiled from its sources, oracle/_ref) timed on
-the host cores as the CPU baseline.
+rotating buffers), FP64 against mixed-precision PCG on the lognormal
+pressure system, the 256-system batch (FP64, mixed, and one at a time), the
+roofline of the dominant kernel, and the reference's own OpenMP Fortran
+solver (compiled from its sources, oracle/_ref) timed on the host cores as
+the CPU baseline.
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W]
 """
@@ -41,13 +43,26 @@ def spmv_bytes(neq: int, nnz_off: int, vbytes: int = 8) -> int:
 
 
 def bicgstab_iter_bytes(neq: int, nnz_off: int) -> int:
-    """Per iteration: two SpMV plus 24 vector streams of 8 bytes per row."""
-    return 2 * spmv_bytes(neq, nnz_off) + 24 * 8 * neq
+    """Algorithmic bytes of one iteration of the two-phase BiCGSTAB (the
+    default recurrence).  P: the matrix (ad + value/column per off-diagonal),
+    reads s, t, p, v, x, rhat and writes x, b, p, v; Q: the matrix, reads b,
+    v, rhat and writes s, t.  Each vector element counted once (neighbour
+    gathers are cache hits); the Jacobi diagonal is the matrix's ad."""
+    per_matrix = 8 * neq + 12 * nnz_off
+    return 2 * per_matrix + (10 + 7) * 8 * neq
+
+
+def bicgstab_solve_bytes(neq: int, nnz_off: int, its: int) -> int:
+    """its iterations + the P phase that completes the last one + setup
+    (b, ad read; x, r, p, b written) + final (matrix, x, b)."""
+    p_phase = 8 * neq + 12 * nnz_off + 10 * 8 * neq
+    return its * bicgstab_iter_bytes(neq, nnz_off) + p_phase + 6 * 8 * neq + (8 * neq + 12 * nnz_off + 2 * 8 * neq)
 
 
 def pcg_iter_bytes(neq: int, nnz_off: int) -> int:
-    """Per iteration: one SpMV plus 11 vector streams of 8 bytes per row."""
-    return spmv_bytes(neq, nnz_off) + 11 * 8 * neq
+    """One iteration of the single-reduction PCG (the default recurrence):
+    the matrix, reads r, w, s, p, x and writes p, s, x, r, w."""
+    return 8 * neq + 12 * nnz_off + 10 * 8 * neq
 
 
 def gmres_cycle_bytes(neq: int, nnz_off: int, steps: int) -> int:
@@ -93,16 +108,21 @@ def time_spmv(F, ds, neq, nnz, reps=50, rotate=1):
     return t, spmv_bytes(neq, nnz) / t / 1e9
 
 
-def cpu_baseline(fmt, method, tol, sample_solves=1):
+def cpu_baseline(fmt, method, tol, budget_s=10.0):
     """The reference Fortran (oracle/_ref) with OpenMP on the host cores: the
     solver's own OpenMP routines (pbicgstab_omp + matvec_ellpack_omp +
-    dot_ompL4).  Falls back to the sequential C restatement."""
+    dot_ompL4), repeated for about budget_s seconds.  Falls back to the
+    sequential C restatement."""
     import oracle
     threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or min(os.cpu_count() or 1, 16)
     if oracle.have_ref():
-        r = oracle.ref()
+        oracle.ref()
         oracle.ref_set_omp(1, threads)
         x0 = np.zeros(fmt["neq"])
+        t0 = time.perf_counter()
+        oracle.krylov(method, fmt, tol=tol, use_ref=True, omp=True, nthreads=threads, x0=x0, init=True)
+        first = time.perf_counter() - t0
+        sample_solves = max(1, min(1000, int(budget_s / max(first, 1e-6))))
         t0 = time.perf_counter()
         for _ in range(sample_solves):
             oracle.krylov(method, fmt, tol=tol, use_ref=True, omp=True, nthreads=threads, x0=x0, init=True)
@@ -194,7 +214,8 @@ def main():
     x = ds.x.get()
     relres = float(np.linalg.norm(s.b - P.scipy_csr(s) @ x) / np.linalg.norm(s.b))
     its = st["iterations"]
-    solve_bytes = its * bicgstab_iter_bytes(neq, nnz) + 2 * spmv_bytes(neq, nnz)
+    solve_bytes = bicgstab_solve_bytes(neq, nnz, its)
+    launches = 2 * its + 4   # SETUP, P/Q per iteration, the closing P and Q, FINAL (+ a no-row flush)
     kt = float(np.mean(kern))
     achieved = solve_bytes / kt / 1e9
     log(f"BiCGSTAB: {its} it, {kt * 1e3:.3f} ms/solve kernel, wall {wall / args.steps * 1e3:.3f} ms, relres {relres:.2e}")
@@ -218,12 +239,19 @@ def main():
                    "iterations": its, "true_relres": relres, "engine": st["engine"],
                    "grid_blocks": st["grid_blocks"], "block_size": st["block_size"],
                    "device": info["name"]},
+        # the dominant kernel is bicg2_kernel<0> (ELLPACK width 4): one launch
+        # per phase.  achieved = algorithmic bytes per launch / event time per
+        # launch on the solve's stream, i.e. bytes per solve / time per solve
+        # (launch gaps and the host's done-checks included, so a lower bound
+        # of the kernel-only rate rocprofv3 reports)
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS,
-                     "traffic": read_traffic("bicg_persistent_advdiff1024_traffic.txt"),
-                     "kernel": "fvc::bicg_persistent<0> (one launch per solve)",
-                     "algorithmic_bytes_per_launch": solve_bytes,
-                     "kernel_ms": kt * 1e3},
+                     "traffic": read_traffic("bicg2_kernel_advdiff1024_traffic.txt"),
+                     "kernel": "fvc::bicg2_kernel<0> (two-phase BiCGSTAB, one launch per phase)",
+                     "launches_per_solve": launches,
+                     "algorithmic_bytes_per_launch": solve_bytes / launches,
+                     "us_per_launch": kt / launches * 1e6,
+                     "solve_ms": kt * 1e3},
     }
 
     if not args.no_extra:
@@ -254,6 +282,48 @@ def main():
                                  "GBs_algorithmic": pb / st_p["seconds"] / 1e9,
                                  "us_per_iteration": st_p["seconds"] / st_p["iterations"] * 1e6}
         log("pcg", extra["pcg_pressure"])
+        # mixed precision (FP32 inner PCG, FP64 residuals and refinement) against
+        # FP64 on the 1M-cell lognormal-conductivity pressure system
+        sl = P.config_pressure_lognormal(n)
+        dsl = F.DeviceSystem(P.to_ell(sl))
+        Al = P.scipy_csr(sl)
+        mixed = {}
+        for name, prec in (("fp64", F.PREC_FP64), ("mixed", F.PREC_MIXED)):
+            dsl.solve(F.PCG, tol=tol, precision=prec, **opts)
+            stl = dsl.solve(F.PCG, tol=tol, precision=prec, **opts)
+            xl = dsl.x.get()
+            mixed[name] = {"iterations": stl["iterations"], "refinements": stl["inner_iterations"],
+                           "ms_per_solve": stl["seconds"] * 1e3, "solves_per_s": 1.0 / stl["seconds"],
+                           "true_relres": float(np.linalg.norm(sl.b - Al @ xl) / np.linalg.norm(sl.b)),
+                           "status": stl["status"], "_x": xl}
+        x64 = mixed["fp64"].pop("_x")
+        xm = mixed["mixed"].pop("_x")
+        mixed["mixed"]["rel_err_vs_fp64"] = float(np.linalg.norm(xm - x64) / np.linalg.norm(x64))
+        mixed["speedup_mixed_over_fp64"] = mixed["fp64"]["ms_per_solve"] / mixed["mixed"]["ms_per_solve"]
+        extra["pcg_lognormal_fp64_vs_mixed"] = mixed
+        log("mixed", mixed)
+        # batching: 256 uneven pressure systems (sides log-uniform in [32, 512])
+        # in one persistent launch, FP64 and mixed, against the same systems
+        # solved one at a time in FP64 (all resident in HBM)
+        systems = P.config_batch()
+        fmts = [P.to_ell(q) for q in systems]
+        batch = F.concat_ell(fmts)
+        db = F.DeviceBatch(batch)
+        bres = {"systems": len(fmts), "rows": int(batch["row_offsets"][-1])}
+        for name, prec in (("fp64", F.PREC_FP64), ("mixed", F.PREC_MIXED)):
+            db.solve(tol=tol, precision=prec, stop_rule=F.STOP_RELRES)
+            sts = db.solve(tol=tol, precision=prec, stop_rule=F.STOP_RELRES)
+            t = sts[0]["seconds"]
+            bres[name] = {"ms": t * 1e3, "solves_per_s": len(sts) / t,
+                          "max_iterations": max(q["iterations"] for q in sts),
+                          "all_converged": all(q["status"] == 0 for q in sts)}
+        single = 0.0
+        for f in fmts:
+            d1 = F.DeviceSystem(f)
+            single += d1.solve(F.PCG, tol=tol, **opts)["seconds"]
+        bres["one_at_a_time_fp64"] = {"ms": single * 1e3, "solves_per_s": len(fmts) / single}
+        extra["batch256"] = bres
+        log("batch", bres)
         result["extra"] = extra
 
     if not args.no_cpu:

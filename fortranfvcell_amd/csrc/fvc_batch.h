@@ -36,8 +36,8 @@ struct BatchArgs {
   double *partials;         // kMaxK per block
   double *x;
   const double *b;
-  double *v0, *v1, *v2, *v3, *v4;   // FP64 work vectors (concatenated)
-  float *f0, *f1, *f2, *f3, *f4;    // FP32 work vectors (mixed)
+  double *v[5];                     // FP64 work vectors (concatenated)
+  float *f[5];                      // FP32 work vectors (mixed)
   float *av32;                      // FP32 scaled values (mixed)
   double *s;                        // 1/sqrt(a_ii) (mixed)
   float *dsq;                       // sqrt(a_ii) (mixed)
@@ -69,6 +69,9 @@ __device__ __forceinline__ void batch_mat(const BatchArgs &B, const SysDesc &D, 
   A.tptr = A.tsrc = A.tpos = nullptr;
 }
 
+// FP64: Jacobi-PCG in the reference's two-reduction form (pcg_phase).  With
+// a few blocks per system a group barrier is cheap, and the one-reduction
+// form's extra neighbour gathers cost more than its saved barrier.
 template <int F>
 __global__ __launch_bounds__(1024) void pcg_batch_persistent(BatchArgs B) {
   __shared__ double lds[kMaxK * kMaxWaves];
@@ -86,11 +89,11 @@ __global__ __launch_bounds__(1024) void pcg_batch_persistent(BatchArgs B) {
   a.m = a.A.ad;
   a.x = B.x + off;
   a.b = B.b + off;
-  a.r = B.v0 + off;
-  a.z = B.v1 + off;
-  a.p0 = B.v2 + off;
-  a.p1 = B.v3 + off;
-  a.q = B.v4 + off;
+  a.r = B.v[0] + off;
+  a.z = B.v[1] + off;
+  a.p0 = B.v[2] + off;
+  a.p1 = B.v[3] + off;
+  a.q = B.v[4] + off;
   a.tol = B.tol;
   a.maxit = B.maxit;
   a.x0_zero = B.x0_zero;
@@ -104,6 +107,7 @@ __global__ __launch_bounds__(1024) void pcg_batch_persistent(BatchArgs B) {
   pcg_phase<F>(a, PCG_FINAL, lds);
 }
 
+// Mixed precision (fvc_mixed.h) per system.
 template <int F>
 __global__ __launch_bounds__(1024) void mixed_batch_persistent(BatchArgs B) {
   __shared__ double lds[kMaxK * kMaxWaves];
@@ -128,27 +132,17 @@ __global__ __launch_bounds__(1024) void mixed_batch_persistent(BatchArgs B) {
   a.dsq = B.dsq + off;
   a.x = B.x + off;
   a.b = B.b + off;
-  a.y = B.f0 + off;
-  a.r = B.f1 + off;
-  a.p0 = B.f2 + off;
-  a.p1 = B.f3 + off;
-  a.q = B.f4 + off;
+  a.y = B.f[0] + off;
+  a.r = B.f[1] + off;
+  a.p0 = B.f[2] + off;
+  a.p1 = B.f[3] + off;
+  a.q = B.f[4] + off;
   a.tol = B.tol;
   a.delta = 0.1;
   a.maxit = B.maxit;
   a.x0_zero = B.x0_zero;
   a.max_inner = 1000;
-  if (!mixed_phase<F>(a, MX_SETUP, lds)) return;
-  while (!ld_int(&a.st->done)) {
-    if (!mixed_phase<F>(a, MX_A, lds)) return;
-    if (ld_int(&a.st->done)) break;
-    if (!mixed_phase<F>(a, MX_B, lds)) return;
-    if (ld_int(&a.st->step_i)) {
-      if (!mixed_phase<F>(a, MX_X, lds)) return;
-      if (!mixed_phase<F>(a, MX_R, lds)) return;
-    }
-  }
-  mixed_phase<F>(a, MX_FINAL, lds);
+  mixed_run<F>(a, lds);
 }
 
 // S A S in FP32 for every system of the batch (rows are independent; the

@@ -23,7 +23,8 @@ struct alignas(128) KState {
   double bnorm2, res2, econv, rnorm2, upd_est, d_prev, pad2, pad3;
   int iter, status, done, inner;      // inner: GMRES Arnoldi steps (nit)
   int step_i, mgs_j, inner_done, ni;  // GMRES position in the cycle
-  int cycle, pad4, pad5, pad6;
+  int cycle, pend, next, pad6;         // pend: deferred reduction pending (Defer);
+                                       // next: the phase a state-driven method runs next
 };
 
 __device__ __forceinline__ int ld_int(const int *p) {
@@ -35,6 +36,80 @@ __device__ __forceinline__ double ld_dbl(const double *p) {
   return __builtin_bit_cast(double, u);
 }
 __device__ __forceinline__ bool finite(double v) { return __builtin_isfinite(v); }
+
+// --------------------------------------------------- deferred reductions --
+//
+// The kernel-per-phase engine of the fused methods.  A launch never waits on
+// another block: each block writes the block sums of its phase's values; the
+// next launch's blocks each sum all G partials (in the same order as the
+// grid barrier's epilogue, so both engines give bit-identical results) and
+// apply the phase's scalar recurrence to their own copy of the state, in LDS.
+// Kernel boundaries order the memory, so no fence and no atomic is needed.
+// State and partials are double-buffered: a launch reads set `par` and
+// writes set 1 - par; block 0 writes the updated state.  KState::pend is
+// 1 + the phase whose partials wait for their recurrence (0: none).
+struct Defer {
+  KState *st;     // two states
+  double *part;   // two arrays of kMaxK * G partials
+  int par;
+};
+
+// M: nred(ph) values reduced by phase ph; skip(ph, state) whether the phase's
+// rows are skipped (solve finished); rows(ph, state, v); epi(ph, state, t).
+// ph = kPhaseNext: the phase the state names (KState::next, set by the
+// recurrences of methods whose phase order depends on the iterates);
+// other ph < 0: no rows, only the pending recurrence (a flush).
+constexpr int kPhaseNext = -2;
+
+template <class M>
+__device__ void deferred_phase(const Grid &g, const Defer &d, int ph, const M &m, KState *S,
+                               double *lds) {
+  constexpr int nw = (int)(sizeof(KState) / sizeof(unsigned));
+  const unsigned *in = reinterpret_cast<const unsigned *>(d.st + d.par);
+  for (int k = threadIdx.x; k < nw; k += blockDim.x) reinterpret_cast<unsigned *>(S)[k] = in[k];
+  __syncthreads();
+  if (S->pend) {
+    const int pp = S->pend - 1;
+    const double *pin = d.part + (size_t)d.par * kMaxK * g.G;
+    double t[kMaxK] = {0.0, 0.0, 0.0, 0.0};
+    switch (m.nred(pp)) {
+      case 0: break;
+      case 1: sum_partials<1>(pin, g.G, t, lds); break;
+      case 2: sum_partials<2>(pin, g.G, t, lds); break;
+      case 3: sum_partials<3>(pin, g.G, t, lds); break;
+      default: sum_partials<4>(pin, g.G, t, lds); break;
+    }
+    if (threadIdx.x == 0) {
+      m.epi(pp, S, t);
+      S->pend = 0;
+    }
+    __syncthreads();
+  }
+  if (ph == kPhaseNext) ph = S->done ? -1 : S->next;
+  if (ph >= 0 && !m.skip(ph, *S)) {
+    double v[kMaxK] = {0.0, 0.0, 0.0, 0.0};
+    m.rows(ph, S, v);
+    const int K = m.nred(ph);
+    switch (K) {
+      case 0: break;
+      case 1: block_sum<1>(v, lds); break;
+      case 2: block_sum<2>(v, lds); break;
+      case 3: block_sum<3>(v, lds); break;
+      default: block_sum<4>(v, lds); break;
+    }
+    if (threadIdx.x == 0) {
+      double *out = d.part + (size_t)(1 - d.par) * kMaxK * g.G;
+      const int c = g.chunk();
+      for (int k = 0; k < K; ++k) out[(size_t)k * g.G + c] = v[k];
+      S->pend = ph + 1;
+    }
+  }
+  if (g.bid() == 0) {
+    __syncthreads();
+    unsigned *out = reinterpret_cast<unsigned *>(d.st + 1 - d.par);
+    for (int k = threadIdx.x; k < nw; k += blockDim.x) out[k] = reinterpret_cast<const unsigned *>(S)[k];
+  }
+}
 
 // Gather functors: the vector an SpMV multiplies, element j (0-based).
 struct XVec {
@@ -234,7 +309,15 @@ __global__ __launch_bounds__(1024) void pcg_kernel(PcgArgs a) {
 // reduction per iteration suffices.  The SpMV operand u' at the neighbours is
 // formed on the fly from r, w and the previous s there; those three vectors
 // alternate between two buffers (odd iterations read set 0 and write set 1).
+//
+// Each phase is split into its row part (cg1_rows: this block's rows, the
+// values to reduce) and its scalar part (cg1_epi: the recurrence on the
+// reduced values), so that both engines can drive it: the persistent kernel
+// reduces at a grid barrier, the kernel-per-phase engine defers the scalar
+// part to the next launch (fvc_device.h, Defer).
 enum { CG1_SETUP = 0, CG1_W0 = 1, CG1_ITER = 2, CG1_FINAL = 3 };
+
+__host__ __device__ constexpr int cg1_nred(int ph) { return ph == CG1_W0 ? 1 : ph == CG1_FINAL ? 2 : 3; }
 
 struct Cg1Args {
   Mat A;
@@ -246,6 +329,7 @@ struct Cg1Args {
   double *r0, *r1, *w0, *w1, *s0, *s1, *p;
   double tol;
   int maxit, x0_zero, stop_rule, phase;
+  Defer d;
 };
 
 // u' at row j: (r_j - alpha (w_j + beta s_j)) / m_j  (s = w in iteration 1)
@@ -264,12 +348,10 @@ struct XDiv {
 };
 
 template <int F>
-__device__ bool cg1_phase(const Cg1Args &a, int ph, double *lds) {
+__device__ void cg1_rows(const Cg1Args &a, int ph, const KState *S, double *v) {
   const int i0 = a.g.row_begin(), i1 = a.g.row_end();
-  KState *st = a.st;
   if (ph == CG1_SETUP) {
-    // r = b - A x (or r = b for x0 = 0); d = r.M^-1 r; conv as the reference
-    double v[3] = {0.0, 0.0, 0.0};
+    // r = b - A x (or r = b for x0 = 0); d = r.M^-1 r; ||b||^2, ||r||^2
     for (int i = i0 + (int)threadIdx.x; i < i1; i += blockDim.x) {
       double ax;
       if (a.x0_zero) {
@@ -285,49 +367,20 @@ __device__ bool cg1_phase(const Cg1Args &a, int ph, double *lds) {
       v[1] += bi * bi;
       v[2] += ri * ri;
     }
-    return grid_sync<3>(a.g, v, lds, [&](const double *t) {
-      st->d = t[0];
-      st->bnorm2 = t[1];
-      st->rnorm2 = t[2];
-      st->iter = 0;
-      st->inner = 0;
-      st->status = S_MAXIT;
-      st->done = 0;
-      st->beta = 0.0;
-      if (a.stop_rule == 1) {
-        st->conv = a.tol * sqrt(t[1]);
-        st->last = sqrt(t[2]);
-        if (st->last <= st->conv) { st->status = S_OK; st->done = 1; }
-      } else {
-        st->conv = a.tol * sqrt(fabs(t[0]));
-        st->last = sqrt(fabs(t[0]));
-      }
-      if (!finite(t[0])) { st->status = S_NONFINITE; st->done = 1; }
-      if (a.maxit <= 0 && !st->done) st->done = 1;
-    });
-  }
-  if (ph == CG1_W0) {
-    // w = A u, u = r/m on the fly; alpha = d / w.u
+  } else if (ph == CG1_W0) {
+    // w = A u, u = r/m on the fly; -> w.u
     const XDiv xf{a.r0, a.m};
-    double v[1] = {0.0};
     for (int i = i0 + (int)threadIdx.x; i < i1; i += blockDim.x) {
       const double wi = row_apply<F>(a.A, i, xf);
       a.w0[i] = wi;
       v[0] += wi * xf(i);
     }
-    return grid_sync<1>(a.g, v, lds, [&](const double *t) {
-      st->alpha = st->d / t[0];
-      if (t[0] == 0.0) { st->status = S_BREAKDOWN; st->done = 1; }
-      else if (!finite(st->alpha)) { st->status = S_NONFINITE; st->done = 1; }
-    });
-  }
-  if (ph == CG1_ITER) {
-    const int j = ld_int(&st->iter) + 1;
+  } else if (ph == CG1_ITER) {
+    const int j = ld_int(&S->iter) + 1;
     const bool odd = (j & 1) != 0;
     const double *rc = odd ? a.r0 : a.r1, *wc = odd ? a.w0 : a.w1, *sc = odd ? a.s0 : a.s1;
     double *rn = odd ? a.r1 : a.r0, *wn = odd ? a.w1 : a.w0, *sn = odd ? a.s1 : a.s0;
-    const XCgNext xf{rc, wc, sc, a.m, ld_dbl(&st->alpha), ld_dbl(&st->beta), j == 1};
-    double v[3] = {0.0, 0.0, 0.0};
+    const XCgNext xf{rc, wc, sc, a.m, ld_dbl(&S->alpha), ld_dbl(&S->beta), j == 1};
     for (int i = i0 + (int)threadIdx.x; i < i1; i += blockDim.x) {
       const double mi = a.m[i];
       const double ui = rc[i] / mi;
@@ -345,43 +398,81 @@ __device__ bool cg1_phase(const Cg1Args &a, int ph, double *lds) {
       v[1] += wi * un;
       v[2] += ri * ri;
     }
-    return grid_sync<3>(a.g, v, lds, [&](const double *t) {
-      const double gamma = t[0];
-      const double beta = gamma / st->d;
-      const double den = t[1] - beta * gamma / st->alpha;
-      const double alpha = gamma / den;
-      st->beta = beta;
-      st->alpha = alpha;
-      st->d = gamma;
-      st->rnorm2 = t[2];
-      st->iter = j;
-      bool conv;
-      if (a.stop_rule == 1) {
-        st->last = sqrt(t[2]);
-        conv = st->last <= st->conv;
-      } else {
-        st->last = sqrt(fabs(gamma));
-        conv = st->last < st->conv;   // src/Iterativos.f:73
-      }
-      if (conv) { st->status = S_OK; st->done = 1; }
-      else if (!finite(gamma) || !finite(beta)) { st->status = S_NONFINITE; st->done = 1; }
-      else if (den == 0.0) { st->status = S_BREAKDOWN; st->done = 1; }
-      else if (!finite(alpha)) { st->status = S_NONFINITE; st->done = 1; }
-      else if (j >= a.maxit) { st->status = S_MAXIT; st->done = 1; }
-    });
+  } else {
+    // CG1_FINAL: energy norm x.Ax (src/Iterativos.f:82-83) and ||b - Ax||^2
+    for (int i = i0 + (int)threadIdx.x; i < i1; i += blockDim.x) {
+      const double ax = row_apply<F>(a.A, i, XVec{a.x});
+      const double ri = a.b[i] - ax;
+      v[0] += a.x[i] * ax;
+      v[1] += ri * ri;
+    }
   }
-  // CG1_FINAL: energy norm x.Ax (src/Iterativos.f:82-83) and ||b - Ax||^2
-  double v[2] = {0.0, 0.0};
-  for (int i = i0 + (int)threadIdx.x; i < i1; i += blockDim.x) {
-    const double ax = row_apply<F>(a.A, i, XVec{a.x});
-    const double ri = a.b[i] - ax;
-    v[0] += a.x[i] * ax;
-    v[1] += ri * ri;
-  }
-  return grid_sync<2>(a.g, v, lds, [&](const double *t) {
+}
+
+__device__ inline void cg1_epi(const Cg1Args &a, int ph, KState *st, const double *t) {
+  if (ph == CG1_SETUP) {
+    st->d = t[0];
+    st->bnorm2 = t[1];
+    st->rnorm2 = t[2];
+    st->iter = 0;
+    st->inner = 0;
+    st->status = S_MAXIT;
+    st->done = 0;
+    st->beta = 0.0;
+    if (a.stop_rule == 1) {
+      st->conv = a.tol * sqrt(t[1]);
+      st->last = sqrt(t[2]);
+      if (st->last <= st->conv) { st->status = S_OK; st->done = 1; }
+    } else {
+      st->conv = a.tol * sqrt(fabs(t[0]));
+      st->last = sqrt(fabs(t[0]));
+    }
+    if (!finite(t[0])) { st->status = S_NONFINITE; st->done = 1; }
+    if (a.maxit <= 0 && !st->done) st->done = 1;
+  } else if (ph == CG1_W0) {
+    st->alpha = st->d / t[0];
+    if (t[0] == 0.0) { st->status = S_BREAKDOWN; st->done = 1; }
+    else if (!finite(st->alpha)) { st->status = S_NONFINITE; st->done = 1; }
+  } else if (ph == CG1_ITER) {
+    const int j = st->iter + 1;
+    const double gamma = t[0];
+    const double beta = gamma / st->d;
+    const double den = t[1] - beta * gamma / st->alpha;
+    const double alpha = gamma / den;
+    st->beta = beta;
+    st->alpha = alpha;
+    st->d = gamma;
+    st->rnorm2 = t[2];
+    st->iter = j;
+    bool conv;
+    if (a.stop_rule == 1) {
+      st->last = sqrt(t[2]);
+      conv = st->last <= st->conv;
+    } else {
+      st->last = sqrt(fabs(gamma));
+      conv = st->last < st->conv;   // src/Iterativos.f:73
+    }
+    if (conv) { st->status = S_OK; st->done = 1; }
+    else if (!finite(gamma) || !finite(beta)) { st->status = S_NONFINITE; st->done = 1; }
+    else if (den == 0.0) { st->status = S_BREAKDOWN; st->done = 1; }
+    else if (!finite(alpha)) { st->status = S_NONFINITE; st->done = 1; }
+    else if (j >= a.maxit) { st->status = S_MAXIT; st->done = 1; }
+  } else {
     st->energy = t[0];
     st->res2 = t[1];
-  });
+  }
+}
+
+template <int F>
+__device__ bool cg1_phase(const Cg1Args &a, int ph, double *lds) {
+  double v[kMaxK] = {0.0, 0.0, 0.0, 0.0};
+  cg1_rows<F>(a, ph, a.st, v);
+  auto epi = [&](const double *t) { cg1_epi(a, ph, a.st, t); };
+  switch (cg1_nred(ph)) {
+    case 1: return grid_sync<1>(a.g, v, lds, epi);
+    case 2: return grid_sync<2>(a.g, v, lds, epi);
+    default: return grid_sync<3>(a.g, v, lds, epi);
+  }
 }
 
 template <int F>
@@ -398,11 +489,19 @@ __global__ __launch_bounds__(1024) void cg1_persistent(Cg1Args a) {
   cg1_phase<F>(a, CG1_FINAL, lds);
 }
 
+// Kernel-per-phase engine: one launch per phase, the reduction deferred.
 template <int F>
 __global__ __launch_bounds__(1024) void cg1_kernel(Cg1Args a) {
   __shared__ double lds[kMaxK * kMaxWaves];
-  if ((a.phase == CG1_W0 || a.phase == CG1_ITER) && ld_int(&a.st->done)) return;
-  cg1_phase<F>(a, a.phase, lds);
+  __shared__ KState S;
+  struct M {
+    const Cg1Args &a;
+    __device__ int nred(int ph) const { return cg1_nred(ph); }
+    __device__ bool skip(int ph, const KState &s) const { return (ph == CG1_W0 || ph == CG1_ITER) && s.done; }
+    __device__ void rows(int ph, const KState *s, double *v) const { cg1_rows<F>(a, ph, s, v); }
+    __device__ void epi(int ph, KState *s, const double *t) const { cg1_epi(a, ph, s, t); }
+  };
+  deferred_phase(a.g, a.d, a.phase, M{a}, &S, lds);
 }
 
 // ------------------------------------------------------------- BiCGSTAB --
@@ -627,11 +726,14 @@ __global__ __launch_bounds__(1024) void bicg_kernel(BicgArgs a) {
 //         reduces t.s, t.t (omega, src/Iterativos.f:180-181) and rhat.s,
 //         rhat.t, which give rhat.b_new = rhat.s - omega rhat.t: the
 //         reference's dot(r, b) (src/Iterativos.f:192) without a third pass.
-// The stop test of iteration j-1 is decided in P(j)'s epilogue, one phase
+// The stop test of iteration j-1 is decided in P(j)'s recurrence, one phase
 // later than the reference, with the same exact dot products; x is then
 // x_(j-1) and the extra direction P(j) built is dropped.  p and v alternate
-// between two buffers (odd iterations read set 0 and write set 1).
+// between two buffers (odd iterations read set 0 and write set 1).  Row and
+// scalar parts are split as for the one-reduction PCG.
 enum { B2_SETUP = 0, B2_P = 1, B2_Q = 2, B2_FINAL = 3 };
+
+__host__ __device__ constexpr int bicg2_nred(int ph) { return ph == B2_Q ? 4 : ph == B2_FINAL ? 2 : 3; }
 
 struct Bicg2Args {
   Mat A;
@@ -645,6 +747,7 @@ struct Bicg2Args {
   double *s, *t, *r;        // half-step residual, A M^-1 s, shadow residual
   double tol;
   int maxit, x0_zero, stop_rule, phase;
+  Defer d;
 };
 
 // p/m at row j for P(j), j > 1: ((s - w t) + beta (p - w v)) / m
@@ -657,12 +760,10 @@ struct XBi2Dir {
 };
 
 template <int F>
-__device__ bool bicg2_phase(const Bicg2Args &a, int ph, double *lds) {
+__device__ void bicg2_rows(const Bicg2Args &a, int ph, const KState *S, double *v) {
   const int i0 = a.g.row_begin(), i1 = a.g.row_end();
-  KState *st = a.st;
   if (ph == B2_SETUP) {
     // r = b - Ax; rhat = r; p = r; b = r; d = r.(r/m); rho = b.rhat = r.r
-    double v[3] = {0.0, 0.0, 0.0};
     for (int i = i0 + (int)threadIdx.x; i < i1; i += blockDim.x) {
       double ax;
       if (a.x0_zero) {
@@ -680,33 +781,23 @@ __device__ bool bicg2_phase(const Bicg2Args &a, int ph, double *lds) {
       v[1] += ri * ri;
       v[2] += bi * bi;
     }
-    return grid_sync<3>(a.g, v, lds, [&](const double *t) {
-      st->d = t[0];
-      st->rr0 = t[1];
-      st->rnorm2 = t[1];
-      st->bnorm2 = t[2];
-      st->iter = 0;
-      st->inner = 0;
-      st->status = S_MAXIT;
-      st->done = 0;
-      if (a.stop_rule == 1) {
-        st->conv = a.tol * sqrt(t[2]);
-        st->last = sqrt(t[1]);
-        if (st->last <= st->conv) { st->status = S_OK; st->done = 1; }
-      } else {
-        st->conv = a.tol * sqrt(fabs(t[0]));
-        st->last = sqrt(fabs(t[0]));
-      }
-      if (!finite(t[0])) { st->status = S_NONFINITE; st->done = 1; }
-      if (a.maxit <= 0 && !st->done) st->done = 1;
-    });
+    return;
   }
-  const int j = ld_int(&st->iter) + 1;
+  if (ph == B2_FINAL) {
+    // energy x.Ax (src/Iterativos.f:204-205) and ||b - Ax||^2
+    for (int i = i0 + (int)threadIdx.x; i < i1; i += blockDim.x) {
+      const double ax = row_apply<F>(a.A, i, XVec{a.x});
+      const double ri = a.b[i] - ax;
+      v[0] += a.x[i] * ax;
+      v[1] += ri * ri;
+    }
+    return;
+  }
+  const int j = ld_int(&S->iter) + 1;
   const bool odd = (j & 1) != 0;
   const double *pc = odd ? a.p0 : a.p1, *vc = odd ? a.v0 : a.v1;
   double *pn = odd ? a.p1 : a.p0, *vn = odd ? a.v1 : a.v0;
   if (ph == B2_P) {
-    double v[3] = {0.0, 0.0, 0.0};
     if (j == 1) {
       const XDiv xf{pc, a.m};
       for (int i = i0 + (int)threadIdx.x; i < i1; i += blockDim.x) {
@@ -716,8 +807,8 @@ __device__ bool bicg2_phase(const Bicg2Args &a, int ph, double *lds) {
         v[0] += vi * a.r[i];
       }
     } else {
-      const double alpha = ld_dbl(&st->alpha);
-      const XBi2Dir xf{a.s, a.t, pc, vc, a.m, ld_dbl(&st->beta), ld_dbl(&st->omega)};
+      const double alpha = ld_dbl(&S->alpha);
+      const XBi2Dir xf{a.s, a.t, pc, vc, a.m, ld_dbl(&S->beta), ld_dbl(&S->omega)};
       for (int i = i0 + (int)threadIdx.x; i < i1; i += blockDim.x) {
         const double mi = a.m[i];
         const double si = a.s[i];
@@ -734,67 +825,91 @@ __device__ bool bicg2_phase(const Bicg2Args &a, int ph, double *lds) {
         v[2] += bi * bi;
       }
     }
-    return grid_sync<3>(a.g, v, lds, [&](const double *t) {
-      if (j > 1) {
-        // the stop test of iteration j-1 (src/Iterativos.f:188-190)
-        const double beta1 = t[1] / st->d;
-        st->rnorm2 = t[2];
-        bool conv;
-        if (a.stop_rule == 1) {
-          st->last = sqrt(t[2]);
-          conv = st->last <= st->conv;
-        } else {
-          st->last = sqrt(fabs(beta1));
-          conv = st->last < st->conv;
-        }
-        if (conv) { st->status = S_OK; st->done = 1; return; }
-        if (!finite(beta1)) { st->status = S_NONFINITE; st->done = 1; return; }
-        if (j - 1 >= a.maxit) { st->status = S_MAXIT; st->done = 1; return; }
-      }
-      const double alpha = st->rr0 / t[0];
-      st->alpha = alpha;
-      if (t[0] == 0.0) { st->status = S_BREAKDOWN; st->done = 1; }
-      else if (!finite(alpha)) { st->status = S_NONFINITE; st->done = 1; }
-    });
+    return;
   }
-  if (ph == B2_Q) {
-    const XBiRes xf{a.bb, vn, a.m, ld_dbl(&st->alpha)};
-    double v[4] = {0.0, 0.0, 0.0, 0.0};
-    for (int i = i0 + (int)threadIdx.x; i < i1; i += blockDim.x) {
-      const double si = a.bb[i] - xf.alpha * vn[i];
-      const double ti = row_apply<F>(a.A, i, xf);
-      const double ri = a.r[i];
-      a.s[i] = si;
-      a.t[i] = ti;
-      v[0] += ti * si;
-      v[1] += ti * ti;
-      v[2] += ri * si;
-      v[3] += ri * ti;
-    }
-    return grid_sync<4>(a.g, v, lds, [&](const double *t) {
-      const double w = t[0] / t[1];
-      const double rho = t[2] - w * t[3];
-      const double beta = (rho / st->rr0) * (st->alpha / w);
-      st->omega = w;
-      st->beta = beta;
-      st->rr0 = rho;
-      st->iter = j;
-      if (t[1] == 0.0 || w == 0.0) { st->status = S_BREAKDOWN; st->done = 1; }
-      else if (!finite(w) || !finite(beta)) { st->status = S_NONFINITE; st->done = 1; }
-    });
-  }
-  // B2_FINAL: energy x.Ax (src/Iterativos.f:204-205) and ||b - Ax||^2
-  double v[2] = {0.0, 0.0};
+  // B2_Q
+  const XBiRes xf{a.bb, vn, a.m, ld_dbl(&S->alpha)};
   for (int i = i0 + (int)threadIdx.x; i < i1; i += blockDim.x) {
-    const double ax = row_apply<F>(a.A, i, XVec{a.x});
-    const double ri = a.b[i] - ax;
-    v[0] += a.x[i] * ax;
-    v[1] += ri * ri;
+    const double si = a.bb[i] - xf.alpha * vn[i];
+    const double ti = row_apply<F>(a.A, i, xf);
+    const double ri = a.r[i];
+    a.s[i] = si;
+    a.t[i] = ti;
+    v[0] += ti * si;
+    v[1] += ti * ti;
+    v[2] += ri * si;
+    v[3] += ri * ti;
   }
-  return grid_sync<2>(a.g, v, lds, [&](const double *t) {
+}
+
+__device__ inline void bicg2_epi(const Bicg2Args &a, int ph, KState *st, const double *t) {
+  if (ph == B2_SETUP) {
+    st->d = t[0];
+    st->rr0 = t[1];
+    st->rnorm2 = t[1];
+    st->bnorm2 = t[2];
+    st->iter = 0;
+    st->inner = 0;
+    st->status = S_MAXIT;
+    st->done = 0;
+    if (a.stop_rule == 1) {
+      st->conv = a.tol * sqrt(t[2]);
+      st->last = sqrt(t[1]);
+      if (st->last <= st->conv) { st->status = S_OK; st->done = 1; }
+    } else {
+      st->conv = a.tol * sqrt(fabs(t[0]));
+      st->last = sqrt(fabs(t[0]));
+    }
+    if (!finite(t[0])) { st->status = S_NONFINITE; st->done = 1; }
+    if (a.maxit <= 0 && !st->done) st->done = 1;
+  } else if (ph == B2_P) {
+    const int j = st->iter + 1;
+    if (j > 1) {
+      // the stop test of iteration j-1 (src/Iterativos.f:188-190)
+      const double beta1 = t[1] / st->d;
+      st->rnorm2 = t[2];
+      bool conv;
+      if (a.stop_rule == 1) {
+        st->last = sqrt(t[2]);
+        conv = st->last <= st->conv;
+      } else {
+        st->last = sqrt(fabs(beta1));
+        conv = st->last < st->conv;
+      }
+      if (conv) { st->status = S_OK; st->done = 1; return; }
+      if (!finite(beta1)) { st->status = S_NONFINITE; st->done = 1; return; }
+      if (j - 1 >= a.maxit) { st->status = S_MAXIT; st->done = 1; return; }
+    }
+    const double alpha = st->rr0 / t[0];
+    st->alpha = alpha;
+    if (t[0] == 0.0) { st->status = S_BREAKDOWN; st->done = 1; }
+    else if (!finite(alpha)) { st->status = S_NONFINITE; st->done = 1; }
+  } else if (ph == B2_Q) {
+    const double w = t[0] / t[1];
+    const double rho = t[2] - w * t[3];
+    const double beta = (rho / st->rr0) * (st->alpha / w);
+    st->omega = w;
+    st->beta = beta;
+    st->rr0 = rho;
+    st->iter = st->iter + 1;
+    if (t[1] == 0.0 || w == 0.0) { st->status = S_BREAKDOWN; st->done = 1; }
+    else if (!finite(w) || !finite(beta)) { st->status = S_NONFINITE; st->done = 1; }
+  } else {
     st->energy = t[0];
     st->res2 = t[1];
-  });
+  }
+}
+
+template <int F>
+__device__ bool bicg2_phase(const Bicg2Args &a, int ph, double *lds) {
+  double v[kMaxK] = {0.0, 0.0, 0.0, 0.0};
+  bicg2_rows<F>(a, ph, a.st, v);
+  auto epi = [&](const double *t) { bicg2_epi(a, ph, a.st, t); };
+  switch (bicg2_nred(ph)) {
+    case 2: return grid_sync<2>(a.g, v, lds, epi);
+    case 3: return grid_sync<3>(a.g, v, lds, epi);
+    default: return grid_sync<4>(a.g, v, lds, epi);
+  }
 }
 
 template <int F>
@@ -813,11 +928,19 @@ __global__ __launch_bounds__(1024) void bicg2_persistent(Bicg2Args a) {
   bicg2_phase<F>(a, B2_FINAL, lds);
 }
 
+// Kernel-per-phase engine: one launch per phase, the reduction deferred.
 template <int F>
 __global__ __launch_bounds__(1024) void bicg2_kernel(Bicg2Args a) {
   __shared__ double lds[kMaxK * kMaxWaves];
-  if ((a.phase == B2_P || a.phase == B2_Q) && ld_int(&a.st->done)) return;
-  bicg2_phase<F>(a, a.phase, lds);
+  __shared__ KState S;
+  struct M {
+    const Bicg2Args &a;
+    __device__ int nred(int ph) const { return bicg2_nred(ph); }
+    __device__ bool skip(int ph, const KState &s) const { return (ph == B2_P || ph == B2_Q) && s.done; }
+    __device__ void rows(int ph, const KState *s, double *v) const { bicg2_rows<F>(a, ph, s, v); }
+    __device__ void epi(int ph, KState *s, const double *t) const { bicg2_epi(a, ph, s, t); }
+  };
+  deferred_phase(a.g, a.d, a.phase, M{a}, &S, lds);
 }
 
 // ---------------------------------------------------------------- GMRES --

@@ -88,6 +88,10 @@ __global__ void scale_to_f32_kernel(Mat A, const double *m, float *av32, double 
 
 enum { MX_SETUP = 0, MX_A = 1, MX_B = 2, MX_X = 3, MX_R = 4, MX_FINAL = 5 };
 
+__host__ __device__ constexpr int mixed_nred(int ph) {
+  return ph == MX_A ? 1 : ph == MX_X ? 0 : (ph == MX_B || ph == MX_FINAL) ? 2 : 3;
+}
+
 struct MixedArgs {
   Mat A;         // FP64 matrix (residual recomputation, final energy)
   Mat32 A32;     // FP32 scaled matrix (inner iterations)
@@ -101,20 +105,24 @@ struct MixedArgs {
   float *y, *r, *p0, *p1, *q;
   double tol, delta;
   int maxit, x0_zero, max_inner, phase;
+  Defer d;
 };
 
 // KState use: d = r~.r~ (FP64 accumulation), d_prev = its value before the
 // current iteration, conv = tol*||b||, last = the latest residual estimate,
 // upd_est = the estimate at the last reliable update, cycle = number of
-// reliable updates, inner = FP32 iterations since it, step_i = update due.
+// reliable updates, inner = FP32 iterations since it, step_i = update due,
+// next = the phase that follows (the order depends on the iterates: after
+// B either the next iteration's A or an update's X and R).
+// Row and scalar parts are split as for the one-reduction PCG, so that the
+// persistent kernel, the batched kernel and the kernel-per-phase engine with
+// deferred reductions all drive the same code.
 template <int F>
-__device__ bool mixed_phase(const MixedArgs &a, int ph, double *lds) {
+__device__ void mixed_rows(const MixedArgs &a, int ph, const KState *S, double *v) {
   const int i0 = a.g.row_begin(), i1 = a.g.row_end();
-  KState *st = a.st;
   if (ph == MX_SETUP || ph == MX_R) {
     // r64 = b - A x (FP64); r~ = S r64 rounded to FP32; y = 0
     const bool zero = (ph == MX_SETUP) && a.x0_zero;
-    double v[3] = {0.0, 0.0, 0.0};
     for (int i = i0 + (int)threadIdx.x; i < i1; i += blockDim.x) {
       double ax;
       if (zero) {
@@ -132,39 +140,13 @@ __device__ bool mixed_phase(const MixedArgs &a, int ph, double *lds) {
       v[1] += ri * ri;
       v[2] += bi * bi;
     }
-    return grid_sync<3>(a.g, v, lds, [&](const double *t) {
-      if (ph == MX_SETUP) {
-        st->bnorm2 = t[2];
-        st->conv = a.tol * sqrt(t[2]);
-        st->iter = 0;
-        st->cycle = 0;
-        st->status = S_MAXIT;
-        st->done = 0;
-      } else {
-        st->cycle = st->cycle + 1;
-      }
-      st->res2 = t[1];
-      st->last = sqrt(t[1]);
-      st->upd_est = st->last;
-      // the recomputed residual replaces the recursive one; the direction
-      // keeps its history: beta = r~new.r~new / r~old.r~old
-      if (ph == MX_R) st->beta = t[0] / st->d_prev;
-      st->d = t[0];
-      st->inner = 0;
-      st->step_i = 0;
-      if (st->last <= st->conv) { st->status = S_OK; st->done = 1; }
-      else if (!finite(t[0]) || !finite(t[1])) { st->status = S_NONFINITE; st->done = 1; }
-      else if (st->iter >= a.maxit) { st->status = S_MAXIT; st->done = 1; }
-    });
-  }
-  if (ph == MX_A) {
-    // p = r~ + beta p (on the fly, ping-pong); q = (S A S) p; alpha = r~.r~ / p.q
-    const int j = ld_int(&st->iter) + 1;
+  } else if (ph == MX_A) {
+    // p = r~ + beta p (on the fly, ping-pong); q = (S A S) p; -> p.q
+    const int j = ld_int(&S->iter) + 1;
     float *pn = (j & 1) ? a.p1 : a.p0;
     const float *po = (j & 1) ? a.p0 : a.p1;
     const bool restart = j == 1;
-    double v[1] = {0.0};
-    const XDir32 xf{a.r, po, restart ? 0.0f : (float)ld_dbl(&st->beta)};
+    const XDir32 xf{a.r, po, restart ? 0.0f : (float)ld_dbl(&S->beta)};
     for (int i = i0 + (int)threadIdx.x; i < i1; i += blockDim.x) {
       float qi, pi;
       if (restart) {
@@ -178,18 +160,11 @@ __device__ bool mixed_phase(const MixedArgs &a, int ph, double *lds) {
       a.q[i] = qi;
       v[0] += (double)pi * (double)qi;
     }
-    return grid_sync<1>(a.g, v, lds, [&](const double *t) {
-      st->alpha = st->d / t[0];
-      if (t[0] == 0.0) { st->status = S_BREAKDOWN; st->done = 1; }
-      else if (!finite(st->alpha)) { st->status = S_NONFINITE; st->done = 1; }
-    });
-  }
-  if (ph == MX_B) {
-    // y += alpha p; r~ -= alpha q; estimate ||S^-1 r~||; decide on an update
-    const int j = ld_int(&st->iter) + 1;
+  } else if (ph == MX_B) {
+    // y += alpha p; r~ -= alpha q; -> r~.r~ and the unscaled estimate ||S^-1 r~||^2
+    const int j = ld_int(&S->iter) + 1;
     const float *pn = (j & 1) ? a.p1 : a.p0;
-    const float alpha = (float)ld_dbl(&st->alpha);
-    double v[2] = {0.0, 0.0};
+    const float alpha = (float)ld_dbl(&S->alpha);
     for (int i = i0 + (int)threadIdx.x; i < i1; i += blockDim.x) {
       const float yi = a.y[i] + alpha * pn[i];
       const float ri = a.r[i] - alpha * a.q[i];
@@ -199,38 +174,91 @@ __device__ bool mixed_phase(const MixedArgs &a, int ph, double *lds) {
       const double ru = (double)ri * (double)a.dsq[i];
       v[1] += ru * ru;
     }
-    return grid_sync<2>(a.g, v, lds, [&](const double *t) {
-      st->iter = j;
-      st->inner = st->inner + 1;
-      const double est = sqrt(t[1]);
-      st->last = est;
-      st->d_prev = st->d;
-      st->beta = t[0] / st->d;
-      st->d = t[0];
-      const bool upd = est <= st->conv || est < a.delta * st->upd_est || st->inner >= a.max_inner ||
-                       j >= a.maxit || !finite(t[0]);
-      st->step_i = upd ? 1 : 0;
-      if (t[0] == 0.0) st->step_i = 1;
-    });
-  }
-  if (ph == MX_X) {
+  } else if (ph == MX_X) {
     // fold the FP32 increment into the FP64 solution: x += S y
     for (int i = i0 + (int)threadIdx.x; i < i1; i += blockDim.x) a.x[i] = a.x[i] + a.s[i] * (double)a.y[i];
-    double v[1] = {0.0};
-    return grid_sync<0>(a.g, v, lds, [&](const double *) {});
+  } else {
+    // MX_FINAL: energy x.Ax and ||b - Ax||^2 in FP64
+    for (int i = i0 + (int)threadIdx.x; i < i1; i += blockDim.x) {
+      const double ax = row_apply<F>(a.A, i, XVec{a.x});
+      const double ri = a.b[i] - ax;
+      v[0] += a.x[i] * ax;
+      v[1] += ri * ri;
+    }
   }
-  // MX_FINAL: energy x.Ax and ||b - Ax||^2 in FP64
-  double v[2] = {0.0, 0.0};
-  for (int i = i0 + (int)threadIdx.x; i < i1; i += blockDim.x) {
-    const double ax = row_apply<F>(a.A, i, XVec{a.x});
-    const double ri = a.b[i] - ax;
-    v[0] += a.x[i] * ax;
-    v[1] += ri * ri;
-  }
-  return grid_sync<2>(a.g, v, lds, [&](const double *t) {
+}
+
+__device__ inline void mixed_epi(const MixedArgs &a, int ph, KState *st, const double *t) {
+  if (ph == MX_SETUP || ph == MX_R) {
+    if (ph == MX_SETUP) {
+      st->bnorm2 = t[2];
+      st->conv = a.tol * sqrt(t[2]);
+      st->iter = 0;
+      st->cycle = 0;
+      st->status = S_MAXIT;
+      st->done = 0;
+    } else {
+      st->cycle = st->cycle + 1;
+    }
+    st->res2 = t[1];
+    st->last = sqrt(t[1]);
+    st->upd_est = st->last;
+    // the recomputed residual replaces the recursive one; the direction
+    // keeps its history: beta = r~new.r~new / r~old.r~old
+    if (ph == MX_R) st->beta = t[0] / st->d_prev;
+    st->d = t[0];
+    st->inner = 0;
+    st->step_i = 0;
+    st->next = MX_A;
+    if (st->last <= st->conv) { st->status = S_OK; st->done = 1; }
+    else if (!finite(t[0]) || !finite(t[1])) { st->status = S_NONFINITE; st->done = 1; }
+    else if (st->iter >= a.maxit) { st->status = S_MAXIT; st->done = 1; }
+  } else if (ph == MX_A) {
+    st->alpha = st->d / t[0];
+    st->next = MX_B;
+    if (t[0] == 0.0) { st->status = S_BREAKDOWN; st->done = 1; }
+    else if (!finite(st->alpha)) { st->status = S_NONFINITE; st->done = 1; }
+  } else if (ph == MX_B) {
+    const int j = st->iter + 1;
+    st->iter = j;
+    st->inner = st->inner + 1;
+    const double est = sqrt(t[1]);
+    st->last = est;
+    st->d_prev = st->d;
+    st->beta = t[0] / st->d;
+    st->d = t[0];
+    const bool upd = est <= st->conv || est < a.delta * st->upd_est || st->inner >= a.max_inner ||
+                     j >= a.maxit || !finite(t[0]) || t[0] == 0.0;
+    st->step_i = upd ? 1 : 0;
+    st->next = upd ? MX_X : MX_A;
+  } else if (ph == MX_X) {
+    st->next = MX_R;
+  } else {
     st->energy = t[0];
     st->res2 = t[1];
-  });
+  }
+}
+
+template <int F>
+__device__ bool mixed_phase(const MixedArgs &a, int ph, double *lds) {
+  double v[kMaxK] = {0.0, 0.0, 0.0, 0.0};
+  mixed_rows<F>(a, ph, a.st, v);
+  auto epi = [&](const double *t) { mixed_epi(a, ph, a.st, t); };
+  switch (mixed_nred(ph)) {
+    case 0: return grid_sync<0>(a.g, v, lds, epi);
+    case 1: return grid_sync<1>(a.g, v, lds, epi);
+    case 2: return grid_sync<2>(a.g, v, lds, epi);
+    default: return grid_sync<3>(a.g, v, lds, epi);
+  }
+}
+
+// Persistent driver (one system, or one system's block group of a batch).
+template <int F>
+__device__ void mixed_run(const MixedArgs &a, double *lds) {
+  if (!mixed_phase<F>(a, MX_SETUP, lds)) return;
+  while (!ld_int(&a.st->done))
+    if (!mixed_phase<F>(a, ld_int(&a.st->next), lds)) return;
+  mixed_phase<F>(a, MX_FINAL, lds);
 }
 
 template <int F>
@@ -240,26 +268,23 @@ __global__ __launch_bounds__(1024) void mixed_persistent(MixedArgs a) {
   if (threadIdx.x == 0) s_ep = 0u;
   __syncthreads();
   a.g.ep = &s_ep;
-  if (!mixed_phase<F>(a, MX_SETUP, lds)) return;
-  while (!ld_int(&a.st->done)) {
-    if (!mixed_phase<F>(a, MX_A, lds)) return;
-    if (ld_int(&a.st->done)) break;
-    if (!mixed_phase<F>(a, MX_B, lds)) return;
-    if (ld_int(&a.st->step_i)) {
-      if (!mixed_phase<F>(a, MX_X, lds)) return;
-      if (!mixed_phase<F>(a, MX_R, lds)) return;
-    }
-  }
-  mixed_phase<F>(a, MX_FINAL, lds);
+  mixed_run<F>(a, lds);
 }
 
+// Kernel-per-phase engine: a.phase is MX_SETUP, kPhaseNext, MX_FINAL or a
+// flush; reductions deferred to the next launch (fvc_krylov.h, Defer).
 template <int F>
 __global__ __launch_bounds__(1024) void mixed_kernel(MixedArgs a) {
   __shared__ double lds[kMaxK * kMaxWaves];
-  const int ph = a.phase;
-  if (ph != MX_SETUP && ph != MX_FINAL && ld_int(&a.st->done)) return;
-  if ((ph == MX_X || ph == MX_R) && !ld_int(&a.st->step_i)) return;
-  mixed_phase<F>(a, ph, lds);
+  __shared__ KState S;
+  struct M {
+    const MixedArgs &a;
+    __device__ int nred(int ph) const { return mixed_nred(ph); }
+    __device__ bool skip(int ph, const KState &s) const { return ph != MX_SETUP && ph != MX_FINAL && s.done; }
+    __device__ void rows(int ph, const KState *s, double *v) const { mixed_rows<F>(a, ph, s, v); }
+    __device__ void epi(int ph, KState *s, const double *t) const { mixed_epi(a, ph, s, t); }
+  };
+  deferred_phase(a.g, a.d, a.phase, M{a}, &S, lds);
 }
 
 }  // namespace fvc

@@ -56,8 +56,10 @@ struct DevState {
   char *arena = nullptr;
   size_t arena_bytes = 0;
   KState *host_st = nullptr;   // pinned
+  KState *host_ring = nullptr; // pinned, two slots (overlapped done-checks)
   double *host_scalar = nullptr;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  hipEvent_t evq[2] = {nullptr, nullptr};
 };
 
 DevState g_dev[64];
@@ -77,6 +79,9 @@ int current_device(DevState **out) {
       d.cus = prop.multiProcessorCount;
       std::snprintf(d.name, sizeof d.name, "%s (%s)", prop.name, prop.gcnArchName);
       HIPCHK(hipHostMalloc((void **)&d.host_st, sizeof(KState), hipHostMallocDefault));
+      HIPCHK(hipHostMalloc((void **)&d.host_ring, 2 * sizeof(KState), hipHostMallocDefault));
+      HIPCHK(hipEventCreateWithFlags(&d.evq[0], hipEventDisableTiming));
+      HIPCHK(hipEventCreateWithFlags(&d.evq[1], hipEventDisableTiming));
       HIPCHK(hipHostMalloc((void **)&d.host_scalar, 64, hipHostMallocDefault));
       HIPCHK(hipEventCreate(&d.ev0));
       HIPCHK(hipEventCreate(&d.ev1));
@@ -232,6 +237,40 @@ int read_state(DevState &d, KState *st, hipStream_t s) {
 void fill_stats(fvc_stats *stats, const KState &hs, int code, bool persistent, float ms, int G, int bs);
 int finish(const KState &hs, unsigned *ctr);
 
+// Kernel-per-phase driver: enqueues iterations in batches and checks for
+// convergence one batch behind, so the GPU never idles on the host.  After
+// each batch the latest state goes to pinned host memory behind an event;
+// the host then waits only for the previous batch's event, while the GPU
+// runs the current one.  Launches after convergence find `done` set and
+// return at once (at most one batch of them).  `iteration()` enqueues one
+// iteration; `state()` is the device state the last launch wrote.
+template <class Iter, class State>
+int drive_batches(DevState &d, hipStream_t s, int maxit, Iter &&iteration, State &&state) {
+  constexpr int kMaxBatch = 8;
+  int launched = 0, batch = 2, k = 0;
+  bool waiting[2] = {false, false};
+  while (launched < maxit) {
+    const int nb = std::min(batch, maxit - launched);
+    for (int it = 0; it < nb; ++it) {
+      const int rc = iteration();
+      if (rc) return rc;
+    }
+    launched += nb;
+    const int slot = k & 1, prev = slot ^ 1;
+    HIPCHK(hipMemcpyAsync(d.host_ring + slot, state(), sizeof(KState), hipMemcpyDeviceToHost, s));
+    HIPCHK(hipEventRecord(d.evq[slot], s));
+    waiting[slot] = true;
+    if (waiting[prev]) {
+      HIPCHK(hipEventSynchronize(d.evq[prev]));
+      waiting[prev] = false;
+      if (d.host_ring[prev].done) break;
+    }
+    ++k;
+    batch = std::min(batch * 2, kMaxBatch);
+  }
+  return FVC_OK;
+}
+
 template <int F>
 int run_solve(DevState &d, const SolveIn &in, fvc_stats *stats, hipStream_t s) {
   const int neq = in.A.neq;
@@ -260,7 +299,11 @@ int run_solve(DevState &d, const SolveIn &in, fvc_stats *stats, hipStream_t s) {
   if (rc) return rc;
   if (occ < 1) return fail(FVC_DEVICE_ERROR, "persistent kernel does not fit a CU at block size %d", bs);
   const int G = choose_blocks(neq, d.cus, occ, in.G_req);
-  bool persistent = in.engine != FVC_ENGINE_KERNELS;
+  // AUTO: PCG and BiCGSTAB run one kernel per phase (with kernel boundaries
+  // for ordering, a phase needs no fence and no atomic; measured faster on
+  // MI355X than the grid barrier of one persistent launch); GMRES, whose
+  // Arnoldi step is a chain of short phases, runs persistent.
+  bool persistent = in.engine == FVC_ENGINE_PERSISTENT || (in.engine == FVC_ENGINE_AUTO && in.code == FVC_GMRES);
   if (persistent && G > occ * d.cus) {
     if (in.engine == FVC_ENGINE_PERSISTENT)
       return fail(FVC_BAD_ARGUMENT, "%d blocks exceed the %d co-resident blocks of a persistent launch", G, occ * d.cus);
@@ -275,13 +318,16 @@ int run_solve(DevState &d, const SolveIn &in, fvc_stats *stats, hipStream_t s) {
   int nvec = in.code == FVC_PCG ? (fused ? 7 : 5) : in.code == FVC_BICGSTAB ? (fused ? 8 : 10) : (k + 1);
   size_t bytes = align_up((kCtrWords + (size_t)G) * sizeof(unsigned)) + align_up(sizeof(KState)) +
                  align_up(kMaxK * (size_t)G * sizeof(double)) + (size_t)nvec * align_up(n * sizeof(double)) +
-                 align_up((size_t)(k + 2) * (k + 2) * sizeof(double)) * 2 + align_up(n * sizeof(double)) + 4096;
+                 align_up((size_t)(k + 2) * (k + 2) * sizeof(double)) * 2 + align_up(n * sizeof(double)) +
+                 align_up(2 * sizeof(KState)) + align_up(2 * kMaxK * (size_t)G * sizeof(double)) + 4096;
   rc = ensure_arena(d, bytes);
   if (rc) return rc;
   Carve cv{d.arena};
   unsigned *ctr = cv.take<unsigned>(kCtrWords + (size_t)G);
   KState *st = cv.take<KState>(1);
   double *partials = cv.take<double>(kMaxK * (size_t)G);
+  // deferred reductions (kernel-per-phase engine of the fused methods)
+  Defer dfr{cv.take<KState>(2), cv.take<double>(2 * kMaxK * (size_t)G), 0};
   std::vector<double *> vec(nvec);
   for (int i = 0; i < nvec; ++i) vec[i] = cv.take<double>(n);
   double *gsmall = cv.take<double>((size_t)(k + 2) * (k + 2) * 2);
@@ -306,23 +352,24 @@ int run_solve(DevState &d, const SolveIn &in, fvc_stats *stats, hipStream_t s) {
       HIPCHK(hipEventRecord(d.ev1, s));
       if ((rc = read_state(d, st, s))) return rc;
     } else {
-      a.phase = CG1_SETUP;
-      if ((rc = launch(kfn, false, G, bs, a, s))) return rc;
-      a.phase = CG1_W0;
-      if ((rc = launch(kfn, false, G, bs, a, s))) return rc;
-      int batch = 16, launched = 0;
-      a.phase = CG1_ITER;
-      for (;;) {
-        for (int it = 0; it < batch && launched < in.maxit; ++it, ++launched)
-          if ((rc = launch(kfn, false, G, bs, a, s))) return rc;
-        if ((rc = read_state(d, st, s))) return rc;
-        if (d.host_st->done || launched >= in.maxit) break;
-        batch = std::min(batch * 2, 512);
-      }
-      a.phase = CG1_FINAL;
-      if ((rc = launch(kfn, false, G, bs, a, s))) return rc;
+      // each launch reads state/partials set par and writes set 1 - par
+      auto phase = [&](int ph) {
+        a.phase = ph;
+        a.d = dfr;
+        dfr.par ^= 1;
+        return launch(kfn, false, G, bs, a, s);
+      };
+      hipLaunchKernelGGL(reset_kernel, dim3(1), dim3(256), 0, s, ctr, 0, dfr.st, 2);
+      if ((rc = phase(CG1_SETUP))) return rc;
+      if ((rc = phase(CG1_W0))) return rc;
+      if ((rc = drive_batches(d, s, in.maxit, [&] { return phase(CG1_ITER); },
+                              [&] { return dfr.st + dfr.par; })))
+        return rc;
+      if ((rc = phase(CG1_ITER))) return rc;   // applies the last iteration's recurrence
+      if ((rc = phase(CG1_FINAL))) return rc;
+      if ((rc = phase(-1))) return rc;          // applies FINAL's
       HIPCHK(hipEventRecord(d.ev1, s));
-      if ((rc = read_state(d, st, s))) return rc;
+      if ((rc = read_state(d, dfr.st + dfr.par, s))) return rc;
     }
   } else if (in.code == FVC_BICGSTAB && fused) {
     Bicg2Args a{};
@@ -336,26 +383,27 @@ int run_solve(DevState &d, const SolveIn &in, fvc_stats *stats, hipStream_t s) {
       HIPCHK(hipEventRecord(d.ev1, s));
       if ((rc = read_state(d, st, s))) return rc;
     } else {
-      a.phase = B2_SETUP;
-      if ((rc = launch(kfn, false, G, bs, a, s))) return rc;
-      int batch = 8, launched = 0;
-      for (;;) {
-        for (int it = 0; it < batch && launched < in.maxit; ++it, ++launched)
-          for (int ph = B2_P; ph <= B2_Q; ++ph) {
-            a.phase = ph;
-            if ((rc = launch(kfn, false, G, bs, a, s))) return rc;
-          }
-        if ((rc = read_state(d, st, s))) return rc;
-        if (d.host_st->done || launched >= in.maxit) break;
-        batch = std::min(batch * 2, 256);
-      }
-      // the P phase that completes the last iteration and decides its stop test
-      a.phase = B2_P;
-      if ((rc = launch(kfn, false, G, bs, a, s))) return rc;
-      a.phase = B2_FINAL;
-      if ((rc = launch(kfn, false, G, bs, a, s))) return rc;
+      auto phase = [&](int ph) {
+        a.phase = ph;
+        a.d = dfr;
+        dfr.par ^= 1;
+        return launch(kfn, false, G, bs, a, s);
+      };
+      hipLaunchKernelGGL(reset_kernel, dim3(1), dim3(256), 0, s, ctr, 0, dfr.st, 2);
+      if ((rc = phase(B2_SETUP))) return rc;
+      auto iteration = [&] {
+        const int r = phase(B2_P);
+        return r ? r : phase(B2_Q);
+      };
+      if ((rc = drive_batches(d, s, in.maxit, iteration, [&] { return dfr.st + dfr.par; }))) return rc;
+      // the P phase that completes the last iteration and decides its stop
+      // test, then one that applies that decision
+      if ((rc = phase(B2_P))) return rc;
+      if ((rc = phase(B2_Q))) return rc;
+      if ((rc = phase(B2_FINAL))) return rc;
+      if ((rc = phase(-1))) return rc;
       HIPCHK(hipEventRecord(d.ev1, s));
-      if ((rc = read_state(d, st, s))) return rc;
+      if ((rc = read_state(d, dfr.st + dfr.par, s))) return rc;
     }
   } else if (in.code == FVC_PCG) {
     PcgArgs a{};
@@ -372,18 +420,14 @@ int run_solve(DevState &d, const SolveIn &in, fvc_stats *stats, hipStream_t s) {
     } else {
       a.phase = PCG_SETUP;
       if ((rc = launch(kfn, false, G, bs, a, s))) return rc;
-      int batch = 8, launched = 0;
-      for (;;) {
-        for (int it = 0; it < batch && launched < in.maxit; ++it, ++launched) {
-          a.phase = PCG_A;
-          if ((rc = launch(kfn, false, G, bs, a, s))) return rc;
-          a.phase = PCG_B;
-          if ((rc = launch(kfn, false, G, bs, a, s))) return rc;
-        }
-        if ((rc = read_state(d, st, s))) return rc;
-        if (d.host_st->done || launched >= in.maxit) break;
-        batch = std::min(batch * 2, 256);
-      }
+      auto iteration = [&] {
+        a.phase = PCG_A;
+        const int r = launch(kfn, false, G, bs, a, s);
+        if (r) return r;
+        a.phase = PCG_B;
+        return launch(kfn, false, G, bs, a, s);
+      };
+      if ((rc = drive_batches(d, s, in.maxit, iteration, [&] { return st; }))) return rc;
       a.phase = PCG_FINAL;
       if ((rc = launch(kfn, false, G, bs, a, s))) return rc;
       HIPCHK(hipEventRecord(d.ev1, s));
@@ -403,17 +447,15 @@ int run_solve(DevState &d, const SolveIn &in, fvc_stats *stats, hipStream_t s) {
     } else {
       a.phase = BI_SETUP;
       if ((rc = launch(kfn, false, G, bs, a, s))) return rc;
-      int batch = 8, launched = 0;
-      for (;;) {
-        for (int it = 0; it < batch && launched < in.maxit; ++it, ++launched)
-          for (int ph = BI_F1; ph <= BI_F3; ++ph) {
-            a.phase = ph;
-            if ((rc = launch(kfn, false, G, bs, a, s))) return rc;
-          }
-        if ((rc = read_state(d, st, s))) return rc;
-        if (d.host_st->done || launched >= in.maxit) break;
-        batch = std::min(batch * 2, 256);
-      }
+      auto iteration = [&] {
+        for (int ph = BI_F1; ph <= BI_F3; ++ph) {
+          a.phase = ph;
+          const int r = launch(kfn, false, G, bs, a, s);
+          if (r) return r;
+        }
+        return 0;
+      };
+      if ((rc = drive_batches(d, s, in.maxit, iteration, [&] { return st; }))) return rc;
       a.phase = BI_FINAL;
       if ((rc = launch(kfn, false, G, bs, a, s))) return rc;
       HIPCHK(hipEventRecord(d.ev1, s));
@@ -518,7 +560,10 @@ int run_mixed(DevState &d, const SolveIn &in, fvc_stats *stats, hipStream_t s) {
     int rc = occupancy(pfn, bs, &occ);
     if (rc) return rc;
     const int G = choose_blocks(neq, d.cus, occ, in.G_req);
-    bool persistent = in.engine != FVC_ENGINE_KERNELS && G <= occ * d.cus;
+    // AUTO runs one kernel per phase, as for FP64 PCG
+    const bool persistent = in.engine == FVC_ENGINE_PERSISTENT;
+    if (persistent && G > occ * d.cus)
+      return fail(FVC_BAD_ARGUMENT, "%d blocks exceed the %d co-resident blocks of a persistent launch", G, occ * d.cus);
     const size_t n = (size_t)neq;
     const size_t nval = (F == F_CSRD) ? (size_t)0 : (size_t)in.A.w * n;
     size_t nv = nval;
@@ -529,12 +574,14 @@ int run_mixed(DevState &d, const SolveIn &in, fvc_stats *stats, hipStream_t s) {
     }
     size_t bytes = align_up((kCtrWords + (size_t)G) * sizeof(unsigned)) + align_up(sizeof(KState)) +
                    align_up(kMaxK * (size_t)G * sizeof(double)) + align_up(nv * sizeof(float)) +
-                   align_up(n * sizeof(double)) + 6 * align_up(n * sizeof(float)) + 4096;
+                   align_up(n * sizeof(double)) + 6 * align_up(n * sizeof(float)) +
+                   align_up(2 * sizeof(KState)) + align_up(2 * kMaxK * (size_t)G * sizeof(double)) + 4096;
     if ((rc = ensure_arena(d, bytes))) return rc;
     Carve cv{d.arena};
     unsigned *ctr = cv.take<unsigned>(kCtrWords + (size_t)G);
     KState *st = cv.take<KState>(1);
     double *partials = cv.take<double>(kMaxK * (size_t)G);
+    Defer dfr{cv.take<KState>(2), cv.take<double>(2 * kMaxK * (size_t)G), 0};
     float *av32 = cv.take<float>(std::max<size_t>(nv, 1));
     double *sc = cv.take<double>(n);
     float *dsq = cv.take<float>(n);
@@ -567,28 +614,30 @@ int run_mixed(DevState &d, const SolveIn &in, fvc_stats *stats, hipStream_t s) {
     hipLaunchKernelGGL(scale_to_f32_kernel<F>, dim3((neq + 255) / 256), dim3(256), 0, s, in.A, in.m,
                        av32, sc, dsq);
     HIPCHK(hipGetLastError());
+    const KState *final_state = st;
     if (persistent) {
       a.phase = -1;
       if ((rc = launch(pfn, true, G, bs, a, s))) return rc;
     } else {
-      a.phase = MX_SETUP;
-      if ((rc = launch(kfn, false, G, bs, a, s))) return rc;
-      int batch = 8, launched = 0;
-      for (;;) {
-        for (int it = 0; it < batch && launched < in.maxit; ++it, ++launched)
-          for (int ph : {MX_A, MX_B, MX_X, MX_R}) {
-            a.phase = ph;
-            if ((rc = launch(kfn, false, G, bs, a, s))) return rc;
-          }
-        if ((rc = read_state(d, st, s))) return rc;
-        if (d.host_st->done || launched >= in.maxit) break;
-        batch = std::min(batch * 2, 256);
-      }
-      a.phase = MX_FINAL;
-      if ((rc = launch(kfn, false, G, bs, a, s))) return rc;
+      auto phase = [&](int ph) {
+        a.phase = ph;
+        a.d = dfr;
+        dfr.par ^= 1;
+        return launch(kfn, false, G, bs, a, s);
+      };
+      hipLaunchKernelGGL(reset_kernel, dim3(1), dim3(256), 0, s, ctr, 0, dfr.st, 2);
+      if ((rc = phase(MX_SETUP))) return rc;
+      // an inner iteration is two launches (A, B), a reliable update two more
+      // (X, R), and the device stops at maxit: this many launches reach `done`
+      const int cap = 4 * std::max(in.maxit, 0) + 16;
+      if ((rc = drive_batches(d, s, cap, [&] { return phase(kPhaseNext); }, [&] { return dfr.st + dfr.par; })))
+        return rc;
+      if ((rc = phase(MX_FINAL))) return rc;
+      if ((rc = phase(-1))) return rc;
+      final_state = dfr.st + dfr.par;
     }
     HIPCHK(hipEventRecord(d.ev1, s));
-    if ((rc = read_state(d, st, s))) return rc;
+    if ((rc = read_state(d, const_cast<KState *>(final_state), s))) return rc;
     float ms = 0.f;
     HIPCHK(hipEventElapsedTime(&ms, d.ev0, d.ev1));
     fill_stats(stats, *d.host_st, FVC_PCG, persistent, ms, G, bs);
@@ -980,14 +1029,12 @@ int run_batch(DevState &d, int nsys, const std::vector<int> &row_off, int w, con
   B.stop_rule = o.stop_rule;
   B.mixed = mixed;
   if (mixed) {
-    B.f0 = cv.take<float>(n); B.f1 = cv.take<float>(n); B.f2 = cv.take<float>(n);
-    B.f3 = cv.take<float>(n); B.f4 = cv.take<float>(n);
+    for (float *&f : B.f) f = cv.take<float>(n);
     B.av32 = cv.take<float>(n * w);
     B.s = cv.take<double>(n);
     B.dsq = cv.take<float>(n);
   } else {
-    B.v0 = cv.take<double>(n); B.v1 = cv.take<double>(n); B.v2 = cv.take<double>(n);
-    B.v3 = cv.take<double>(n); B.v4 = cv.take<double>(n);
+    for (double *&v : B.v) v = cv.take<double>(n);
   }
   B.ctr = ctr;
   B.flags = ctr + (size_t)wave * kCtrWords;

@@ -65,9 +65,12 @@ extern "C" {
 #define FVC_STOP_RELRES 1       /* recursive ||r||_2 <= tol*||b||_2, then a true-residual check */
 
 /* Engines. */
-#define FVC_ENGINE_AUTO 0
-#define FVC_ENGINE_KERNELS 1     /* one kernel per phase, reductions by the last block */
-#define FVC_ENGINE_PERSISTENT 2  /* one cooperative kernel per solve, grid barriers     */
+#define FVC_ENGINE_AUTO 0        /* KERNELS for PCG and BiCGSTAB, PERSISTENT for GMRES   */
+#define FVC_ENGINE_KERNELS 1     /* one kernel per phase; fused recurrences sum the block
+                                    partials at the next launch, the others in the last
+                                    block of the launch                                 */
+#define FVC_ENGINE_PERSISTENT 2  /* one launch per solve, all blocks co-resident, grid
+                                    barriers with bounded waits                         */
 
 /* Arithmetic of the iteration. */
 #define FVC_PREC_FP64 0         /* everything in FP64 (the reference's real*8)              */

@@ -49,6 +49,21 @@ def test_mixed_engines_bit_identical(fvc):
     xk, sk = fvc.solve(f, fvc.PCG, precision=fvc.PREC_MIXED, engine=fvc.ENGINE_KERNELS)
     assert sp["status"] == 0 and sk["status"] == 0
     assert np.array_equal(xp, xk) and sp["iterations"] == sk["iterations"]
+    assert sp["inner_iterations"] == sk["inner_iterations"]
+
+
+def test_mixed_maxit_and_warm_start(fvc):
+    # maxit stops the FP32 iteration count; a warm start from the converged
+    # solution needs no iteration at all
+    s = P.config_pressure_lognormal(32)
+    f = P.to_ell(s)
+    for engine in (fvc.ENGINE_PERSISTENT, fvc.ENGINE_KERNELS):
+        _, st = fvc.solve(f, fvc.PCG, precision=fvc.PREC_MIXED, engine=engine, maxit=5)
+        assert st["status"] == 1 and st["iterations"] == 5, st
+        x, st = fvc.solve(f, fvc.PCG, precision=fvc.PREC_MIXED, engine=engine, stop_rule=fvc.STOP_RELRES)
+        assert st["status"] == 0
+        x2, st2 = fvc.solve(f, fvc.PCG, x0=x, precision=fvc.PREC_MIXED, engine=engine, x0_rule=fvc.X0_GIVEN)
+        assert st2["status"] == 0 and st2["iterations"] == 0 and np.array_equal(x2, x)
 
 
 def test_mixed_unsupported_combinations(fvc):
