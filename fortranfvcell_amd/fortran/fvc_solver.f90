@@ -88,13 +88,26 @@ subroutine solver(ad, au, al, x, b, iax, ja, neq, nad, unsym, tol, maxit, code, 
   logical :: openmpCell, openmpSolver
   common /omp_control/ num_threads, openmpCell, openmpSolver, nThreadsCell, &
                        nThreadsSolver, nThreadsDot, nThreadsMatVec
+  ! COMMON /timectrl/ as laid out in include/time.fi:10-12.  The reference's
+  ! solver leaves its start stamp and elapsed time in time0/time
+  ! (src/Iterativos.f:42,85-86,153,208) and its kernels accumulate matvectime,
+  ! dottime and precontime (src/Matvec.f:27-37,394-399; src/Solver.f:83-85).
+  ! The GPU fuses SpMV, dot products and the Jacobi step into the same
+  ! kernels, so the whole solve time is added to matvectime.
+  real*8 :: solvTime, elmTime, vecTime, dsTime, numeqTime, reordtime, totaltime, matvectime, &
+            dottime, time, time0, readtime, viztime, grtime, precontime
+  common /timectrl/ solvTime, elmTime, vecTime, dsTime, numeqTime, reordtime, totaltime, &
+                    matvectime, dottime, time, time0, readtime, viztime, grtime, precontime
   type(fvc_stats_t) :: st
   integer(c_int) :: rc
   integer(c_int32_t) :: iuns, iomp
   logical :: omp_path, log_open
   character(len=15) :: label
   integer :: nlab
+  integer(8) :: clk, clk_rate
 
+  call system_clock(clk, clk_rate)
+  time0 = dble(clk) / dble(max(clk_rate, 1_8))
   iuns = 0
   if (unsym) iuns = 1
   iomp = 0
@@ -104,6 +117,8 @@ subroutine solver(ad, au, al, x, b, iax, ja, neq, nad, unsym, tol, maxit, code, 
                   int(nlit, c_int32_t), iomp, st)
   fvc_last_status = rc
   fvc_last_stats = st
+  time = st%seconds
+  matvectime = matvectime + st%seconds
 
   ! which reference routine this call replaces, for the log label
   omp_path = openmpSolver .and. (matriz == 4 .or. (matriz == 1 .and. (code == 1 .or. unsym)))

@@ -43,11 +43,20 @@ _ref = None
 
 
 def build(ref: bool | None = None) -> None:
-    """Compile the C restatement (and the reference, when its sources exist)."""
+    """Compile the C restatement; when the reference's sources exist, also the
+    reference's solver path (oracle/_ref/libfvcref.so) and, once the drop-in
+    is built, the reference program's objects linked against it
+    (oracle/_ref/libmvf_dropin.so)."""
     targets = ["oracle"]
     if ref or (ref is None and os.path.isdir("/root/reference/src")):
         targets.append("ref")
+        pkg = os.path.join(os.path.dirname(HERE), "fortranfvcell_amd")
+        if all(os.path.exists(os.path.join(pkg, f)) for f in ("libfvcell.so", "libfvcell_fortran.a")):
+            targets.append("dropin")
     subprocess.run(["make", "-s", "-C", HERE] + targets, check=True)
+
+
+DROPIN_PATH = os.path.join(HERE, "_ref", "libmvf_dropin.so")
 
 
 def lib():

@@ -1,5 +1,5 @@
 ! drive_solver.f90 -- a caller of the reference's solver(...) interface, as
-! src/Simple.f calls it, linked against libfvcell_fortran.so (the drop-in for
+! src/Simple.f calls it, linked against libfvcell_fortran.a (the drop-in for
 ! Solver.o).  Reads one system from a raw stream file, calls
 !   solver(ad,au,al,x,b,iax,ja,neq,nad,unsym,tol,maxit,code,matriz,nlit)
 ! and getRes, and writes x plus the status of the last solve.
@@ -9,7 +9,8 @@
 !         n_ad, n_al, n_au; float64 tol; int32 iax(n_iax), ja(n_ja);
 !         float64 ad(n_ad), al(n_al), au(n_au), b(neq), x0(neq)
 ! output: float64 x(neq); int32 status, iterations; float64 energy;
-!         float64 u(neq) = x permuted back by getRes(num = reversed ids)
+!         float64 u(neq) = x permuted back by getRes(num = reversed ids);
+!         float64 time, matvectime, dottime, precontime of COMMON /timectrl/
 program drive_solver
   implicit none
   integer :: num_threads, nThreadsCell, nThreadsSolver, nThreadsDot, nThreadsMatVec
@@ -23,6 +24,10 @@ program drive_solver
   real*8, allocatable :: ad(:), al(:), au(:), b(:), x(:), u(:)
   character(len=512) :: fin, fout, flog
   logical :: unsym
+  real*8 :: solvTime, elmTime, vecTime, dsTime, numeqTime, reordtime, totaltime, matvectime, &
+            dottime, time, time0, readtime, viztime, grtime, precontime
+  common /timectrl/ solvTime, elmTime, vecTime, dsTime, numeqTime, reordtime, totaltime, &
+                    matvectime, dottime, time, time0, readtime, viztime, grtime, precontime
 
   call get_command_argument(1, fin)
   call get_command_argument(2, fout)
@@ -46,6 +51,9 @@ program drive_solver
   openmpSolver = (omp /= 0)
   nThreadsSolver = 1
   unsym = (unsym_i /= 0)
+  matvectime = 0.0d0
+  dottime = 0.0d0
+  precontime = 0.0d0
   open(13, file=trim(flog), status='replace', action='write')
   call solver(ad, au, al, x, b, iax, ja, neq, nad, unsym, tol, maxit, code, matriz, nlit)
   close(13)
@@ -59,5 +67,6 @@ program drive_solver
   write(11) status, iterations
   write(11) energy
   write(11) u
+  write(11) time, matvectime, dottime, precontime
   close(11)
 end program drive_solver

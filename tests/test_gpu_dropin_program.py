@@ -1,0 +1,58 @@
+"""The drop-in inside the reference program's own object set
+(oracle/_ref/libmvf_dropin.so: all reference objects but Solver.o and
+Main.o, linked against libfvcell_fortran.a): solver_ called with the
+reference's calling convention returns the oracle's solution, and the
+reference's own COMMON blocks are the ones the drop-in reads
+(/omp_control/, include/openmp.fi:16-18) and writes (/timectrl/,
+include/time.fi:10-12)."""
+import ctypes as C
+import os
+
+import numpy as np
+import pytest
+
+import oracle
+from fortranfvcell_amd import problems as P
+
+pytestmark = [pytest.mark.gpu,
+              pytest.mark.skipif(not os.path.exists(oracle.DROPIN_PATH),
+                                 reason="oracle/_ref/libmvf_dropin.so is built only where the reference sources are")]
+
+
+@pytest.fixture(scope="module")
+def prog(fvc):
+    L = C.CDLL(oracle.DROPIN_PATH, mode=C.RTLD_GLOBAL)
+    L.solver_.restype = None
+    return L
+
+
+def _call(L, f, code, unsym, tol=1e-10, maxit=20000):
+    neq = f["neq"]
+    ad = np.ascontiguousarray(f["ad"], np.float64)
+    al = np.ascontiguousarray(f["al"], np.float64)
+    au = al if f.get("au") is None else np.ascontiguousarray(f["au"], np.float64)
+    iax = np.ascontiguousarray(f["iax"], np.int32)
+    ja = np.ascontiguousarray(f["ja"], np.int32)
+    x = np.zeros(neq)
+    b = np.ascontiguousarray(f["b"], np.float64).copy()
+    i32 = lambda v: C.byref(C.c_int32(v))
+    p = lambda a: a.ctypes.data_as(C.c_void_p)
+    L.solver_(p(ad), p(au), p(al), p(x), p(b), p(iax), p(ja), i32(neq), i32(f["nad"]), i32(1 if unsym else 0),
+              C.byref(C.c_double(tol)), i32(maxit), i32(code), i32(f["matriz"]), i32(2))
+    return x
+
+
+@pytest.mark.parametrize("code,lay,omp", [(1, "ell", 1), (2, "ell", 1), (2, "csrd", 0)])
+def test_solver_through_reference_objects(prog, orc, code, lay, omp):
+    s = P.config_poisson(24) if code == 1 else P.config_advdiff(24, peclet_max=60.0)
+    f = P.to_ell(s) if lay == "ell" else P.to_csrd(s)
+    print(f"reference objects + drop-in: code={code} {lay} openmpSolver={omp}", flush=True)
+    ompc = (C.c_int32 * 7).in_dll(prog, "omp_control_")        # num_threads, openmpCell, openmpSolver, ...
+    ompc[2] = omp
+    ompc[4] = 1
+    tc = (C.c_double * 15).in_dll(prog, "timectrl_")           # ..., matvectime (7), dottime (8), time (9)
+    tc[7] = 0.0
+    x = _call(prog, f, code, True)
+    xo, _, so = orc.solver(f, code, omp=bool(omp))
+    assert np.linalg.norm(x - xo) <= 1e-8 * np.linalg.norm(xo)
+    assert tc[9] > 0.0 and tc[7] == tc[9] and tc[8] == 0.0

@@ -36,7 +36,9 @@ def _run(tmp_path, f, code, unsym, omp, tol=1e-10, maxit=20000, x0=None):
     x = np.frombuffer(raw[: 8 * neq], np.float64)
     status, iters = np.frombuffer(raw[8 * neq: 8 * neq + 8], np.int32)
     energy = np.frombuffer(raw[8 * neq + 8: 8 * neq + 16], np.float64)[0]
-    u = np.frombuffer(raw[8 * neq + 16:], np.float64)
+    u = np.frombuffer(raw[8 * neq + 16: 16 * neq + 16], np.float64)
+    timers = np.frombuffer(raw[16 * neq + 16:], np.float64)
+    _run.timers = dict(zip(("time", "matvectime", "dottime", "precontime"), timers))
     return x, int(status), int(iters), energy, u, open(flog).read(), r.stdout
 
 
@@ -54,6 +56,10 @@ def test_fortran_dropin_matches_oracle(fvc, orc, tmp_path, code, lay, omp):
     assert abs(energy - so["energy"]) <= 1e-8 * abs(so["energy"])
     # getRes with num(i) = neq+1-i: u(i) = x(num(i))
     assert np.array_equal(u, x[::-1])
+    # COMMON /timectrl/: time = this solve's time, added to matvectime
+    tm = _run.timers
+    assert tm["time"] > 0 and tm["matvectime"] == tm["time"]
+    assert tm["dottime"] == 0 and tm["precontime"] == 0
     label = {(1, True): "PCG OMP      : ", (1, False): "PCG:       ", (2, True): "PBICGSTAB OMP: ",
              (2, False): "PBICGSTAB: ", (3, False): "GMRES: "}[(code, omp)]
     # write(13,'(a,a,i9,a,d20.10,a,d20.10)') label,"it",j," energy norm ",energy," tol ",tol
