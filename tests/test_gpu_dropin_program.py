@@ -21,7 +21,9 @@ pytestmark = [pytest.mark.gpu,
 
 @pytest.fixture(scope="module")
 def prog(fvc):
-    L = C.CDLL(oracle.DROPIN_PATH, mode=C.RTLD_GLOBAL)
+    # RTLD_LOCAL: the drop-in's solver_ must not enter the process-wide
+    # scope, where the reference's own library (the oracle) would bind to it
+    L = C.CDLL(oracle.DROPIN_PATH, mode=C.RTLD_LOCAL)
     L.solver_.restype = None
     return L
 
