@@ -108,13 +108,14 @@ def time_spmv(F, ds, neq, nnz, reps=50, rotate=1):
     return t, spmv_bytes(neq, nnz) / t / 1e9
 
 
-def cpu_baseline(fmt, method, tol, budget_s=10.0):
+def cpu_baseline(fmt, method, tol, budget_s=10.0, threads=None):
     """The reference Fortran (oracle/_ref) with OpenMP on the host cores: the
     solver's own OpenMP routines (pbicgstab_omp + matvec_ellpack_omp +
     dot_ompL4), repeated for about budget_s seconds.  Falls back to the
     sequential C restatement."""
     import oracle
-    threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or min(os.cpu_count() or 1, 16)
+    if threads is None:
+        threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or min(os.cpu_count() or 1, 16)
     if oracle.have_ref():
         oracle.ref()
         oracle.ref_set_omp(1, threads)
@@ -165,6 +166,44 @@ def calibrate_copy(mib: int, reps: int) -> None:
                       "GBs_read_plus_write": 2 * nbytes / t / 1e9}), flush=True)
 
 
+def bench_c3b(n: int, steps: int) -> None:
+    """FP64 and mixed-precision Jacobi-PCG (FP32 matrix and inner iterations,
+    FP64 residual recomputation) to true relative residual 1e-10 on the
+    lognormal(0.5) pressure system of an n x n quad mesh split into 2 n^2
+    triangles (ELLPACK w = 3); the error of the mixed solution is stated
+    against the FP64 GPU solution (itself checked against the oracle at the
+    sizes the oracle finishes)."""
+    import fortranfvcell_amd as F
+    from fortranfvcell_amd import problems as P
+    F.init(0)
+    t0 = time.perf_counter()
+    s = P.config_pressure_lognormal(n, triangles=True)
+    ell = P.to_ell(s)
+    log(f"generated {s.neq} triangle cells in {time.perf_counter() - t0:.1f}s")
+    ds = F.DeviceSystem(ell)
+    del ell
+    opts = dict(stop_rule=F.STOP_RELRES, x0_rule=F.X0_ZERO)
+    out = {}
+    for name, prec in (("fp64", F.PREC_FP64), ("mixed", F.PREC_MIXED)):
+        ts = []
+        for _ in range(steps):
+            st = ds.solve(F.PCG, tol=1e-10, maxit=200000, precision=prec, **opts)
+            ts.append(st["seconds"])
+            log(name, st["iterations"], st["inner_iterations"], f"{st['seconds']:.2f}s", st["status"])
+        x = ds.x.get()
+        out[name] = {"iterations": st["iterations"], "refinements": st["inner_iterations"],
+                     "s_per_solve": float(np.mean(ts)), "true_relres": st["true_relres"],
+                     "status": st["status"], "_x": x}
+    x64 = out["fp64"].pop("_x")
+    xm = out["mixed"].pop("_x")
+    out["mixed"]["rel_err_vs_fp64"] = float(np.linalg.norm(xm - x64) / np.linalg.norm(x64))
+    nnz = int(s.nbr.__ge__(0).sum())
+    print(json.dumps({"workload": f"lognormal pressure, {n}x{n} quads split in 2 triangles, ELLPACK w=3",
+                      "neq": s.neq, "nnz_offdiag": nnz, "steps": steps, **out,
+                      "speedup_mixed_over_fp64": out["fp64"]["s_per_solve"] / out["mixed"]["s_per_solve"]}),
+          flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -175,9 +214,14 @@ def main():
     ap.add_argument("--no-extra", action="store_true")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run the plain copy kernel on MIB MiB (hardware byte-counter reference)")
+    ap.add_argument("--c3b", type=int, default=0, metavar="N",
+                    help="only run FP64 against mixed-precision PCG on the lognormal pressure system "
+                         "of an N x N quad mesh split into 2 N^2 triangles (N = 4096: 33.5M cells)")
     args = ap.parse_args()
     if args.calibrate_copy:
         return calibrate_copy(args.calibrate_copy, max(args.steps, 1))
+    if args.c3b:
+        return bench_c3b(args.c3b, max(args.steps, 1))
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU only (asked for --gpus {args.gpus}); "
               "the solver path is a single-GPU design, see the sharding notes", file=sys.stderr)
@@ -324,11 +368,29 @@ def main():
         bres["one_at_a_time_fp64"] = {"ms": single * 1e3, "solves_per_s": len(fmts) / single}
         extra["batch256"] = bres
         log("batch", bres)
+        # the drop-in as the Fortran program calls it: host arrays in, x out
+        # (H2D of the matrix and vectors and D2H of x inside every solve)
+        F.solve(ell, F.BICGSTAB, tol=tol, **opts)
+        t0 = time.perf_counter()
+        nd = 5
+        for _ in range(nd):
+            _, sth = F.solve(ell, F.BICGSTAB, tol=tol, **opts)
+        dt = (time.perf_counter() - t0) / nd
+        extra["dropin_host_arrays"] = {"solves_per_s": 1.0 / dt, "ms_per_solve": dt * 1e3,
+                                       "device_ms_per_solve": sth["seconds"] * 1e3,
+                                       "note": "fvc_solve with host arrays: upload of A, b, x0 and download "
+                                               "of x included (PCIe); value above is device-resident"}
+        log("dropin", extra["dropin_host_arrays"])
         result["extra"] = extra
 
     if not args.no_cpu:
         result["cpu_baseline"] = cpu_baseline(ell, "bicgstab", tol)
+        result["cpu_baseline"]["host_cpus"] = os.cpu_count()
         log("cpu", result["cpu_baseline"])
+        if not args.no_extra:
+            one = cpu_baseline(ell, "bicgstab", tol, budget_s=5.0, threads=1)
+            result.setdefault("extra", {})["cpu_reference_1thread"] = one
+            log("cpu 1 thread", one)
     print(json.dumps(result), flush=True)
 
 

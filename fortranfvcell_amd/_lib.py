@@ -38,7 +38,8 @@ class Options(C.Structure):
     _fields_ = [("x0_rule", C.c_int32), ("gmres_k", C.c_int32), ("stop_rule", C.c_int32),
                 ("true_residual", C.c_int32), ("engine", C.c_int32), ("block_size", C.c_int32),
                 ("grid_blocks", C.c_int32), ("precision", C.c_int32), ("batch_work_exp10", C.c_int32),
-                ("barrier", C.c_int32), ("recurrence", C.c_int32), ("reserved", C.c_int32 * 5)]
+                ("barrier", C.c_int32), ("recurrence", C.c_int32), ("refresh_every", C.c_int32),
+                ("refresh_drop_e3", C.c_int32), ("reserved", C.c_int32 * 3)]
 
 
 class Stats(C.Structure):

@@ -42,7 +42,9 @@ struct BatchArgs {
   double *s;                        // 1/sqrt(a_ii) (mixed)
   float *dsq;                       // sqrt(a_ii) (mixed)
   double tol;
+  double delta;                     // mixed: residual drop that triggers an FP64 update
   int maxit, x0_zero, stop_rule, mixed;
+  int max_inner;                    // mixed: FP32 iterations between updates, at most
 };
 
 __device__ __forceinline__ void batch_grid(const BatchArgs &B, int sys, const SysDesc &D, Grid &g) {
@@ -138,10 +140,10 @@ __global__ __launch_bounds__(1024) void mixed_batch_persistent(BatchArgs B) {
   a.p1 = B.f[3] + off;
   a.q = B.f[4] + off;
   a.tol = B.tol;
-  a.delta = 0.1;
+  a.delta = B.delta;
   a.maxit = B.maxit;
   a.x0_zero = B.x0_zero;
-  a.max_inner = 1000;
+  a.max_inner = B.max_inner;
   mixed_run<F>(a, lds);
 }
 

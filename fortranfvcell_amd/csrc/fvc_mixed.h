@@ -8,10 +8,11 @@
 // reference's 1e60 pinning penalty (src/CellibPressure.f:414,557) overflows
 // FP32, the scaled row does not.  The FP32 recursion drifts from the true
 // residual, so whenever its unscaled estimate ||S^-1 r~|| falls by a factor
-// `delta` (or reaches the tolerance) the solution increment is folded into
-// the FP64 x and the residual is recomputed in FP64 from the FP64 matrix
-// ("reliable updates", van der Vorst & Ye 2000; Clark et al. 2010).  The
-// solve stops when that FP64 residual satisfies ||b - A x|| <= tol ||b||.
+// `delta`, reaches the tolerance, or `max_inner` FP32 iterations have passed,
+// the solution increment is folded into the FP64 x and the residual is
+// recomputed in FP64 from the FP64 matrix ("reliable updates", van der Vorst
+// & Ye 2000; Clark et al. 2010); the search direction is kept.  The solve
+// stops when that FP64 residual satisfies ||b - A x|| <= tol ||b||.
 #pragma once
 #include "fvc_krylov.h"
 

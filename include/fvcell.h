@@ -62,7 +62,9 @@ extern "C" {
 
 /* Stopping rules. */
 #define FVC_STOP_REFERENCE 0    /* each method's own test (src/Iterativos.f:73,190,376)     */
-#define FVC_STOP_RELRES 1       /* recursive ||r||_2 <= tol*||b||_2, then a true-residual check */
+#define FVC_STOP_RELRES 1       /* recursive ||r||_2 <= tol*||b||_2; PCG and BiCGSTAB then check
+                                   the true FP64 residual and restart from x while it misses
+                                   the tolerance (iterations include the restarts)          */
 
 /* Engines. */
 #define FVC_ENGINE_AUTO 0        /* KERNELS for PCG and BiCGSTAB, PERSISTENT for GMRES   */
@@ -97,7 +99,11 @@ typedef struct fvc_options {
   int32_t barrier;        /* persistent engine's grid barrier: 0 per-block flags,
                              1 arrival counters                                     */
   int32_t recurrence;     /* FVC_REC_*; PCG and BiCGSTAB                              */
-  int32_t reserved[5];
+  int32_t refresh_every;  /* mixed: at most this many FP32 iterations between FP64
+                             residual recomputations; 0 = 200                         */
+  int32_t refresh_drop_e3; /* mixed: recompute when the residual estimate fell by
+                             this factor (thousandths) since the last one; 0 = 300   */
+  int32_t reserved[3];
 } fvc_options;
 
 typedef struct fvc_stats {
