@@ -13,6 +13,7 @@ from __future__ import annotations
 
 import ctypes as C
 import os
+import sys
 import subprocess
 
 import numpy as np
@@ -105,7 +106,17 @@ def ref(pool_words: int = 1 << 27, omp: int = 0, nthreads: int = 1):
         L.ref_pbicgstab.argtypes = [I, I, I, I, P, P, P, P, P, P, P, P, D, I, I, I]
         L.ref_wtime.restype = D
         L.ref_wtime.argtypes = []
-        L.ref_init(pool_words, omp, nthreads)
+        # the reference's init_malloc prints a banner on unit 6; keep it off
+        # the caller's stdout (a benchmark's stdout is one JSON line)
+        sys.stdout.flush()
+        saved, quiet = os.dup(1), os.open(os.devnull, os.O_WRONLY)
+        try:
+            os.dup2(quiet, 1)
+            L.ref_init(pool_words, omp, nthreads)
+        finally:
+            os.dup2(saved, 1)
+            os.close(saved)
+            os.close(quiet)
         dev_null = b"/dev/null"
         L.ref_open_log(dev_null, len(dev_null))   # unit 13, the solver log
         _ref = L

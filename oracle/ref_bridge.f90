@@ -37,6 +37,7 @@ subroutine ref_init(pool_words, omp, nthreads) bind(C, name='ref_init')
   if (.not. pool_ready) then
     maxmem = pool_words
     call init_malloc()
+    flush(6)           ! its banner leaves now, while the caller may have silenced stdout
     pool_ready = .true.
   end if
   openmpSolver = (omp /= 0)
