@@ -51,11 +51,11 @@ def save_cache(cache: dict) -> None:
 
 
 def candidates(cus: int, neq: int):
-    """Launch shapes worth timing: blocks from half the CUs to three per CU,
+    """Launch shapes worth timing: blocks from half the CUs to four per CU,
     never fewer than 64 rows per block."""
     out = []
     for bs in BLOCK_SIZES:
-        for per_cu in (0.5, 1, 2, 3):
+        for per_cu in (0.5, 1, 2, 4):
             G = int(cus * per_cu)
             if G < 8 or neq // G < 64:
                 continue

@@ -196,6 +196,19 @@ Grid make_grid(int neq, int G, unsigned *ctr, double *partials, bool persistent,
   return g;
 }
 
+// The kernel-per-phase engine is not bound to co-resident blocks.  Its
+// deferred reductions have every block sum all G partials, a cost that grows
+// as G^2; on MI355X at 1M cells four blocks per CU were best (BiCGSTAB 3.13 ms
+// at 512 threads against 3.24 ms at two blocks per CU; PCG 34.9 us per
+// iteration at 256 threads against 38.1).
+int kernels_blocks(int neq, int cus, int requested, int per_cu = 4) {
+  int G = requested > 0 ? requested : per_cu * cus;
+  G = std::min(G, std::max(1, (neq + 63) / 64));
+  if (G >= 64) G -= G % 8;
+  return std::max(G, 1);
+}
+int kernels_block(int code, bool fused) { return (code == FVC_PCG && fused) ? 256 : 512; }
+
 int choose_blocks(int neq, int cus, int occ_blocks, int requested) {
   int G = requested > 0 ? requested : cus * std::max(1, std::min(occ_blocks, 2));
   G = std::min(G, std::max(1, (neq + 63) / 64));
@@ -283,10 +296,15 @@ int drive_batches(DevState &d, hipStream_t s, int maxit, Iter &&iteration, State
 template <int F>
 int run_solve(DevState &d, const SolveIn &in, fvc_stats *stats, hipStream_t s) {
   const int neq = in.A.neq;
-  const int bs = in.bs > 0 ? in.bs : kDefaultBlock;
+  const bool fused = in.recurrence == FVC_REC_FUSED;
+  // AUTO: PCG and BiCGSTAB run one kernel per phase (with kernel boundaries
+  // for ordering, a phase needs no fence and no atomic; measured faster on
+  // MI355X than the grid barrier of one persistent launch); GMRES, whose
+  // Arnoldi step is a chain of short phases, runs persistent.
+  bool persistent = in.engine == FVC_ENGINE_PERSISTENT || (in.engine == FVC_ENGINE_AUTO && in.code == FVC_GMRES);
+  const int bs = in.bs > 0 ? in.bs : persistent ? kDefaultBlock : kernels_block(in.code, fused);
   if (bs % 64 != 0 || bs > 1024) return fail(FVC_BAD_ARGUMENT, "block size %d must be a multiple of 64 <= 1024", bs);
   const void *pfn = nullptr, *kfn = nullptr;
-  const bool fused = in.recurrence == FVC_REC_FUSED;
   if (in.code == FVC_PCG && fused) {
     pfn = (const void *)&cg1_persistent<F>;
     kfn = (const void *)&cg1_kernel<F>;
@@ -307,12 +325,7 @@ int run_solve(DevState &d, const SolveIn &in, fvc_stats *stats, hipStream_t s) {
   int rc = occupancy(pfn, bs, &occ);
   if (rc) return rc;
   if (occ < 1) return fail(FVC_DEVICE_ERROR, "persistent kernel does not fit a CU at block size %d", bs);
-  const int G = choose_blocks(neq, d.cus, occ, in.G_req);
-  // AUTO: PCG and BiCGSTAB run one kernel per phase (with kernel boundaries
-  // for ordering, a phase needs no fence and no atomic; measured faster on
-  // MI355X than the grid barrier of one persistent launch); GMRES, whose
-  // Arnoldi step is a chain of short phases, runs persistent.
-  bool persistent = in.engine == FVC_ENGINE_PERSISTENT || (in.engine == FVC_ENGINE_AUTO && in.code == FVC_GMRES);
+  const int G = persistent ? choose_blocks(neq, d.cus, occ, in.G_req) : kernels_blocks(neq, d.cus, in.G_req);
   if (persistent && G > occ * d.cus) {
     if (in.engine == FVC_ENGINE_PERSISTENT)
       return fail(FVC_BAD_ARGUMENT, "%d blocks exceed the %d co-resident blocks of a persistent launch", G, occ * d.cus);
@@ -562,15 +575,18 @@ int run_mixed(DevState &d, const SolveIn &in, fvc_stats *stats, hipStream_t s) {
     return fail(FVC_UNSUPPORTED, "mixed precision supports ELLPACK and unsymmetric CSRD storage");
   } else {
     const int neq = in.A.neq;
+    // AUTO runs one kernel per phase, as for FP64 PCG
+    const bool persistent = in.engine == FVC_ENGINE_PERSISTENT;
     const int bs = in.bs > 0 ? in.bs : kDefaultBlock;
+    if (bs % 64 != 0 || bs > 1024) return fail(FVC_BAD_ARGUMENT, "block size %d must be a multiple of 64 <= 1024", bs);
     const void *pfn = (const void *)&mixed_persistent<F>;
     const void *kfn = (const void *)&mixed_kernel<F>;
     int occ = 0;
     int rc = occupancy(pfn, bs, &occ);
     if (rc) return rc;
-    const int G = choose_blocks(neq, d.cus, occ, in.G_req);
-    // AUTO runs one kernel per phase, as for FP64 PCG
-    const bool persistent = in.engine == FVC_ENGINE_PERSISTENT;
+    // mixed phases are lighter (FP32): two blocks of 512 per CU measured best
+    // (1M cells: 232 ms against 249 ms at four per CU)
+    const int G = persistent ? choose_blocks(neq, d.cus, occ, in.G_req) : kernels_blocks(neq, d.cus, in.G_req, 2);
     if (persistent && G > occ * d.cus)
       return fail(FVC_BAD_ARGUMENT, "%d blocks exceed the %d co-resident blocks of a persistent launch", G, occ * d.cus);
     const size_t n = (size_t)neq;
