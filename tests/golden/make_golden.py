@@ -9,7 +9,8 @@ energy norm it writes to its unit-13 log (src/Iterativos.f:91-93,216-218,
 statistics (iterations, energy, sum and sum of squares of x, 64 sampled
 entries).  Run in the development container (needs the reference build):
 
-    python tests/golden/make_golden.py
+    python tests/golden/make_golden.py           # ref_solver_small.npz, ref_solver_1024_stats.npz
+    python tests/golden/make_golden.py configs   # ref_configs.npz
 """
 import os
 import re
@@ -121,5 +122,99 @@ def main():
     np.savez_compressed(os.path.join(HERE, "ref_solver_1024_stats.npz"), **stats)
 
 
+def _relres(s, x):
+    return float(np.linalg.norm(s.b - P.scipy_csr(s) @ x) / np.linalg.norm(s.b))
+
+
+def _ladder(s, run):
+    """The largest tol of a decade ladder at which the reference's own stopping
+    rule leaves a true relative residual <= 1e-10 (its BiCGSTAB and GMRES
+    rules are not the residual)."""
+    for tol in (1e-10, 3e-11, 1e-11, 3e-12, 1e-12, 3e-13, 1e-13):
+        out = run(tol)
+        if _relres(s, out[0][0]) <= 1e-10:
+            return tol, out
+    raise RuntimeError("the reference never reached relative residual 1e-10")
+
+
+def configs():
+    """Fixtures of the hot-path configurations (reference outputs on the
+    project's seeded systems): C1 64x64 pinned Poisson, CSRD PCG; C2 64x64
+    and 128x128 advection-diffusion, ELLPACK BiCGSTAB and GMRES(30); C3
+    64x64 lognormal pressure on quads and on triangles, ELLPACK PCG; C4 four
+    small uneven batch systems, ELLPACK PCG; C5 the 32x32 heat problem's
+    forward solution, adjoint (the reference's BiCGSTAB on the explicit A^T,
+    CSRD) and gradient, with LU central differences on 8 cells; and 1024x1024
+    statistics of GMRES(30) on ELLPACK and BiCGSTAB on CSRD."""
+    sys.path.insert(0, os.path.dirname(HERE))
+    from heat_fd import fd_gradient_lu
+    arrays, meta = {}, []
+
+    def solver_case(key, s, f, code):
+        tol, ((x, _), label, it, en) = _ladder(
+            s, lambda t: _ref_call(lambda: oracle.ref_solver(f, code, unsym=True, omp=False, tol=t)))
+        arrays[key + "_x"] = x
+        meta.append((key, it, en, tol, label))
+        print(key, it, en, tol, label, flush=True)
+
+    def gmres_case(key, s, f):
+        tol, ((x, _, _), label, it, en) = _ladder(
+            s, lambda t: _ref_call(lambda: oracle.krylov("gmres", f, k=30, tol=t, use_ref=True)))
+        arrays[key + "_x"] = x
+        meta.append((key, it, en, tol, label))
+        print(key, it, en, tol, label, flush=True)
+
+    s = P.config_poisson(64)
+    solver_case("c1_poisson64_csrd_pcg", s, P.to_csrd(s), 1)
+    for n in (64, 128):
+        s = P.config_advdiff(n)
+        solver_case(f"c2_advdiff{n}_ell_bicgstab", s, P.to_ell(s), 2)
+        gmres_case(f"c2_advdiff{n}_ell_gmres30", s, P.to_ell(s))
+    for tri in (False, True):
+        s = P.config_pressure_lognormal(64, triangles=tri)
+        solver_case(f"c3_lognormal64_{'tri' if tri else 'quad'}_ell_pcg", s, P.to_ell(s), 1)
+    for i, s in enumerate(P.config_batch(count=4, lo=8, hi=48)):
+        solver_case(f"c4_batch{i}_ell_pcg", s, P.to_ell(s), 1)
+    keys = np.array([m[0] for m in meta])
+
+    # C5: forward T and adjoint lambda with the reference's BiCGSTAB, tol 1e-15
+    op, k, subset = P.heat_problem(32, subset_frac=0.05)
+    sh = op.assemble(k)
+    T, _, _ = oracle.krylov("bicgstab", P.to_ell(sh), tol=1e-15, use_ref=True)
+    g = np.zeros(sh.neq)
+    g[subset] = 2.0 * T[subset] / len(subset)
+    lam, _, _ = oracle.krylov("bicgstab", P.to_csrd(P.transpose_system(sh)), b=g, tol=1e-15, use_ref=True)
+    gk = op.grad_k(k, T, lam)
+    cells = np.sort(np.random.default_rng(32).choice(sh.neq, 8, replace=False))
+    fd = fd_gradient_lu(op, k, T, subset, cells)
+    print("c5 fd vs adjoint", np.max(np.abs(fd - gk[cells]) / np.abs(fd)), flush=True)
+
+    # 1024x1024 statistics
+    rng = np.random.default_rng(11)
+    s = P.config_advdiff(1024)
+    stats = {}
+    for name, run in (("advdiff1024_ell_gmres30",
+                       lambda: oracle.krylov("gmres", P.to_ell(s), k=30, tol=1e-10, use_ref=True)[:2]),
+                      ("advdiff1024_csrd_bicgstab",
+                       lambda: oracle.ref_solver(P.to_csrd(s), 2, unsym=True, omp=False, tol=1e-10))):
+        (x, _), label, it, en = _ref_call(run)
+        idx = np.sort(rng.choice(s.neq, 64, replace=False))
+        stats.update({f"{name}_iterations": np.int64(it), f"{name}_energy": np.float64(en),
+                      f"{name}_xsum": np.float64(x.sum()), f"{name}_xsq": np.float64((x * x).sum()),
+                      f"{name}_idx": idx, f"{name}_xsample": x[idx]})
+        print(name, it, en, label, flush=True)
+
+    np.savez_compressed(os.path.join(HERE, "ref_configs.npz"), keys=keys,
+                        iterations=np.array([m[1] for m in meta], np.int64),
+                        energy=np.array([m[2] for m in meta]), tol=np.array([m[3] for m in meta]),
+                        labels=np.array([m[4] for m in meta]),
+                        c5_subset=subset, c5_T=T, c5_lam=lam, c5_gk=gk, c5_cells=cells, c5_fd=fd,
+                        **arrays, **stats)
+
+
 if __name__ == "__main__":
-    main()
+    if len(sys.argv) > 1 and sys.argv[1] == "configs":
+        oracle.build(ref=True)
+        configs()
+    else:
+        main()

@@ -10,6 +10,7 @@ import scipy.sparse.linalg as sla
 
 import oracle
 from fortranfvcell_amd import problems as P
+from heat_fd import loss_difference, perturbation_rhs
 
 pytestmark = pytest.mark.gpu
 
@@ -21,39 +22,9 @@ def _loss_grad(solver, k, subset):
     return loss_and_grad(solver, k, subset)
 
 
-def _perturbation_rhs(op, k, T, c, dk):
-    """r = (b(k + dk e_c) - b(k)) - (A(k + dk e_c) - A(k)) T, one face at a time.
-
-    A face's diffusion coefficient is D = geo / u with u = a/k_P + (1-a)/k_N;
-    raising k_c by dk changes 1/u by a dk / (k_c (k_c + dk) u u') on c's own
-    faces (c is P, share a) and by (1-a) dk / (...) on the neighbours' faces
-    that point back at c (c is N there).  Forming the change this way instead
-    of subtracting assembled coefficients avoids the eps/dk relative rounding
-    of that subtraction, which otherwise leaves a non-conservative residual
-    that swamps the finite difference on cells whose gradient is a near
-    cancellation (lambda_c ~ lambda_N far from the loss cells)."""
-    nbr = op.mesh.nbr
-    back = P.ell_back_slots(nbr)
-    r = np.zeros(op.mesh.neq)
-    kc, kp = k[c], k[c] + dk
-    for s in range(nbr.shape[1]):
-        n = nbr[c, s]
-        if n >= 0:
-            a = op.alpha[c, s]
-            u, up = a / kc + (1 - a) / k[n], a / kp + (1 - a) / k[n]
-            r[c] -= a * dk / (kc * kp * u * up) * op.geo[c, s] * (T[c] - T[n])
-            t = back[c, s]
-            a2 = op.alpha[n, t]
-            u2, u2p = a2 / k[n] + (1 - a2) / kc, a2 / k[n] + (1 - a2) / kp
-            r[n] -= (1 - a2) * dk / (kc * kp * u2 * u2p) * op.geo[n, t] * (T[n] - T[c])
-        else:
-            r[c] += dk * op.wall_g[c, s] * (op.wall_T[c, s] - T[c])
-    return r
-
-
 def _fd_correction(op, solver, k, T, subset, c):
     """Central difference of L = mean(T[S]^2) in k_c, each side evaluated as
-    T(k +- h e_c) = T + d+-,  d+- = A(k +- h e_c)^-1 r+-  (r from _perturbation_rhs):
+    T(k +- h e_c) = T + d+-,  d+- = A(k +- h e_c)^-1 r+-  (r from perturbation_rhs):
     the exact difference of the two solutions (for the solution T of A(k) T = b),
     solved for directly so that it does not cancel against T.  The loss
     difference is formed as mean((d+ - d-) (2 T + d+ + d-)) over S, equal to
@@ -65,24 +36,8 @@ def _fd_correction(op, solver, k, T, subset, c):
         kk = k.copy()
         kk[c] += sgn * h
         solver.load(kk)
-        d.append(solver.solve_rhs(_perturbation_rhs(op, k, T, c, sgn * h))[subset])
-    dp, dm = d
-    return np.mean((dp - dm) * (2.0 * T[subset] + dp + dm)) / (2 * h)
-
-
-def test_perturbation_rhs_matches_assembled_difference():
-    op, k, subset = P.heat_problem(24, subset_frac=0.05)
-    s0 = op.assemble(k)
-    T = np.linspace(0.0, 1.0, s0.neq) ** 2
-    A0 = P.scipy_csr(s0)
-    for c in (0, 23, 300, 575):
-        for dk in (0.3 * k[c], -0.2 * k[c]):
-            kk = k.copy()
-            kk[c] += dk
-            s = op.assemble(kk)
-            want = (s.b - s0.b) - (P.scipy_csr(s) @ T - A0 @ T)
-            got = _perturbation_rhs(op, k, T, c, dk)
-            assert np.allclose(got, want, rtol=1e-12, atol=1e-13 * np.abs(want).max())
+        d.append(solver.solve_rhs(perturbation_rhs(op, k, T, c, sgn * h))[subset])
+    return loss_difference(T, subset, d[0], d[1], h)
 
 
 def test_adjoint_vs_direct_fd_small(fvc):
