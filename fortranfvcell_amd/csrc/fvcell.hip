@@ -695,8 +695,13 @@ int dispatch_solve(DevState &d, const SolveIn &in, fvc_stats *stats, hipStream_t
     int rc = dispatch_fp64(d, in, st, s);
     SolveIn again = in;
     again.x0_zero = 0;
+    // A restart that does not halve the true residual has reached the FP64
+    // floor of this system (a tolerance near machine precision): stop there;
+    // stats.true_relres reports what was reached.
+    double prev = 2.0 * st->true_relres;
     for (int restart = 0; restart < 8 && rc == FVC_OK && st->true_relres > in.tol &&
-                          st->iterations < in.maxit; ++restart) {
+                          st->true_relres <= 0.5 * prev && st->iterations < in.maxit; ++restart) {
+      prev = st->true_relres;
       again.maxit = in.maxit - st->iterations;
       rc = dispatch_fp64(d, again, &part, s);
       const int its = st->iterations + part.iterations;

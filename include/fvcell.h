@@ -64,7 +64,9 @@ extern "C" {
 #define FVC_STOP_REFERENCE 0    /* each method's own test (src/Iterativos.f:73,190,376)     */
 #define FVC_STOP_RELRES 1       /* recursive ||r||_2 <= tol*||b||_2; PCG and BiCGSTAB then check
                                    the true FP64 residual and restart from x while it misses
-                                   the tolerance (iterations include the restarts)          */
+                                   the tolerance and each restart at least halves it
+                                   (iterations include the restarts; true_relres reports
+                                   what was reached)                                         */
 
 /* Engines. */
 #define FVC_ENGINE_AUTO 0        /* KERNELS for PCG and BiCGSTAB, PERSISTENT for GMRES   */
