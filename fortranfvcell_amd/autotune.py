@@ -15,9 +15,7 @@ import json
 import math
 import os
 
-import numpy as np
-
-from . import (BICGSTAB, ELLPACK, GMRES, PCG, DeviceSystem, device_info)
+from . import GMRES, DeviceSystem, device_info
 
 CACHE_PATH = os.environ.get("FVC_TUNE_CACHE", os.path.join(os.path.expanduser("~"), ".cache",
                                                             "fvcell_tune.json"))
@@ -106,13 +104,14 @@ def tune(fmts: dict, method: int, iters: int = 60, use_cache: bool = True, verbo
 
 def tune_batch(batch: dict, precision: int = 0, verbose: bool = False) -> dict:
     """Best (threads per block, block-allocation exponent) for a batch, by
-    timing whole batched solves (each system runs to its tolerance)."""
-    from . import solve_batch
+    timing whole batched solves (each system runs to its tolerance) of the
+    batch resident in HBM."""
+    from . import DeviceBatch
+    db = DeviceBatch(batch)
     best = None
     for bs in BLOCK_SIZES:
         for exp10 in (10, 15, 20):
-            _, sts = solve_batch(batch, tol=1e-10, precision=precision, stop_rule=1, block_size=bs,
-                                 batch_work_exp10=exp10)
+            sts = db.solve(tol=1e-10, precision=precision, stop_rule=1, block_size=bs, batch_work_exp10=exp10)
             t = sts[0]["seconds"]
             if verbose:
                 print(f"tune batch bs={bs} exp={exp10 / 10} {t * 1e3:.2f} ms", flush=True)
