@@ -1025,23 +1025,52 @@ int dot_impl(const double *a, const double *b, int n, double *result, bool on_de
 // ------------------------------------------------------------- batches --
 
 // Block groups per system: g_s proportional to neq_s^exp (PCG work ~ rows x
-// iterations, iterations ~ sqrt(rows) on these meshes), at least one, at most
-// one per 64 rows, the total within the co-resident capacity.
+// iterations, iterations ~ sqrt(rows) on these meshes), so that the systems
+// finish together; at least one block each, at most one per 64 rows, the
+// total within the co-resident capacity.  Shares are rounded by largest
+// remainder, so the large systems that set the batch's time are not starved
+// by the one-block floor of the small ones.
 std::vector<int> allocate_blocks(const std::vector<int> &neq, int cap, double expo) {
   const int n = (int)neq.size();
   std::vector<double> w(n);
   double tot = 0;
   for (int i = 0; i < n; ++i) tot += (w[i] = std::pow((double)std::max(neq[i], 1), expo));
-  std::vector<int> g(n, 1);
+  std::vector<int> g(n, 1), lim(n);
+  for (int i = 0; i < n; ++i) lim[i] = std::max(1, (neq[i] + 63) / 64);
   int used = n;
-  const int spare = cap - n;
-  if (spare > 0) {
+  if (cap > n) {
+    // systems whose share is below one block keep one; the rest share what is left
+    double rest = tot;
+    int avail = cap;
+    std::vector<char> floor1(n, 0);
+    for (bool changed = true; changed;) {
+      changed = false;
+      for (int i = 0; i < n; ++i)
+        if (!floor1[i] && (w[i] / rest) * avail < 1.0) {
+          floor1[i] = 1;
+          rest -= w[i];
+          --avail;
+          changed = true;
+        }
+      if (rest <= 0.0) break;
+    }
+    std::vector<std::pair<double, int>> frac;
+    used = 0;
     for (int i = 0; i < n; ++i) {
-      const int lim = std::max(1, (neq[i] + 63) / 64);
-      const int extra = (int)std::floor(spare * w[i] / tot);
-      const int add = std::min(extra, lim - 1);
-      g[i] += add;
-      used += add;
+      if (floor1[i]) {
+        g[i] = 1;
+      } else {
+        const double t = (w[i] / rest) * avail;
+        g[i] = std::min(lim[i], std::max(1, (int)std::floor(t)));
+        if (g[i] < lim[i]) frac.push_back({t - std::floor(t), i});
+      }
+      used += g[i];
+    }
+    std::sort(frac.begin(), frac.end(), [](const auto &a, const auto &b) { return a.first > b.first; });
+    for (const auto &f : frac) {
+      if (used >= cap) break;
+      ++g[f.second];
+      ++used;
     }
   }
   return g;
@@ -1058,8 +1087,10 @@ int run_batch(DevState &d, int nsys, const std::vector<int> &row_off, int w, con
   int occ = 0;
   int rc = occupancy(fn, bs, &occ);
   if (rc) return rc;
+  // every block of a batch must be co-resident: up to four per CU, as the
+  // occupancy API allows
   const int cap = std::max(1, o.grid_blocks > 0 ? std::min(o.grid_blocks, occ * d.cus)
-                                               : d.cus * std::max(1, std::min(occ, 2)));
+                                               : d.cus * std::max(1, std::min(occ, 4)));
   const int total = row_off[nsys];
   // systems go in waves of at most cap/2 systems (each needs a block)
   const int wave = std::max(1, std::min(nsys, cap / 2 > 0 ? cap / 2 : 1));
