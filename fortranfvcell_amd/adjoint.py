@@ -1,8 +1,10 @@
 """Differentiable finite-volume solve: T = A(k)^-1 (b(k) + q) with a
 hand-written backward.
 
-Forward: assemble the heat operator A(k), b(k) (problems.HeatOperator, the
-reference's coefficient forms), solve with Jacobi-BiCGSTAB on the GPU.
+Forward: assemble the heat operator A(k), b(k) on the GPU
+(fvc_assemble_heat_dev, the reference's coefficient forms, the same
+arithmetic as problems.HeatOperator.assemble), solve with Jacobi-BiCGSTAB on
+the GPU.
 Backward: the adjoint solve A^T lam = dL/dT, again Jacobi-BiCGSTAB on the GPU,
 on the same ELLPACK structure with transposed values (fvc_ell_transpose_dev),
 then the face-flux gradient kernel (fvc_grad_k_dev) gives dL/dk; dL/dq = lam.
@@ -39,6 +41,9 @@ class HeatSolver:
         self.alpha = DeviceArray.from_host(np.ascontiguousarray(np.where(inner, op.alpha, 0.0).ravel()))
         self.wall_g = DeviceArray.from_host(np.ascontiguousarray(op.wall_g.ravel()))
         self.wall_T = DeviceArray.from_host(np.ascontiguousarray(op.wall_T.ravel()))
+        self.conv = DeviceArray.from_host(np.ascontiguousarray(op.conv.ravel()))
+        self.conv_f = DeviceArray.from_host(np.ascontiguousarray(op.convF))
+        self.q = DeviceArray.empty(self.neq)
         n, nw = self.neq, self.neq * self.w
         self.ad = DeviceArray.empty(n)
         self.al = DeviceArray.empty(nw)
@@ -63,15 +68,18 @@ class HeatSolver:
         check(rc, (0,) if strict else _NONFATAL)
         return st.as_dict()
 
-    def load(self, k: np.ndarray, q: np.ndarray | None = None) -> P.System:
-        """Assemble A(k), b(k) + q and put them in HBM."""
-        s = self.op.assemble(np.asarray(k, dtype=np.float64), q)
-        inner = s.nbr >= 0
-        self.ad.upload(s.ad)
-        self.al.upload(np.where(inner, s.off, 0.0).ravel())
-        self.b.upload(s.b)
+    def load(self, k: np.ndarray, q: np.ndarray | None = None) -> None:
+        """Assemble A(k), b(k) + q on the GPU (fvc_assemble_heat_dev): only k
+        (and q) cross PCIe; the geometry and convection data stay resident."""
         self.k.upload(np.asarray(k, dtype=np.float64))
-        return s
+        qp = None
+        if q is not None:
+            self.q.upload(np.asarray(q, dtype=np.float64))
+            qp = _ptr(self.q)
+        check(load().fvc_assemble_heat_dev(self.neq, self.w, _ptr(self.ja), _ptr(self.k), qp, _ptr(self.geo),
+                                           _ptr(self.alpha), _ptr(self.conv), _ptr(self.conv_f),
+                                           _ptr(self.wall_g), _ptr(self.wall_T), _ptr(self.ad), _ptr(self.al),
+                                           _ptr(self.b), None))
 
     def forward(self, k: np.ndarray, q: np.ndarray | None = None) -> np.ndarray:
         self.load(k, q)

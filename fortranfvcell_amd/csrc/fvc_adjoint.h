@@ -57,6 +57,49 @@ __global__ void grad_k_kernel(int neq, int w, const int *__restrict__ ja, const 
   grad[c] = acc;
 }
 
+// Face-based assembly of the heat / transport operator A(k) and b(k) in the
+// reference's ELLPACK layout, from per-cell conductivities and fixed
+// geometry and flux data (the coefficient forms of src/CellibTrans.f):
+//   interior face s of cell c, neighbour N:
+//     1/k_f = a/k_c + (1-a)/k_N                  (src/CellibTrans.f:510-511)
+//     D_f = k_f * geo,  a_nb = D_f + conv        (:527, src/CellibSimple.f:789-816)
+//     al(s, c) = -a_nb                           (src/EllPack.f:85)
+//   Dirichlet face: ap_w = k_c * wall_g, added to the diagonal and, times
+//     T_wall, to b                               (src/CellibTrans.f:488-491)
+//   ad = (sum a_nb + convF) + sum ap_w           (src/CellibSimple.f:856-859)
+// One lane per cell, slots summed left to right: the same operations in the
+// same order as the host assembly, so the result is bit-identical to it.
+__global__ void assemble_heat_kernel(int neq, int w, const int *__restrict__ ja, const double *__restrict__ k,
+                                     const double *__restrict__ q, const double *__restrict__ geo,
+                                     const double *__restrict__ alpha, const double *__restrict__ conv,
+                                     const double *__restrict__ conv_f, const double *__restrict__ wall_g,
+                                     const double *__restrict__ wall_T, double *__restrict__ ad,
+                                     double *__restrict__ al, double *__restrict__ b) {
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= neq) return;
+  const double kc = k[c];
+  double sa = 0.0, sw = 0.0, sb = 0.0;
+  for (int s = 0; s < w; ++s) {
+    const size_t e = (size_t)w * c + s;
+    const double gf = geo[e];
+    double anb = 0.0;
+    if (gf != 0.0) {
+      const double a = alpha[e];
+      const double kf = 1.0 / (a / kc + (1.0 - a) / k[ja[e] - 1]);
+      anb = kf * gf + conv[e];
+      al[e] = -anb;
+    } else {
+      al[e] = 0.0;
+    }
+    sa = sa + anb;
+    const double apw = kc * wall_g[e];
+    sw = sw + apw;
+    sb = sb + apw * wall_T[e];
+  }
+  ad[c] = (sa + conv_f[c]) + sw;
+  b[c] = q ? sb + q[c] : sb;
+}
+
 // Plain device copy, four 16-byte loads in flight per lane (bandwidth reference).
 __global__ void copy_kernel(const uint4 *__restrict__ src, uint4 *__restrict__ dst, size_t n16) {
   const size_t stride = (size_t)gridDim.x * blockDim.x;

@@ -1317,6 +1317,19 @@ int fvc_grad_k_dev(int32_t neq, int32_t w, const int32_t *ja, const double *k, c
   return FVC_OK;
 }
 
+int fvc_assemble_heat_dev(int32_t neq, int32_t w, const int32_t *ja, const double *k, const double *q,
+                          const double *geo, const double *alpha, const double *conv, const double *conv_f,
+                          const double *wall_g, const double *wall_t, double *ad, double *al, double *b,
+                          void *stream) {
+  if (neq <= 0 || w < 1 || !ja || !k || !geo || !alpha || !conv || !conv_f || !wall_g || !wall_t || !ad ||
+      !al || !b)
+    return fail(FVC_BAD_ARGUMENT, "bad assembly arguments");
+  hipLaunchKernelGGL(assemble_heat_kernel, dim3((neq + 255) / 256), dim3(256), 0, (hipStream_t)stream, neq, w,
+                     ja, k, q, geo, alpha, conv, conv_f, wall_g, wall_t, ad, al, b);
+  HIPCHK(hipGetLastError());
+  return FVC_OK;
+}
+
 int fvc_matvec(int32_t matriz, int32_t unsym, int32_t neq, int32_t nad, const double *ad,
                const double *au, const double *al, const int32_t *iax, const int32_t *ja,
                const double *x, double *y) {

@@ -200,6 +200,20 @@ int fvc_grad_k_dev(int32_t neq, int32_t w, const int32_t *ja, const double *k, c
                    const double *lam, const double *geo, const double *alpha, const double *wall_g,
                    const double *wall_t, double *grad_k, void *stream);
 
+/* fvc_assemble_heat_dev: the operator whose gradient fvc_grad_k_dev takes,
+ * assembled on the GPU in the reference's ELLPACK layout (ja as given,
+ * al = -a_nb, ad, b) from per-cell k (and an optional source q, NULL for
+ * none), with the reference's coefficient forms: harmonic face conductivity
+ * (src/CellibTrans.f:510-511), a_nb = k_f*geo + conv (src/CellibTrans.f:527,
+ * src/CellibSimple.f:789-816), Dirichlet faces as above, and
+ * ad = (sum a_nb + conv_f) + sum k_c*wall_g (src/CellibSimple.f:856-859).
+ * conv (per slot) and conv_f (per cell) are the fixed upwind convection
+ * coefficients.  Replaces the per-solve upload of a host-assembled A. */
+int fvc_assemble_heat_dev(int32_t neq, int32_t w, const int32_t *ja, const double *k, const double *q,
+                          const double *geo, const double *alpha, const double *conv,
+                          const double *conv_f, const double *wall_g, const double *wall_t,
+                          double *ad, double *al, double *b, void *stream);
+
 /* ----------------------------------------------------------------------------
  * SpMV y = A x.  Replaces the matvec_* routines of src/Matvec.f:21-300 and
  * src/Matvec_omp.f (matvec_csrd[_ilu2]_omp, matvec_csrd_sym_omp family,

@@ -155,3 +155,19 @@ def test_adjoint_full_config_512(fvc):
     err_all = np.linalg.norm(gk - gk_lu) / np.linalg.norm(gk_lu)
     print(f"all {s.neq} cells vs sparse-LU adjoint: relative 2-norm error {err_all:.2e}", flush=True)
     assert err_all <= 1e-8
+
+
+def test_gpu_assembly_is_bit_identical_to_host(fvc):
+    """fvc_assemble_heat_dev against problems.HeatOperator.assemble: the same
+    coefficient forms in the same order, so equal to the last bit."""
+    from fortranfvcell_amd.adjoint import HeatSolver
+    op, k, subset = P.heat_problem(64, subset_frac=0.05)
+    solver = HeatSolver(op)
+    q = np.sin(np.arange(op.mesh.neq) * 0.01)
+    for qq in (None, q):
+        solver.load(k, qq)
+        s = op.assemble(k, qq)
+        inner = s.nbr >= 0
+        assert np.array_equal(solver.ad.get(), s.ad)
+        assert np.array_equal(solver.al.get(), np.where(inner, s.off, 0.0).ravel())
+        assert np.array_equal(solver.b.get(), s.b)
