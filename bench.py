@@ -49,7 +49,7 @@ def bicgstab_iter_bytes(neq: int, nnz_off: int) -> int:
     v, rhat and writes s, t.  Each vector element counted once (neighbour
     gathers are cache hits); the Jacobi diagonal is the matrix's ad."""
     per_matrix = 8 * neq + 12 * nnz_off
-    return 2 * per_matrix + (10 + 7) * 8 * neq
+    return 2 * per_matrix + (10 + 5) * 8 * neq
 
 
 def bicgstab_solve_bytes(neq: int, nnz_off: int, its: int) -> int:
