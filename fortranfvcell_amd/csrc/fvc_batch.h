@@ -37,7 +37,8 @@ struct BatchArgs {
   double *x;
   const double *b;
   double *v[5];                     // FP64 work vectors (concatenated)
-  float *f[5];                      // FP32 work vectors (mixed)
+  float *f[4];                      // FP32 work vectors (mixed): y, then r, p, q
+  size_t fs;                        // with their second buffers at +fs
   float *av32;                      // FP32 scaled values (mixed)
   double *s;                        // 1/sqrt(a_ii) (mixed)
   float *dsq;                       // sqrt(a_ii) (mixed)
@@ -136,9 +137,9 @@ __global__ __launch_bounds__(1024) void mixed_batch_persistent(BatchArgs B) {
   a.b = B.b + off;
   a.y = B.f[0] + off;
   a.r = B.f[1] + off;
-  a.p0 = B.f[2] + off;
-  a.p1 = B.f[3] + off;
-  a.q = B.f[4] + off;
+  a.p = B.f[2] + off;
+  a.q = B.f[3] + off;
+  a.ps = B.fs;
   a.tol = B.tol;
   a.delta = B.delta;
   a.maxit = B.maxit;

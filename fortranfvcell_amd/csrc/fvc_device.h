@@ -16,7 +16,7 @@ namespace fvc {
 
 constexpr int kWave = 64;
 constexpr int kMaxWaves = 16;   // up to 1024 threads per block
-constexpr int kMaxK = 4;        // partial sums per reduction
+constexpr int kMaxK = 5;        // partial sums per reduction
 
 // Row-operator kinds (device formats).
 enum Fmt : int {
@@ -41,32 +41,72 @@ struct Mat {
   const int *tpos;     // position (0-based) of that entry in al/au
 };
 
-// (A x)_i with x supplied by a gather functor xf(j), j 0-based.
 template <int F, class X>
-__device__ __forceinline__ double row_apply(const Mat &A, int i, const X &xf) {
-  if constexpr (F == F_ELL4) {
-    // src/Matvec.f:289-296: y = t1+t2+t3+t4+t5, t5 = ad*x(i)
+__device__ __forceinline__ double row_apply(const Mat &A, int i, const X &xf);
+
+// One matrix row held in registers.  ELLPACK rows of width 3 and 4 are loaded
+// by load() ahead of their use (a phase issues them before its deferred
+// scalar recurrence, so HBM streams while the block reduces); other formats
+// load in apply().  apply() is the only definition of the row's arithmetic.
+template <int F>
+struct RowRegs {
+  __device__ __forceinline__ void load(const Mat &, int) {}
+  template <class X>
+  __device__ __forceinline__ double apply(const Mat &A, int i, const X &xf) const {
+    return row_apply<F>(A, i, xf);
+  }
+};
+template <>
+struct RowRegs<F_ELL4> {
+  double a0, a1, a2, a3, d;
+  int c0, c1, c2, c3;
+  __device__ __forceinline__ void load(const Mat &A, int i) {
     const double *a = A.av + 4 * (size_t)i;
     const int *c = A.ja + 4 * (size_t)i;
-    const double a0 = a[0], a1 = a[1], a2 = a[2], a3 = a[3];
-    const int c0 = c[0], c1 = c[1], c2 = c[2], c3 = c[3];
-    const double t5 = A.ad[i] * xf(i);
+    a0 = a[0]; a1 = a[1]; a2 = a[2]; a3 = a[3];
+    c0 = c[0]; c1 = c[1]; c2 = c[2]; c3 = c[3];
+    d = A.ad[i];
+  }
+  // src/Matvec.f:289-296: y = t1+t2+t3+t4+t5, t5 = ad*x(i)
+  template <class X>
+  __device__ __forceinline__ double apply(const Mat &, int i, const X &xf) const {
+    const double t5 = d * xf(i);
     const double t1 = a0 * xf(c0 - 1);
     const double t2 = a1 * xf(c1 - 1);
     const double t3 = a2 * xf(c2 - 1);
     const double t4 = a3 * xf(c3 - 1);
     return (((t1 + t2) + t3) + t4) + t5;
-  } else if constexpr (F == F_ELL3) {
-    // src/Matvec.f:281-287: y = t1+t2+t3+t4, t4 = ad*x(i)
+  }
+};
+template <>
+struct RowRegs<F_ELL3> {
+  double a0, a1, a2, d;
+  int c0, c1, c2;
+  __device__ __forceinline__ void load(const Mat &A, int i) {
     const double *a = A.av + 3 * (size_t)i;
     const int *c = A.ja + 3 * (size_t)i;
-    const double a0 = a[0], a1 = a[1], a2 = a[2];
-    const int c0 = c[0], c1 = c[1], c2 = c[2];
-    const double t4 = A.ad[i] * xf(i);
+    a0 = a[0]; a1 = a[1]; a2 = a[2];
+    c0 = c[0]; c1 = c[1]; c2 = c[2];
+    d = A.ad[i];
+  }
+  // src/Matvec.f:281-287: y = t1+t2+t3+t4, t4 = ad*x(i)
+  template <class X>
+  __device__ __forceinline__ double apply(const Mat &, int i, const X &xf) const {
+    const double t4 = d * xf(i);
     const double t1 = a0 * xf(c0 - 1);
     const double t2 = a1 * xf(c1 - 1);
     const double t3 = a2 * xf(c2 - 1);
     return ((t1 + t2) + t3) + t4;
+  }
+};
+
+// (A x)_i with x supplied by a gather functor xf(j), j 0-based.
+template <int F, class X>
+__device__ __forceinline__ double row_apply(const Mat &A, int i, const X &xf) {
+  if constexpr (F == F_ELL4 || F == F_ELL3) {
+    RowRegs<F> r;
+    r.load(A, i);
+    return r.apply(A, i, xf);
   } else if constexpr (F == F_ELLW) {
     // Same pattern for any width: slots left to right, diagonal last.
     const double *a = A.av + (size_t)A.w * (size_t)i;
@@ -108,8 +148,24 @@ __device__ __forceinline__ double wave_sum(double v) {
   return v;   // valid in lane 0
 }
 
+// A block barrier that orders LDS only: the block's outstanding global loads
+// and stores stay in flight across it.  For code in which no thread reads
+// global data another thread of the block wrote in the same launch.
+__device__ __forceinline__ void lds_barrier() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+  __builtin_amdgcn_s_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
+
+template <bool LdsOnly>
+__device__ __forceinline__ void block_barrier() {
+  if constexpr (LdsOnly) lds_barrier();
+  else __syncthreads();
+}
+
 // Sums K values over the block; the totals land in thread 0's v[].
-template <int K>
+// LdsOnly: the barriers order LDS only (see lds_barrier).
+template <int K, bool LdsOnly = false>
 __device__ __forceinline__ void block_sum(double *v, double *lds) {
   const int lane = threadIdx.x & (kWave - 1), wid = threadIdx.x / kWave;
   const int nw = (blockDim.x + kWave - 1) / kWave;
@@ -119,7 +175,7 @@ __device__ __forceinline__ void block_sum(double *v, double *lds) {
 #pragma unroll
     for (int k = 0; k < K; ++k) lds[k * kMaxWaves + wid] = v[k];
   }
-  __syncthreads();
+  block_barrier<LdsOnly>();
   if (threadIdx.x == 0) {
 #pragma unroll
     for (int k = 0; k < K; ++k) {
@@ -128,23 +184,94 @@ __device__ __forceinline__ void block_sum(double *v, double *lds) {
       v[k] = s;
     }
   }
-  __syncthreads();
+  block_barrier<LdsOnly>();
 }
 
-// Deterministic sum of G partials per value (partials[k*G + c]).
-template <int K>
+// Deterministic sum of G partials per value (partials[k*G + c]): thread t
+// adds c = t, t + nt, t + 2 nt, ... in that order, then the block sums the
+// threads.  The loads of U consecutive terms are issued together, so a
+// thread waits for one round trip per U terms, not one per term (U = 1 for
+// kernels whose registers are already scarce).
+template <int K, bool LdsOnly = false, int U = 4>
 __device__ __forceinline__ void sum_partials(const double *partials, int G, double *v,
                                              double *lds) {
+  const int nt = blockDim.x;
+  double s[K];
 #pragma unroll
-  for (int k = 0; k < K; ++k) {
-    double s = 0.0;
-    for (int c = threadIdx.x; c < G; c += blockDim.x) s += partials[(size_t)k * G + c];
-    v[k] = s;
+  for (int k = 0; k < K; ++k) s[k] = 0.0;
+  if constexpr (U == 1) {
+    for (int c = threadIdx.x; c < G; c += nt) {
+#pragma unroll
+      for (int k = 0; k < K; ++k) s[k] += partials[(size_t)k * G + c];
+    }
+  } else {
+  for (int c = threadIdx.x; c < G; c += U * nt) {
+    double q[U][K];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int cu = c + u * nt;
+#pragma unroll
+      for (int k = 0; k < K; ++k) q[u][k] = cu < G ? partials[(size_t)k * G + cu] : 0.0;
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      if (c + u * nt < G) {
+#pragma unroll
+        for (int k = 0; k < K; ++k) s[k] += q[u][k];
+      }
+    }
   }
-  block_sum<K>(v, lds);
+  }
+#pragma unroll
+  for (int k = 0; k < K; ++k) v[k] = s[k];
+  block_sum<K, LdsOnly>(v, lds);
 }
 
+// The same sum with the loads issued ahead: load() takes the thread's terms
+// of the first KM values (at most four terms each, G <= 4 nt) into registers
+// before the caller knows which K <= KM it needs; sum<K>() then adds the
+// first K values in sum_partials' order.  With G > 4 nt load() holds nothing
+// and sum<K>() falls back to sum_partials.
+template <int KM>
+struct PartialRegs {
+  double q[4][KM];
+  bool held;
+  __device__ __forceinline__ void load(const double *partials, int G) {
+    const int nt = blockDim.x, c = threadIdx.x;
+    held = G <= 4 * nt;
+    if (!held) return;
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const int cu = c + u * nt;
+#pragma unroll
+      for (int k = 0; k < KM; ++k) q[u][k] = cu < G ? partials[(size_t)k * G + cu] : 0.0;
+    }
+  }
+  template <int K, bool LdsOnly>
+  __device__ __forceinline__ void sum(const double *partials, int G, double *v, double *lds) const {
+    static_assert(K <= KM, "more values than loaded");
+    if (!held) {
+      sum_partials<K, LdsOnly>(partials, G, v, lds);
+      return;
+    }
+    const int nt = blockDim.x, c = threadIdx.x;
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+      double s = 0.0;
+#pragma unroll
+      for (int u = 0; u < 4; ++u)
+        if (c + u * nt < G) s += q[u][k];
+      v[k] = s;
+    }
+    block_sum<K, LdsOnly>(v, lds);
+  }
+};
+
 // -------------------------------------------------- grid barrier + reduce --
+
+// Terms per load batch in the grid barrier's partial sums: one, the
+// persistent kernels (the batched ones above all) have no registers to spare.
+constexpr int kGridSumU = 1;
 //
 // ctr layout (unsigned, 128-byte strides): [g*32] arrival counter of group g
 // (g = block % 8, matching the round-robin placement of blocks on the eight
@@ -249,7 +376,7 @@ __device__ bool grid_reduce(const Grid &g, double *v, double *lds, Epi &&epi) {
     double tot[K > 0 ? K : 1];
     if constexpr (K > 0) {
       double w[K];
-      sum_partials<K>(g.partials, g.G, w, lds);
+      sum_partials<K, false, kGridSumU>(g.partials, g.G, w, lds);
 #pragma unroll
       for (int k = 0; k < K; ++k) tot[k] = w[k];
     }
@@ -323,7 +450,7 @@ __device__ bool grid_reduce_flags(const Grid &g, double *v, double *lds, Epi &&e
     double tot[K > 0 ? K : 1];
     if constexpr (K > 0) {
       double w[K];
-      sum_partials<K>(g.partials, g.G, w, lds);
+      sum_partials<K, false, kGridSumU>(g.partials, g.G, w, lds);
 #pragma unroll
       for (int k = 0; k < K; ++k) tot[k] = w[k];
     }

@@ -20,7 +20,7 @@ constexpr int S_OK = 0, S_MAXIT = 1, S_BREAKDOWN = 2, S_NONFINITE = 3;
 
 struct alignas(128) KState {
   double d, conv, alpha, beta, rr0, omega, energy, last;
-  double bnorm2, res2, econv, rnorm2, upd_est, d_prev, pad2, pad3;
+  double bnorm2, res2, econv, rnorm2, upd_est, d_prev, tscale, pad3;   // tscale: mixed refreshes
   int iter, status, done, inner;      // inner: GMRES Arnoldi steps (nit)
   int step_i, mgs_j, inner_done, ni;  // GMRES position in the cycle
   int cycle, pend, next, pad6;         // pend: deferred reduction pending (Defer);
@@ -55,47 +55,66 @@ struct Defer {
 };
 
 // M: nred(ph) values reduced by phase ph; skip(ph, state) whether the phase's
-// rows are skipped (solve finished); rows(ph, state, v); epi(ph, state, t).
-// ph = kPhaseNext: the phase the state names (KState::next, set by the
-// recurrences of methods whose phase order depends on the iterates);
+// rows are skipped (solve finished); rows(ph, state, v, pre); epi(ph, state,
+// t); prefetch(ph, stored state) -> M::Pre, the operands of the thread's
+// first row that do not depend on the state (an empty struct for methods
+// that load nothing ahead).  ph = kPhaseNext: the phase the state names (KState::next, set by
+// the recurrences of methods whose phase order depends on the iterates);
 // other ph < 0: no rows, only the pending recurrence (a flush).
+//
+// The first row's operands are issued before the pending recurrence, so that
+// HBM streams while the block sums the partials; the block's barriers order
+// LDS only (no thread reads global data another thread wrote in the launch),
+// so those loads and the rows' stores stay in flight across them.  A state
+// that already says the solve is finished loads nothing ahead.
 constexpr int kPhaseNext = -2;
+
+struct NoPre {};
 
 template <class M>
 __device__ void deferred_phase(const Grid &g, const Defer &d, int ph, const M &m, KState *S,
                                double *lds) {
   constexpr int nw = (int)(sizeof(KState) / sizeof(unsigned));
-  const unsigned *in = reinterpret_cast<const unsigned *>(d.st + d.par);
+  const KState *stored = d.st + d.par;
+  const unsigned *in = reinterpret_cast<const unsigned *>(stored);
+  const double *pin = d.part + (size_t)d.par * kMaxK * g.G;
   for (int k = threadIdx.x; k < nw; k += blockDim.x) reinterpret_cast<unsigned *>(S)[k] = in[k];
-  __syncthreads();
+  // the partials of the pending phase, loaded before the state says which
+  // phase that is (the unused values are never read)
+  PartialRegs<M::kRed> pr;
+  pr.load(pin, g.G);
+  typename M::Pre pre{};
+  if (ph != -1 && !stored->done) pre = m.prefetch(ph, *stored);
+  lds_barrier();
   if (S->pend) {
     const int pp = S->pend - 1;
-    const double *pin = d.part + (size_t)d.par * kMaxK * g.G;
-    double t[kMaxK] = {0.0, 0.0, 0.0, 0.0};
+    double t[M::kRed] = {};
     switch (m.nred(pp)) {
       case 0: break;
-      case 1: sum_partials<1>(pin, g.G, t, lds); break;
-      case 2: sum_partials<2>(pin, g.G, t, lds); break;
-      case 3: sum_partials<3>(pin, g.G, t, lds); break;
-      default: sum_partials<4>(pin, g.G, t, lds); break;
+      case 1: pr.template sum<1, true>(pin, g.G, t, lds); break;
+      case 2: pr.template sum<2, true>(pin, g.G, t, lds); break;
+      case 3: if constexpr (M::kRed >= 3) pr.template sum<3, true>(pin, g.G, t, lds); break;
+      case 4: if constexpr (M::kRed >= 4) pr.template sum<4, true>(pin, g.G, t, lds); break;
+      default: if constexpr (M::kRed >= 5) pr.template sum<5, true>(pin, g.G, t, lds); break;
     }
     if (threadIdx.x == 0) {
       m.epi(pp, S, t);
       S->pend = 0;
     }
-    __syncthreads();
+    lds_barrier();
   }
   if (ph == kPhaseNext) ph = S->done ? -1 : S->next;
   if (ph >= 0 && !m.skip(ph, *S)) {
-    double v[kMaxK] = {0.0, 0.0, 0.0, 0.0};
-    m.rows(ph, S, v);
+    double v[M::kRed] = {};
+    m.rows(ph, S, v, pre);
     const int K = m.nred(ph);
     switch (K) {
       case 0: break;
-      case 1: block_sum<1>(v, lds); break;
-      case 2: block_sum<2>(v, lds); break;
-      case 3: block_sum<3>(v, lds); break;
-      default: block_sum<4>(v, lds); break;
+      case 1: block_sum<1, true>(v, lds); break;
+      case 2: block_sum<2, true>(v, lds); break;
+      case 3: if constexpr (M::kRed >= 3) block_sum<3, true>(v, lds); break;
+      case 4: if constexpr (M::kRed >= 4) block_sum<4, true>(v, lds); break;
+      default: if constexpr (M::kRed >= 5) block_sum<5, true>(v, lds); break;
     }
     if (threadIdx.x == 0) {
       double *out = d.part + (size_t)(1 - d.par) * kMaxK * g.G;
@@ -105,7 +124,7 @@ __device__ void deferred_phase(const Grid &g, const Defer &d, int ph, const M &m
     }
   }
   if (g.bid() == 0) {
-    __syncthreads();
+    lds_barrier();
     unsigned *out = reinterpret_cast<unsigned *>(d.st + 1 - d.par);
     for (int k = threadIdx.x; k < nw; k += blockDim.x) out[k] = reinterpret_cast<const unsigned *>(S)[k];
   }
@@ -347,8 +366,23 @@ struct XDiv {
   __device__ __forceinline__ double operator()(int j) const { return r[j] / m[j]; }
 };
 
+// A CG1_ITER row's matrix operands, loadable ahead of the row.
 template <int F>
-__device__ void cg1_rows(const Cg1Args &a, int ph, const KState *S, double *v) {
+struct Cg1Row {
+  RowRegs<F> mr;
+  bool ok;
+};
+template <int F>
+__device__ __forceinline__ Cg1Row<F> cg1_load(const Cg1Args &a, int i) {
+  Cg1Row<F> q;
+  q.mr.load(a.A, i);
+  q.ok = true;
+  return q;
+}
+
+template <int F>
+__device__ void cg1_rows(const Cg1Args &a, int ph, const KState *S, double *v,
+                         const Cg1Row<F> *pre = nullptr) {
   const int i0 = a.g.row_begin(), i1 = a.g.row_end();
   if (ph == CG1_SETUP) {
     // r = b - A x (or r = b for x0 = 0); d = r.M^-1 r; ||b||^2, ||r||^2
@@ -381,7 +415,7 @@ __device__ void cg1_rows(const Cg1Args &a, int ph, const KState *S, double *v) {
     const double *rc = odd ? a.r0 : a.r1, *wc = odd ? a.w0 : a.w1, *sc = odd ? a.s0 : a.s1;
     double *rn = odd ? a.r1 : a.r0, *wn = odd ? a.w1 : a.w0, *sn = odd ? a.s1 : a.s0;
     const XCgNext xf{rc, wc, sc, a.m, ld_dbl(&S->alpha), ld_dbl(&S->beta), j == 1};
-    for (int i = i0 + (int)threadIdx.x; i < i1; i += blockDim.x) {
+    auto row = [&](int i, const Cg1Row<F> &q) {
       const double mi = a.m[i];
       const double ui = rc[i] / mi;
       const double pi = xf.first ? ui : ui + xf.beta * a.p[i];
@@ -389,7 +423,7 @@ __device__ void cg1_rows(const Cg1Args &a, int ph, const KState *S, double *v) {
       a.x[i] = a.x[i] + xf.alpha * pi;
       const double ri = rc[i] - xf.alpha * si;
       const double un = ri / mi;
-      const double wi = row_apply<F>(a.A, i, xf);
+      const double wi = q.mr.apply(a.A, i, xf);
       a.p[i] = pi;
       sn[i] = si;
       rn[i] = ri;
@@ -397,7 +431,13 @@ __device__ void cg1_rows(const Cg1Args &a, int ph, const KState *S, double *v) {
       v[0] += ri * un;
       v[1] += wi * un;
       v[2] += ri * ri;
+    };
+    int i = i0 + (int)threadIdx.x;
+    if (pre && pre->ok && i < i1) {
+      row(i, *pre);
+      i += blockDim.x;
     }
+    for (; i < i1; i += blockDim.x) row(i, cg1_load<F>(a, i));
   } else {
     // CG1_FINAL: energy norm x.Ax (src/Iterativos.f:82-83) and ||b - Ax||^2
     for (int i = i0 + (int)threadIdx.x; i < i1; i += blockDim.x) {
@@ -496,9 +536,17 @@ __global__ __launch_bounds__(1024) void cg1_kernel(Cg1Args a) {
   __shared__ KState S;
   struct M {
     const Cg1Args &a;
+    enum { kRed = 3 };
+    using Pre = Cg1Row<F>;
     __device__ int nred(int ph) const { return cg1_nred(ph); }
     __device__ bool skip(int ph, const KState &s) const { return (ph == CG1_W0 || ph == CG1_ITER) && s.done; }
-    __device__ void rows(int ph, const KState *s, double *v) const { cg1_rows<F>(a, ph, s, v); }
+    __device__ Pre prefetch(int ph, const KState &) const {
+      Pre q{};
+      const int i = a.g.row_begin() + (int)threadIdx.x;
+      if (ph == CG1_ITER && i < a.g.row_end()) q = cg1_load<F>(a, i);
+      return q;
+    }
+    __device__ void rows(int ph, const KState *s, double *v, const Pre &q) const { cg1_rows<F>(a, ph, s, v, &q); }
     __device__ void epi(int ph, KState *s, const double *t) const { cg1_epi(a, ph, s, t); }
   };
   deferred_phase(a.g, a.d, a.phase, M{a}, &S, lds);
@@ -759,8 +807,23 @@ struct XBi2Dir {
   }
 };
 
+// A P or Q row's matrix operands, loadable ahead of the row.
 template <int F>
-__device__ void bicg2_rows(const Bicg2Args &a, int ph, const KState *S, double *v) {
+struct Bi2Row {
+  RowRegs<F> mr;
+  bool ok;
+};
+template <int F>
+__device__ __forceinline__ Bi2Row<F> bicg2_load(const Bicg2Args &a, int i) {
+  Bi2Row<F> q;
+  q.mr.load(a.A, i);
+  q.ok = true;
+  return q;
+}
+
+template <int F>
+__device__ void bicg2_rows(const Bicg2Args &a, int ph, const KState *S, double *v,
+                           const Bi2Row<F> *pre = nullptr) {
   const int i0 = a.g.row_begin(), i1 = a.g.row_end();
   if (ph == B2_SETUP) {
     // r = b - Ax; rhat = r; p = r; b = r; d = r.(r/m); rho = b.rhat = r.r
@@ -809,7 +872,7 @@ __device__ void bicg2_rows(const Bicg2Args &a, int ph, const KState *S, double *
     } else {
       const double alpha = ld_dbl(&S->alpha);
       const XBi2Dir xf{a.s, a.t, pc, vc, a.m, ld_dbl(&S->beta), ld_dbl(&S->omega)};
-      for (int i = i0 + (int)threadIdx.x; i < i1; i += blockDim.x) {
+      auto row = [&](int i, const Bi2Row<F> &q) {
         const double mi = a.m[i];
         const double si = a.s[i];
         const double z2i = si / mi;
@@ -817,21 +880,27 @@ __device__ void bicg2_rows(const Bicg2Args &a, int ph, const KState *S, double *
         a.x[i] = (a.x[i] + alpha * (pc[i] / mi)) + xf.w * z2i;
         a.bb[i] = bi;
         const double pi = bi + xf.beta * (pc[i] - xf.w * vc[i]);
-        const double vi = row_apply<F>(a.A, i, xf);
+        const double vi = q.mr.apply(a.A, i, xf);
         pn[i] = pi;
         vn[i] = vi;
         v[0] += vi * a.r[i];
         v[1] += bi * z2i;
         v[2] += bi * bi;
+      };
+      int i = i0 + (int)threadIdx.x;
+      if (pre && pre->ok && i < i1) {
+        row(i, *pre);
+        i += blockDim.x;
       }
+      for (; i < i1; i += blockDim.x) row(i, bicg2_load<F>(a, i));
     }
     return;
   }
   // B2_Q
   const XBiRes xf{a.bb, vn, a.m, ld_dbl(&S->alpha)};
-  for (int i = i0 + (int)threadIdx.x; i < i1; i += blockDim.x) {
+  auto row = [&](int i, const Bi2Row<F> &q) {
     const double si = a.bb[i] - xf.alpha * vn[i];
-    const double ti = row_apply<F>(a.A, i, xf);
+    const double ti = q.mr.apply(a.A, i, xf);
     const double ri = a.r[i];
     a.s[i] = si;
     a.t[i] = ti;
@@ -839,7 +908,13 @@ __device__ void bicg2_rows(const Bicg2Args &a, int ph, const KState *S, double *
     v[1] += ti * ti;
     v[2] += ri * si;
     v[3] += ri * ti;
+  };
+  int i = i0 + (int)threadIdx.x;
+  if (pre && pre->ok && i < i1) {
+    row(i, *pre);
+    i += blockDim.x;
   }
+  for (; i < i1; i += blockDim.x) row(i, bicg2_load<F>(a, i));
 }
 
 __device__ inline void bicg2_epi(const Bicg2Args &a, int ph, KState *st, const double *t) {
@@ -935,9 +1010,17 @@ __global__ __launch_bounds__(1024) void bicg2_kernel(Bicg2Args a) {
   __shared__ KState S;
   struct M {
     const Bicg2Args &a;
+    enum { kRed = 4 };
+    using Pre = Bi2Row<F>;
     __device__ int nred(int ph) const { return bicg2_nred(ph); }
     __device__ bool skip(int ph, const KState &s) const { return (ph == B2_P || ph == B2_Q) && s.done; }
-    __device__ void rows(int ph, const KState *s, double *v) const { bicg2_rows<F>(a, ph, s, v); }
+    __device__ Pre prefetch(int ph, const KState &) const {
+      Pre q{};
+      const int i = a.g.row_begin() + (int)threadIdx.x;
+      if ((ph == B2_P || ph == B2_Q) && i < a.g.row_end()) q = bicg2_load<F>(a, i);
+      return q;
+    }
+    __device__ void rows(int ph, const KState *s, double *v, const Pre &q) const { bicg2_rows<F>(a, ph, s, v, &q); }
     __device__ void epi(int ph, KState *s, const double *t) const { bicg2_epi(a, ph, s, t); }
   };
   deferred_phase(a.g, a.d, a.phase, M{a}, &S, lds);

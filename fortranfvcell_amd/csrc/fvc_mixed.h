@@ -7,12 +7,11 @@
 // m = ad).  Scaling first makes the unit-diagonal matrix safe in FP32: the
 // reference's 1e60 pinning penalty (src/CellibPressure.f:414,557) overflows
 // FP32, the scaled row does not.  The FP32 recursion drifts from the true
-// residual, so whenever its unscaled estimate ||S^-1 r~|| falls by a factor
-// `delta`, reaches the tolerance, or `max_inner` FP32 iterations have passed,
-// the solution increment is folded into the FP64 x and the residual is
-// recomputed in FP64 from the FP64 matrix ("reliable updates", van der Vorst
-// & Ye 2000; Clark et al. 2010); the search direction is kept.  The solve
-// stops when that FP64 residual satisfies ||b - A x|| <= tol ||b||.
+// residual, so from time to time (see MX_A below) the solution increment is
+// folded into the FP64 x and the residual is recomputed in FP64 from the FP64
+// matrix ("reliable updates", van der Vorst & Ye 2000; Clark et al. 2010);
+// the search direction is kept.  The solve stops when that FP64 residual
+// satisfies ||b - A x|| <= tol ||b||.
 #pragma once
 #include "fvc_krylov.h"
 
@@ -25,20 +24,61 @@ struct Mat32 {
   const int *ja;
 };
 
+template <int F, class X>
+__device__ __forceinline__ float row_apply32(const Mat32 &A, int i, const X &xf);
+
+// One FP32 matrix row in registers, loaded ahead of its use for ELLPACK
+// widths 3 and 4 (as RowRegs for FP64); apply() is the only definition of
+// the row's arithmetic.
+template <int F>
+struct RowRegs32 {
+  __device__ __forceinline__ void load(const Mat32 &, int) {}
+  template <class X>
+  __device__ __forceinline__ float apply(const Mat32 &A, int i, const X &xf) const {
+    return row_apply32<F>(A, i, xf);
+  }
+};
+template <>
+struct RowRegs32<F_ELL4> {
+  float a0, a1, a2, a3;
+  int c0, c1, c2, c3;
+  __device__ __forceinline__ void load(const Mat32 &A, int i) {
+    const float *a = A.av + 4 * (size_t)i;
+    const int *c = A.ja + 4 * (size_t)i;
+    a0 = a[0]; a1 = a[1]; a2 = a[2]; a3 = a[3];
+    c0 = c[0]; c1 = c[1]; c2 = c[2]; c3 = c[3];
+  }
+  template <class X>
+  __device__ __forceinline__ float apply(const Mat32 &, int i, const X &xf) const {
+    const float t1 = a0 * xf(c0 - 1), t2 = a1 * xf(c1 - 1);
+    const float t3 = a2 * xf(c2 - 1), t4 = a3 * xf(c3 - 1);
+    return (((t1 + t2) + t3) + t4) + xf(i);
+  }
+};
+template <>
+struct RowRegs32<F_ELL3> {
+  float a0, a1, a2;
+  int c0, c1, c2;
+  __device__ __forceinline__ void load(const Mat32 &A, int i) {
+    const float *a = A.av + 3 * (size_t)i;
+    const int *c = A.ja + 3 * (size_t)i;
+    a0 = a[0]; a1 = a[1]; a2 = a[2];
+    c0 = c[0]; c1 = c[1]; c2 = c[2];
+  }
+  template <class X>
+  __device__ __forceinline__ float apply(const Mat32 &, int i, const X &xf) const {
+    const float t1 = a0 * xf(c0 - 1), t2 = a1 * xf(c1 - 1), t3 = a2 * xf(c2 - 1);
+    return ((t1 + t2) + t3) + xf(i);
+  }
+};
+
 // (S A S x)_i with unit diagonal; slots left to right, diagonal last.
 template <int F, class X>
 __device__ __forceinline__ float row_apply32(const Mat32 &A, int i, const X &xf) {
-  if constexpr (F == F_ELL4) {
-    const float *a = A.av + 4 * (size_t)i;
-    const int *c = A.ja + 4 * (size_t)i;
-    const float t1 = a[0] * xf(c[0] - 1), t2 = a[1] * xf(c[1] - 1);
-    const float t3 = a[2] * xf(c[2] - 1), t4 = a[3] * xf(c[3] - 1);
-    return (((t1 + t2) + t3) + t4) + xf(i);
-  } else if constexpr (F == F_ELL3) {
-    const float *a = A.av + 3 * (size_t)i;
-    const int *c = A.ja + 3 * (size_t)i;
-    const float t1 = a[0] * xf(c[0] - 1), t2 = a[1] * xf(c[1] - 1), t3 = a[2] * xf(c[2] - 1);
-    return ((t1 + t2) + t3) + xf(i);
+  if constexpr (F == F_ELL4 || F == F_ELL3) {
+    RowRegs32<F> r;
+    r.load(A, i);
+    return r.apply(A, i, xf);
   } else if constexpr (F == F_ELLW) {
     const float *a = A.av + (size_t)A.w * (size_t)i;
     const int *c = A.ja + (size_t)A.w * (size_t)i;
@@ -52,16 +92,6 @@ __device__ __forceinline__ float row_apply32(const Mat32 &A, int i, const X &xf)
     return t;
   }
 }
-
-struct XVec32 {
-  const float *x;
-  __device__ __forceinline__ float operator()(int j) const { return x[j]; }
-};
-struct XDir32 {
-  const float *r, *p;
-  float beta;
-  __device__ __forceinline__ float operator()(int j) const { return r[j] + beta * p[j]; }
-};
 
 // Builds S A S in FP32: av32 = a_ij * s_i * s_j, s_i = 1/sqrt(a_ii); also
 // s (FP64) and dsq = sqrt(a_ii) (FP32, to unscale residual estimates).
@@ -87,10 +117,10 @@ __global__ void scale_to_f32_kernel(Mat A, const double *m, float *av32, double 
   }
 }
 
-enum { MX_SETUP = 0, MX_A = 1, MX_B = 2, MX_X = 3, MX_R = 4, MX_FINAL = 5 };
+enum { MX_SETUP = 0, MX_A = 1, MX_X = 3, MX_R = 4, MX_FINAL = 5 };
 
 __host__ __device__ constexpr int mixed_nred(int ph) {
-  return ph == MX_A ? 1 : ph == MX_X ? 0 : (ph == MX_B || ph == MX_FINAL) ? 2 : 3;
+  return ph == MX_A ? 5 : ph == MX_X ? 0 : ph == MX_FINAL ? 2 : 3;
 }
 
 struct MixedArgs {
@@ -103,27 +133,92 @@ struct MixedArgs {
   const float *dsq;   // sqrt(a_ii)
   double *x;
   const double *b;
-  float *y, *r, *p0, *p1, *q;
+  float *y;
+  float *r, *p, *q;   // two buffers each, the second at +ps: they alternate by
+  size_t ps;          // iteration parity
   double tol, delta;
   int maxit, x0_zero, max_inner, phase;
   Defer d;
 };
 
-// KState use: d = r~.r~ (FP64 accumulation), d_prev = its value before the
-// current iteration, conv = tol*||b||, last = the latest residual estimate,
-// upd_est = the estimate at the last reliable update, cycle = number of
-// reliable updates, inner = FP32 iterations since it, step_i = update due,
-// next = the phase that follows (the order depends on the iterates: after
-// B either the next iteration's A or an update's X and R).
+// The inner iteration is one phase, MX_A, with one reduction: CG in the form
+// of D'Azevedo, Eijkhout & Romine ("Conjugate gradient algorithms with
+// reduced synchronization overhead", 1993).  Iteration j, for its own rows:
+//   completes iteration j-1:  y += alpha p,  r~ -= alpha q   (unless the
+//                             residual is fresh from a refresh);
+//   p = r~ + beta p,  q = (S A S) p  (p of the neighbours formed on the fly
+//                             from r~, q and p of iteration j-1);
+//   reduces p.q, r~.q, q.q, r~.r~ and ||S^-1 r~||^2.
+// The recurrence then takes alpha = rho / p.q with rho = r~.r~ exact, and
+// rho' = rho - 2 alpha r~.q + alpha^2 q.q for the next beta: a one-step
+// recurrence anchored every iteration on the exact rho of this pass, so it
+// does not drift (q is always a true product, never a recurrence).
+// r~, p and q alternate between two buffers because the next iteration's
+// gathers read the old values of other blocks' rows.
+//
+// A reliable update (MX_X: x += S (y + alpha p); MX_R: r~ = S (b - A x) in
+// FP64) is due when
+//   * the scaled residual ||r~'|| (from rho') has fallen by `delta` since the
+//     last update,
+//   * the unscaled estimate ||S^-1 r~|| of this pass reaches tol ||b|| times
+//     `tscale` (not in the first pass after an update, whose residual was
+//     just measured in FP64); tscale starts at 1 and, when an update so
+//     triggered finds the true residual still above the tolerance, shrinks
+//     by the ratio estimate / true residual it observed,
+//   * `max_inner` FP32 iterations have passed since the last update.
+// After an update the direction keeps its history: beta = r~new.r~new / rho.
+//
+// KState use: d = rho of the last pass, rr0 = r~.r~ at the last update,
+// alpha, beta = the pending step and the next direction's coefficient,
+// conv = tol*||b||, last = the latest residual estimate, tscale, cycle =
+// number of updates, inner = FP32 iterations since the last one, step_i = 1
+// while r~ is fresh from an update (nothing pending), ni = 1 to restart the
+// direction (p = r~), inner_done = 1 when the tolerance test triggered the
+// update, next = the phase that follows.
 // Row and scalar parts are split as for the one-reduction PCG, so that the
 // persistent kernel, the batched kernel and the kernel-per-phase engine with
 // deferred reductions all drive the same code.
+
+// p of iteration j at row k: (r~ - alpha q) + beta p, branch-free so that a
+// row's gathers issue together.  alpha = 0 while r~ is fresh from an update
+// (nothing pending) and beta = 0 when the direction restarts; the setup
+// zero-fills the first p and q buffers, so 0 * p and 0 * q are exact zeros.
+struct XMixDir {
+  const float *r, *q, *p;
+  float alpha, beta;
+  __device__ __forceinline__ float rnew(int k) const { return r[k] - alpha * q[k]; }
+  __device__ __forceinline__ float operator()(int k) const { return rnew(k) + beta * p[k]; }
+};
+
+// The operands of an MX_A row that do not alternate between buffers: the
+// FP32 matrix row, y and sqrt(a_ii).
 template <int F>
-__device__ void mixed_rows(const MixedArgs &a, int ph, const KState *S, double *v) {
+struct MixRow {
+  RowRegs32<F> mr;
+  float y, dsq;
+  bool ok;
+};
+template <int F>
+__device__ __forceinline__ MixRow<F> mixed_load(const MixedArgs &a, int i) {
+  MixRow<F> q;
+  q.mr.load(a.A32, i);
+  q.y = a.y[i];
+  q.dsq = a.dsq[i];
+  q.ok = true;
+  return q;
+}
+
+template <int F>
+__device__ __forceinline__ void mixed_rows(const MixedArgs &a, int ph, const KState *S, double *v,
+                                           const MixRow<F> *pre = nullptr) {
   const int i0 = a.g.row_begin(), i1 = a.g.row_end();
   if (ph == MX_SETUP || ph == MX_R) {
-    // r64 = b - A x (FP64); r~ = S r64 rounded to FP32; y = 0
-    const bool zero = (ph == MX_SETUP) && a.x0_zero;
+    // r64 = b - A x (FP64); r~ = S r64 rounded to FP32 into the buffer the
+    // next MX_A reads; y = 0
+    const bool setup = ph == MX_SETUP, zero = setup && a.x0_zero;
+    const int j = setup ? 1 : ld_int(&S->iter) + 1;
+    const size_t oc = (j & 1) ? 0 : a.ps;
+    float *rc = a.r + oc;
     for (int i = i0 + (int)threadIdx.x; i < i1; i += blockDim.x) {
       double ax;
       if (zero) {
@@ -135,49 +230,58 @@ __device__ void mixed_rows(const MixedArgs &a, int ph, const KState *S, double *
       const double bi = a.b[i];
       const double ri = bi - ax;
       const float rt = (float)(ri * a.s[i]);
-      a.r[i] = rt;
+      rc[i] = rt;
       a.y[i] = 0.0f;
+      if (setup) {
+        a.p[oc + i] = 0.0f;
+        a.q[oc + i] = 0.0f;
+      }
       v[0] += (double)rt * (double)rt;
       v[1] += ri * ri;
       v[2] += bi * bi;
     }
   } else if (ph == MX_A) {
-    // p = r~ + beta p (on the fly, ping-pong); q = (S A S) p; -> p.q
     const int j = ld_int(&S->iter) + 1;
-    float *pn = (j & 1) ? a.p1 : a.p0;
-    const float *po = (j & 1) ? a.p0 : a.p1;
-    const bool restart = j == 1;
-    const XDir32 xf{a.r, po, restart ? 0.0f : (float)ld_dbl(&S->beta)};
-    for (int i = i0 + (int)threadIdx.x; i < i1; i += blockDim.x) {
-      float qi, pi;
-      if (restart) {
-        qi = row_apply32<F>(a.A32, i, XVec32{a.r});
-        pi = a.r[i];
-      } else {
-        qi = row_apply32<F>(a.A32, i, xf);
-        pi = xf(i);
-      }
+    const size_t oc = (j & 1) ? 0 : a.ps, on = a.ps - oc;
+    const float *rc = a.r + oc, *pc = a.p + oc, *qc = a.q + oc;
+    float *rn = a.r + on, *pn = a.p + on, *qn = a.q + on;
+    // a fresh residual has nothing pending (alpha = 0), a restart no history
+    // (beta = 0)
+    const float alpha = ld_int(&S->step_i) ? 0.0f : (float)ld_dbl(&S->alpha);
+    const float beta = ld_int(&S->ni) ? 0.0f : (float)ld_dbl(&S->beta);
+    const XMixDir xf{rc, qc, pc, alpha, beta};
+    auto row = [&](int i, const MixRow<F> &w) {
+      a.y[i] = w.y + xf.alpha * pc[i];
+      const float ri = xf.rnew(i);
+      const float pi = xf(i);
+      const float qi = w.mr.apply(a.A32, i, xf);
+      rn[i] = ri;
       pn[i] = pi;
-      a.q[i] = qi;
-      v[0] += (double)pi * (double)qi;
+      qn[i] = qi;
+      const double rd = ri, qd = qi;
+      v[0] += (double)pi * qd;
+      v[1] += rd * qd;
+      v[2] += qd * qd;
+      v[3] += rd * rd;
+      const double ru = rd * (double)w.dsq;
+      v[4] += ru * ru;
+    };
+    int i = i0 + (int)threadIdx.x;
+    if (pre && pre->ok && i < i1) {
+      row(i, *pre);
+      i += blockDim.x;
     }
-  } else if (ph == MX_B) {
-    // y += alpha p; r~ -= alpha q; -> r~.r~ and the unscaled estimate ||S^-1 r~||^2
-    const int j = ld_int(&S->iter) + 1;
-    const float *pn = (j & 1) ? a.p1 : a.p0;
+    for (; i < i1; i += blockDim.x) row(i, mixed_load<F>(a, i));
+  } else if (ph == MX_X) {
+    // complete the last step and fold the FP32 increment into the FP64
+    // solution: x += S (y + alpha p)
+    const int j = ld_int(&S->iter);
+    const float *pj = a.p + ((j & 1) ? a.ps : 0);
     const float alpha = (float)ld_dbl(&S->alpha);
     for (int i = i0 + (int)threadIdx.x; i < i1; i += blockDim.x) {
-      const float yi = a.y[i] + alpha * pn[i];
-      const float ri = a.r[i] - alpha * a.q[i];
-      a.y[i] = yi;
-      a.r[i] = ri;
-      v[0] += (double)ri * (double)ri;
-      const double ru = (double)ri * (double)a.dsq[i];
-      v[1] += ru * ru;
+      const float yi = a.y[i] + alpha * pj[i];
+      a.x[i] = a.x[i] + a.s[i] * (double)yi;
     }
-  } else if (ph == MX_X) {
-    // fold the FP32 increment into the FP64 solution: x += S y
-    for (int i = i0 + (int)threadIdx.x; i < i1; i += blockDim.x) a.x[i] = a.x[i] + a.s[i] * (double)a.y[i];
   } else {
     // MX_FINAL: energy x.Ax and ||b - Ax||^2 in FP64
     for (int i = i0 + (int)threadIdx.x; i < i1; i += blockDim.x) {
@@ -189,8 +293,9 @@ __device__ void mixed_rows(const MixedArgs &a, int ph, const KState *S, double *
   }
 }
 
-__device__ inline void mixed_epi(const MixedArgs &a, int ph, KState *st, const double *t) {
+__device__ __forceinline__ void mixed_epi(const MixedArgs &a, int ph, KState *st, const double *t) {
   if (ph == MX_SETUP || ph == MX_R) {
+    const double tr = sqrt(t[1]);
     if (ph == MX_SETUP) {
       st->bnorm2 = t[2];
       st->conv = a.tol * sqrt(t[2]);
@@ -198,39 +303,47 @@ __device__ inline void mixed_epi(const MixedArgs &a, int ph, KState *st, const d
       st->cycle = 0;
       st->status = S_MAXIT;
       st->done = 0;
+      st->tscale = 1.0;
+      st->ni = 1;
     } else {
       st->cycle = st->cycle + 1;
+      if (st->inner_done && tr > st->conv && st->last < tr) st->tscale = st->tscale * (st->last / tr);
+      // the recomputed residual replaces the recursive one; the direction
+      // keeps its history
+      const double beta = t[0] / st->d;
+      st->ni = (st->d > 0.0 && finite(beta)) ? 0 : 1;
+      st->beta = st->ni ? 0.0 : beta;
     }
     st->res2 = t[1];
-    st->last = sqrt(t[1]);
-    st->upd_est = st->last;
-    // the recomputed residual replaces the recursive one; the direction
-    // keeps its history: beta = r~new.r~new / r~old.r~old
-    if (ph == MX_R) st->beta = t[0] / st->d_prev;
-    st->d = t[0];
+    st->last = tr;
+    st->rr0 = t[0];
     st->inner = 0;
-    st->step_i = 0;
+    st->inner_done = 0;
+    st->step_i = 1;
     st->next = MX_A;
-    if (st->last <= st->conv) { st->status = S_OK; st->done = 1; }
+    if (tr <= st->conv) { st->status = S_OK; st->done = 1; }
     else if (!finite(t[0]) || !finite(t[1])) { st->status = S_NONFINITE; st->done = 1; }
     else if (st->iter >= a.maxit) { st->status = S_MAXIT; st->done = 1; }
   } else if (ph == MX_A) {
-    st->alpha = st->d / t[0];
-    st->next = MX_B;
-    if (t[0] == 0.0) { st->status = S_BREAKDOWN; st->done = 1; }
-    else if (!finite(st->alpha)) { st->status = S_NONFINITE; st->done = 1; }
-  } else if (ph == MX_B) {
+    const double sig = t[0], tau = t[1], th = t[2], rho = t[3];
     const int j = st->iter + 1;
     st->iter = j;
     st->inner = st->inner + 1;
-    const double est = sqrt(t[1]);
+    const double alpha = rho / sig;
+    const double rho1 = rho - 2.0 * alpha * tau + alpha * alpha * th;
+    const double beta = rho1 / rho;
+    const double est = sqrt(t[4]);
+    const bool ok = sig != 0.0 && rho != 0.0 && finite(alpha) && finite(beta);
     st->last = est;
-    st->d_prev = st->d;
-    st->beta = t[0] / st->d;
-    st->d = t[0];
-    const bool upd = est <= st->conv || est < a.delta * st->upd_est || st->inner >= a.max_inner ||
-                     j >= a.maxit || !finite(t[0]) || t[0] == 0.0;
-    st->step_i = upd ? 1 : 0;
+    st->d = rho;
+    st->alpha = ok ? alpha : 0.0;
+    st->beta = ok ? beta : 0.0;
+    st->step_i = 0;
+    st->ni = 0;
+    const bool tolhit = st->inner > 1 && est <= st->conv * st->tscale;
+    const bool drop = sqrt(fmax(rho1, 0.0)) < a.delta * sqrt(st->rr0);
+    st->inner_done = tolhit ? 1 : 0;
+    const bool upd = !ok || tolhit || drop || st->inner >= a.max_inner || j >= a.maxit;
     st->next = upd ? MX_X : MX_A;
   } else if (ph == MX_X) {
     st->next = MX_R;
@@ -241,15 +354,15 @@ __device__ inline void mixed_epi(const MixedArgs &a, int ph, KState *st, const d
 }
 
 template <int F>
-__device__ bool mixed_phase(const MixedArgs &a, int ph, double *lds) {
-  double v[kMaxK] = {0.0, 0.0, 0.0, 0.0};
+__device__ __forceinline__ bool mixed_phase(const MixedArgs &a, int ph, double *lds) {
+  double v[kMaxK] = {};
   mixed_rows<F>(a, ph, a.st, v);
   auto epi = [&](const double *t) { mixed_epi(a, ph, a.st, t); };
   switch (mixed_nred(ph)) {
     case 0: return grid_sync<0>(a.g, v, lds, epi);
-    case 1: return grid_sync<1>(a.g, v, lds, epi);
     case 2: return grid_sync<2>(a.g, v, lds, epi);
-    default: return grid_sync<3>(a.g, v, lds, epi);
+    case 3: return grid_sync<3>(a.g, v, lds, epi);
+    default: return grid_sync<5>(a.g, v, lds, epi);
   }
 }
 
@@ -280,9 +393,21 @@ __global__ __launch_bounds__(1024) void mixed_kernel(MixedArgs a) {
   __shared__ KState S;
   struct M {
     const MixedArgs &a;
+    enum { kRed = 5 };
+    using Pre = MixRow<F>;
     __device__ int nred(int ph) const { return mixed_nred(ph); }
     __device__ bool skip(int ph, const KState &s) const { return ph != MX_SETUP && ph != MX_FINAL && s.done; }
-    __device__ void rows(int ph, const KState *s, double *v) const { mixed_rows<F>(a, ph, s, v); }
+    // the phase a kPhaseNext launch runs is known only after the pending
+    // recurrence: after X it is R, otherwise nearly always A (X follows an
+    // A only at a reliable update), so A's operands are loaded ahead
+    __device__ Pre prefetch(int ph, const KState &stored) const {
+      Pre q{};
+      const int i = a.g.row_begin() + (int)threadIdx.x;
+      const bool a_next = ph == MX_A || (ph == kPhaseNext && stored.pend - 1 != MX_X);
+      if (a_next && i < a.g.row_end()) q = mixed_load<F>(a, i);
+      return q;
+    }
+    __device__ void rows(int ph, const KState *s, double *v, const Pre &q) const { mixed_rows<F>(a, ph, s, v, &q); }
     __device__ void epi(int ph, KState *s, const double *t) const { mixed_epi(a, ph, s, t); }
   };
   deferred_phase(a.g, a.d, a.phase, M{a}, &S, lds);

@@ -207,7 +207,7 @@ int kernels_blocks(int neq, int cus, int requested, int per_cu = 4) {
   if (G >= 64) G -= G % 8;
   return std::max(G, 1);
 }
-int kernels_block(int code, bool fused) { return (code == FVC_PCG && fused) ? 256 : 512; }
+int kernels_block(int code, bool fused) { return (fused && code != FVC_GMRES) ? 256 : 512; }
 
 int choose_blocks(int neq, int cus, int occ_blocks, int requested) {
   int G = requested > 0 ? requested : cus * std::max(1, std::min(occ_blocks, 2));
@@ -577,16 +577,17 @@ int run_mixed(DevState &d, const SolveIn &in, fvc_stats *stats, hipStream_t s) {
     const int neq = in.A.neq;
     // AUTO runs one kernel per phase, as for FP64 PCG
     const bool persistent = in.engine == FVC_ENGINE_PERSISTENT;
-    const int bs = in.bs > 0 ? in.bs : kDefaultBlock;
+    // one-launch inner iterations: four blocks of 256 per CU, as for FP64
+    // PCG, measured best on MI355X (1M cells: 19.3 us per inner iteration
+    // against 21.8 at two blocks of 512 and 29.0 at four of 512)
+    const int bs = in.bs > 0 ? in.bs : persistent ? kDefaultBlock : kernels_block(FVC_PCG, true);
     if (bs % 64 != 0 || bs > 1024) return fail(FVC_BAD_ARGUMENT, "block size %d must be a multiple of 64 <= 1024", bs);
     const void *pfn = (const void *)&mixed_persistent<F>;
     const void *kfn = (const void *)&mixed_kernel<F>;
     int occ = 0;
     int rc = occupancy(pfn, bs, &occ);
     if (rc) return rc;
-    // mixed phases are lighter (FP32): two blocks of 512 per CU measured best
-    // (1M cells: 232 ms against 249 ms at four per CU)
-    const int G = persistent ? choose_blocks(neq, d.cus, occ, in.G_req) : kernels_blocks(neq, d.cus, in.G_req, 2);
+    const int G = persistent ? choose_blocks(neq, d.cus, occ, in.G_req) : kernels_blocks(neq, d.cus, in.G_req);
     if (persistent && G > occ * d.cus)
       return fail(FVC_BAD_ARGUMENT, "%d blocks exceed the %d co-resident blocks of a persistent launch", G, occ * d.cus);
     const size_t n = (size_t)neq;
@@ -599,7 +600,7 @@ int run_mixed(DevState &d, const SolveIn &in, fvc_stats *stats, hipStream_t s) {
     }
     size_t bytes = align_up((kCtrWords + (size_t)G) * sizeof(unsigned)) + align_up(sizeof(KState)) +
                    align_up(kMaxK * (size_t)G * sizeof(double)) + align_up(nv * sizeof(float)) +
-                   align_up(n * sizeof(double)) + 6 * align_up(n * sizeof(float)) +
+                   align_up(n * sizeof(double)) + 8 * align_up(n * sizeof(float)) +
                    align_up(2 * sizeof(KState)) + align_up(2 * kMaxK * (size_t)G * sizeof(double)) + 4096;
     if ((rc = ensure_arena(d, bytes))) return rc;
     Carve cv{d.arena};
@@ -612,10 +613,10 @@ int run_mixed(DevState &d, const SolveIn &in, fvc_stats *stats, hipStream_t s) {
     float *dsq = cv.take<float>(n);
     MixedArgs a{};
     a.y = cv.take<float>(n);
-    a.r = cv.take<float>(n);
-    a.p0 = cv.take<float>(n);
-    a.p1 = cv.take<float>(n);
-    a.q = cv.take<float>(n);
+    a.ps = align_up(n * sizeof(float)) / sizeof(float);
+    a.r = cv.take<float>(2 * a.ps);
+    a.p = cv.take<float>(2 * a.ps);
+    a.q = cv.take<float>(2 * a.ps);
     a.A = in.A;
     a.A32.neq = neq;
     a.A32.w = in.A.w;
@@ -1099,7 +1100,7 @@ int run_batch(DevState &d, int nsys, const std::vector<int> &row_off, int w, con
   size_t bytes = align_up(((size_t)wave * kCtrWords + cap) * sizeof(unsigned)) + align_up((size_t)wave * sizeof(KState)) +
                  align_up((size_t)kMaxK * cap * sizeof(double)) + align_up((size_t)wave * sizeof(SysDesc)) +
                  align_up((size_t)cap * sizeof(int)) + 4096;
-  if (mixed) bytes += 5 * align_up(n * sizeof(float)) + align_up(n * w * sizeof(float)) +
+  if (mixed) bytes += 7 * align_up(n * sizeof(float)) + align_up(n * w * sizeof(float)) +
                       align_up(n * sizeof(double)) + align_up(n * sizeof(float));
   else bytes += 5 * align_up(n * sizeof(double));
   if ((rc = ensure_arena(d, bytes))) return rc;
@@ -1125,7 +1126,9 @@ int run_batch(DevState &d, int nsys, const std::vector<int> &row_off, int w, con
   B.delta = o.refresh_drop_e3 > 0 ? o.refresh_drop_e3 * 1e-3 : kRefreshDrop;
   if (B.delta >= 1.0) return fail(FVC_BAD_ARGUMENT, "refresh_drop_e3 %d must be below 1000", o.refresh_drop_e3);
   if (mixed) {
-    for (float *&f : B.f) f = cv.take<float>(n);
+    B.fs = align_up(n * sizeof(float)) / sizeof(float);
+    B.f[0] = cv.take<float>(n);
+    for (int k = 1; k < 4; ++k) B.f[k] = cv.take<float>(2 * B.fs);
     B.av32 = cv.take<float>(n * w);
     B.s = cv.take<double>(n);
     B.dsq = cv.take<float>(n);
