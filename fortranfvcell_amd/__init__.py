@@ -24,7 +24,7 @@ from ._lib import (BICGSTAB, CSR, CSRC, CSRD, ELLPACK, ENGINE_AUTO, ENGINE_KERNE
                    X0_AS_SEQUENTIAL, X0_GIVEN, X0_ZERO, FvcError, Options, Stats, check,
                    default_options, load)
 
-__all__ = ["solver", "solve", "solve_dev", "matvec", "matvec_dev", "dot", "dot_dev", "getres",
+__all__ = ["solver", "solve", "keep_matrix", "solve_dev", "matvec", "matvec_dev", "dot", "dot_dev", "getres",
            "DeviceArray", "DeviceSystem", "Options", "Stats", "FvcError", "default_options", "load",
            "PCG", "BICGSTAB", "GMRES", "CSRD", "CSR", "CSRC", "ELLPACK"]
 
@@ -93,6 +93,15 @@ def solver(ad, au, al, x, b, iax, ja, neq, nad, unsym, tol, maxit, code, matriz,
                       C.byref(st))
     check(rc, (_lib.FVC_OK,) if raise_on_failure else _NONFATAL)
     return st.as_dict()
+
+
+def keep_matrix(keep: bool = True) -> None:
+    """The next host-array ``solver`` / ``solve`` call on this device reuses
+    the device copy of A uploaded by the previous one, if the format, sizes
+    and array addresses match; the caller guarantees the contents are
+    unchanged (the two pressure-correction solves of a SIMPLE step,
+    src/Simple.f:590,621).  One-shot."""
+    check(load().fvc_keep_matrix(int(bool(keep))))
 
 
 def _fmt_arrays(fmt: dict):

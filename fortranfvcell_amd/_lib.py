@@ -25,7 +25,8 @@ REC_FUSED, REC_REFERENCE = 0, 1
 
 # Every symbol include/fvcell.h declares.
 EXPORTS = [
-    "fvc_default_options", "fvc_init", "fvc_last_error", "fvc_solver", "fvc_solve", "fvc_solve_dev",
+    "fvc_default_options", "fvc_init", "fvc_last_error", "fvc_solver", "fvc_solve", "fvc_keep_matrix",
+    "fvc_solve_dev",
     "fvc_solve_batch", "fvc_solve_batch_dev", "fvc_ell_transpose_dev", "fvc_grad_k_dev",
     "fvc_assemble_heat_dev",
     "fvc_matvec", "fvc_matvec_dev", "fvc_dot", "fvc_dot_dev", "fvc_getres", "fvc_getres_dev",
@@ -78,6 +79,7 @@ def load(path: str = LIB_PATH):
     L.fvc_last_error.argtypes = []
     L.fvc_solver.argtypes = [P, P, P, P, P, P, P, I, I, I, D, I, I, I, I, I, C.POINTER(Stats)]
     L.fvc_solve.argtypes = [I, I, I, I, I, P, P, P, P, P, P, P, D, I, C.POINTER(Options), C.POINTER(Stats)]
+    L.fvc_keep_matrix.argtypes = [I]
     L.fvc_solve_dev.argtypes = [I, I, I, I, I, P, P, P, P, P, P, P, D, I, C.POINTER(Options),
                                 C.POINTER(Stats), P]
     L.fvc_solve_batch.argtypes = [I, I, P, I, P, P, P, P, P, D, I, C.POINTER(Options), P]

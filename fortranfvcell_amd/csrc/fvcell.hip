@@ -53,6 +53,35 @@ constexpr double kRefreshDrop = 0.3;
 
 // ------------------------------------------------------- device workspace --
 
+struct MatDesc {
+  int F = -1;
+  Mat m{};
+  int nval = 0;          // entries of av (and au)
+  bool needs_trans = false;
+  bool csr_diag = false; // Jacobi diagonal must be extracted (full CSR)
+};
+
+// Device buffers the host-array path keeps between calls (grow-only): its
+// uploads reuse them instead of allocating and freeing per call.
+struct DevBufs {
+  static constexpr int kSlots = 12;
+  void *p[kSlots] = {};
+  size_t cap[kSlots] = {};
+  int next = 0;
+};
+
+// What identifies a host matrix for fvc_keep_matrix: its format, shape and
+// the addresses of its arrays (the caller vouches for the contents).
+struct HostMatKey {
+  int matriz = -1, unsym = 0, neq = 0, w = 0;
+  size_t nval = 0;
+  const void *ad = nullptr, *au = nullptr, *al = nullptr, *iax = nullptr, *ja = nullptr;
+  bool operator==(const HostMatKey &o) const {
+    return matriz == o.matriz && unsym == o.unsym && neq == o.neq && w == o.w && nval == o.nval &&
+           ad == o.ad && au == o.au && al == o.al && iax == o.iax && ja == o.ja;
+  }
+};
+
 struct DevState {
   bool ready = false;
   int device = -1;
@@ -67,6 +96,13 @@ struct DevState {
   double *host_scalar = nullptr;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   hipEvent_t evq[2] = {nullptr, nullptr};
+  // host-array solves: the uploaded matrix (kept for fvc_keep_matrix), x, b
+  DevBufs mat_bufs, vec_bufs;
+  HostMatKey mat_key;
+  MatDesc mat_md;
+  const double *mat_m = nullptr;
+  bool mat_valid = false;
+  bool keep_next = false;
 };
 
 DevState g_dev[64];
@@ -128,14 +164,6 @@ int ensure_arena(DevState &d, size_t bytes) {
 }
 
 // ------------------------------------------------------------ matrix view --
-
-struct MatDesc {
-  int F = -1;
-  Mat m{};
-  int nval = 0;          // entries of av (and au)
-  bool needs_trans = false;
-  bool csr_diag = false; // Jacobi diagonal must be extracted (full CSR)
-};
 
 int fmt_for(int matriz, int unsym, int w, int *F) {
   switch (matriz) {
@@ -765,18 +793,36 @@ int check_common(int code, int matriz, int neq, const void *ad, const void *x, c
 
 // Builds the device matrix from host arrays (uploading them) or device
 // arrays (used in place).  Owned uploads land in `own`.
+// With `persist` set, uploads take the next slot of those device buffers
+// (grown when too small) and outlive this object; otherwise they are freed
+// with it.
 struct Owned {
   std::vector<void *> ptrs;
+  DevBufs *persist = nullptr;
   ~Owned() {
     for (void *p : ptrs) (void)hipFree(p);
   }
   template <class T>
   int up(T **dst, const T *src, size_t n) {
     if (n == 0) { *dst = nullptr; return FVC_OK; }
+    const size_t bytes = n * sizeof(T);
     void *p = nullptr;
-    HIPCHK(hipMalloc(&p, n * sizeof(T)));
-    ptrs.push_back(p);
-    if (src) HIPCHK(hipMemcpy(p, src, n * sizeof(T), hipMemcpyHostToDevice));
+    if (persist) {
+      if (persist->next >= DevBufs::kSlots) return fail(FVC_DEVICE_ERROR, "out of upload slots");
+      const int k = persist->next++;
+      if (persist->cap[k] < bytes) {
+        if (persist->p[k]) HIPCHK(hipFree(persist->p[k]));
+        persist->p[k] = nullptr;
+        persist->cap[k] = 0;
+        HIPCHK(hipMalloc(&persist->p[k], bytes));
+        persist->cap[k] = bytes;
+      }
+      p = persist->p[k];
+    } else {
+      HIPCHK(hipMalloc(&p, bytes));
+      ptrs.push_back(p);
+    }
+    if (src) HIPCHK(hipMemcpy(p, src, bytes, hipMemcpyHostToDevice));
     *dst = (T *)p;
     return FVC_OK;
   }
@@ -901,16 +947,48 @@ int solve_impl(int code, int matriz, int unsym, int neq, int nad, const double *
   Owned own;
   MatDesc md;
   const double *m = nullptr;
-  if ((rc = build_device_matrix(matriz, unsym, neq, nad, ad, au, al, iax, ja, on_device, own, md, &m))) {
-    if (stats) stats->status = rc;
-    return rc;
-  }
   double *dx = x;
   const double *db = b;
-  if (!on_device) {
+  if (on_device) {
+    if ((rc = build_device_matrix(matriz, unsym, neq, nad, ad, au, al, iax, ja, true, own, md, &m))) {
+      if (stats) stats->status = rc;
+      return rc;
+    }
+  } else {
+    // host arrays: uploaded into the device's kept buffers, or not uploaded
+    // at all when the caller kept the matrix of the previous host solve
+    const bool keep = d->keep_next;
+    d->keep_next = false;
+    HostMatKey key;
+    key.matriz = matriz;
+    key.unsym = unsym ? 1 : 0;
+    key.neq = neq;
+    key.w = matriz == FVC_ELLPACK ? iax[0] : 0;
+    key.nval = matriz == FVC_ELLPACK ? (size_t)key.w * neq : (size_t)std::max(iax[neq] - 1, 0);
+    key.ad = ad; key.au = au; key.al = al; key.iax = iax; key.ja = ja;
+    if (keep && d->mat_valid && key == d->mat_key) {
+      md = d->mat_md;
+      m = d->mat_m;
+    } else {
+      d->mat_valid = false;
+      d->mat_bufs.next = 0;
+      Owned kept;
+      kept.persist = &d->mat_bufs;
+      if ((rc = build_device_matrix(matriz, unsym, neq, nad, ad, au, al, iax, ja, false, kept, md, &m))) {
+        if (stats) stats->status = rc;
+        return rc;
+      }
+      d->mat_key = key;
+      d->mat_md = md;
+      d->mat_m = m;
+      d->mat_valid = true;
+    }
+    d->vec_bufs.next = 0;
+    Owned vecs;
+    vecs.persist = &d->vec_bufs;
     double *tx = nullptr, *tb = nullptr;
-    if ((rc = own.up(&tx, x, (size_t)neq))) return rc;
-    if ((rc = own.up(&tb, b, (size_t)neq))) return rc;
+    if ((rc = vecs.up(&tx, x, (size_t)neq))) return rc;
+    if ((rc = vecs.up(&tb, b, (size_t)neq))) return rc;
     dx = tx;
     db = tb;
   }
@@ -1267,6 +1345,15 @@ int fvc_solver(const double *ad, const double *au, const double *al, double *x, 
   o.x0_rule = omp_solver ? FVC_X0_AS_OPENMP : FVC_X0_AS_SEQUENTIAL;
   return solve_impl(code, matriz, unsym, neq, nad, ad, au, al, x, b, iax, ja, tol, maxit, &o,
                     stats, false, nullptr);
+}
+
+int fvc_keep_matrix(int32_t keep) {
+  DevState *d = nullptr;
+  const int rc = current_device(&d);
+  if (rc) return rc;
+  std::lock_guard<std::mutex> lk(d->mu);
+  d->keep_next = keep != 0;
+  return FVC_OK;
 }
 
 int fvc_solve(int32_t code, int32_t matriz, int32_t unsym, int32_t neq, int32_t nad,

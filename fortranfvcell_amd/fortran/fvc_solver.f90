@@ -48,6 +48,11 @@ module fvc_capi
     type(c_ptr) function fvc_last_error() bind(C, name='fvc_last_error')
       import :: c_ptr
     end function fvc_last_error
+
+    integer(c_int) function fvc_keep_matrix(keep) bind(C, name='fvc_keep_matrix')
+      import :: c_int, c_int32_t
+      integer(c_int32_t), value :: keep
+    end function fvc_keep_matrix
   end interface
 
   ! .true. mirrors the reference (print the warning and STOP on failure);
@@ -64,6 +69,17 @@ subroutine fvc_set_stop_on_failure(flag)
   logical, intent(in) :: flag
   stop_on_failure = flag
 end subroutine fvc_set_stop_on_failure
+
+! The next solver call reuses the device copy of A uploaded by the previous
+! one (same arrays, contents unchanged by the caller): the second
+! pressure-correction solve of a SIMPLE step (src/Simple.f:621), after which
+! pform rebuilt only the right-hand side.  One-shot.
+subroutine solverKeepMatrix()
+  use fvc_capi
+  implicit none
+  integer(c_int) :: rc
+  rc = fvc_keep_matrix(1_c_int32_t)
+end subroutine solverKeepMatrix
 
 subroutine fvc_get_last(status, iterations, energy, true_relres, seconds)
   use fvc_capi

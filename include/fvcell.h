@@ -103,8 +103,9 @@ typedef struct fvc_options {
   int32_t recurrence;     /* FVC_REC_*; PCG and BiCGSTAB                              */
   int32_t refresh_every;  /* mixed: at most this many FP32 iterations between FP64
                              residual recomputations; 0 = 200                         */
-  int32_t refresh_drop_e3; /* mixed: recompute when the residual estimate fell by
-                             this factor (thousandths) since the last one; 0 = 300   */
+  int32_t refresh_drop_e3; /* mixed: recompute when the FP32 (Jacobi-scaled) residual
+                             norm fell by this factor (thousandths) since the last
+                             one; 0 = 300                                            */
   int32_t reserved[3];
 } fvc_options;
 
@@ -155,6 +156,17 @@ int fvc_solve(int32_t code, int32_t matriz, int32_t unsym, int32_t neq, int32_t 
               const double *ad, const double *au, const double *al, double *x,
               const double *b, const int32_t *iax, const int32_t *ja, double tol,
               int32_t maxit, const fvc_options *opt, fvc_stats *stats);
+
+/* Keeps the matrix between the two pressure-correction solves of a SIMPLE
+ * step (src/Simple.f:590 and :621, between which pform rebuilds only the
+ * right-hand side, src/Simple.f:607-615).  With keep != 0, the next fvc_solver
+ * / fvc_solve call on this device does not upload A again if its format, neq,
+ * entry count and the addresses of ad, au, al, iax, ja equal those of the
+ * previous host-array solve; the caller guarantees that their contents are
+ * unchanged.  One-shot: the call after that uploads as usual.  (The
+ * host-array calls also keep their device copies of A, x and b between calls,
+ * in grow-only buffers, instead of allocating them per call.) */
+int fvc_keep_matrix(int32_t keep);
 
 /* The same on device memory; runs on `stream`.  For ELLPACK, `iax` may be a
  * host or device pointer (its first entry, the width, is read once). */

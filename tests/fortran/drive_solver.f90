@@ -68,5 +68,15 @@ program drive_solver
   write(11) energy
   write(11) u
   write(11) time, matvectime, dottime, precontime
+  ! the second pressure-correction solve of a SIMPLE step: same A, kept on
+  ! the device, right-hand side 2 b; every iterate scales by the power of
+  ! two exactly, so the solution is exactly 2 x
+  b = 2.0d0 * b
+  x = 0.0d0
+  open(13, file=trim(flog)//'.kept', status='replace', action='write')
+  call solverKeepMatrix()
+  call solver(ad, au, al, x, b, iax, ja, neq, nad, unsym, tol, maxit, code, matriz, 2)
+  close(13)
+  write(11) x
   close(11)
 end program drive_solver

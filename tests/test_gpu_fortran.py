@@ -37,8 +37,10 @@ def _run(tmp_path, f, code, unsym, omp, tol=1e-10, maxit=20000, x0=None):
     status, iters = np.frombuffer(raw[8 * neq: 8 * neq + 8], np.int32)
     energy = np.frombuffer(raw[8 * neq + 8: 8 * neq + 16], np.float64)[0]
     u = np.frombuffer(raw[8 * neq + 16: 16 * neq + 16], np.float64)
-    timers = np.frombuffer(raw[16 * neq + 16:], np.float64)
+    timers = np.frombuffer(raw[16 * neq + 16: 16 * neq + 48], np.float64)
     _run.timers = dict(zip(("time", "matvectime", "dottime", "precontime"), timers))
+    # the driver's second solve: solverKeepMatrix, then the same A with 2 b
+    _run.x_kept = np.frombuffer(raw[16 * neq + 48: 24 * neq + 48], np.float64)
     return x, int(status), int(iters), energy, u, open(flog).read(), r.stdout
 
 
@@ -56,6 +58,8 @@ def test_fortran_dropin_matches_oracle(fvc, orc, tmp_path, code, lay, omp):
     assert abs(energy - so["energy"]) <= 1e-8 * abs(so["energy"])
     # getRes with num(i) = neq+1-i: u(i) = x(num(i))
     assert np.array_equal(u, x[::-1])
+    # solverKeepMatrix + solve with 2 b on the kept A: exactly 2 x
+    assert np.array_equal(_run.x_kept, 2.0 * x)
     # COMMON /timectrl/: time = this solve's time, added to matvectime
     tm = _run.timers
     assert tm["time"] > 0 and tm["matvectime"] == tm["time"]
