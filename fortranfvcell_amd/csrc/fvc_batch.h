@@ -37,8 +37,8 @@ struct BatchArgs {
   double *x;
   const double *b;
   double *v[5];                     // FP64 work vectors (concatenated)
-  float *f[4];                      // FP32 work vectors (mixed): y, then r, p, q
-  size_t fs;                        // with their second buffers at +fs
+  float4 *rec;                      // mixed: {r~, q, p, y} records, two buffers,
+  size_t fs;                        // the second at +fs
   float *av32;                      // FP32 scaled values (mixed)
   double *s;                        // 1/sqrt(a_ii) (mixed)
   float *dsq;                       // sqrt(a_ii) (mixed)
@@ -135,10 +135,7 @@ __global__ __launch_bounds__(1024) void mixed_batch_persistent(BatchArgs B) {
   a.dsq = B.dsq + off;
   a.x = B.x + off;
   a.b = B.b + off;
-  a.y = B.f[0] + off;
-  a.r = B.f[1] + off;
-  a.p = B.f[2] + off;
-  a.q = B.f[3] + off;
+  a.rec = B.rec + off;
   a.ps = B.fs;
   a.tol = B.tol;
   a.delta = B.delta;

@@ -133,9 +133,11 @@ struct MixedArgs {
   const float *dsq;   // sqrt(a_ii)
   double *x;
   const double *b;
-  float *y;
-  float *r, *p, *q;   // two buffers each, the second at +ps: they alternate by
-  size_t ps;          // iteration parity
+  // Records {r~, q, p, y} per row, two buffers alternating by iteration
+  // parity, the second at +ps floats.  MX_A reads the current record (a
+  // neighbour's r~, q, p are one 16-byte load) and writes the whole next one.
+  float4 *rec;
+  size_t ps;
   double tol, delta;
   int maxit, x0_zero, max_inner, phase;
   Defer d;
@@ -182,27 +184,29 @@ struct MixedArgs {
 // p of iteration j at row k: (r~ - alpha q) + beta p, branch-free so that a
 // row's gathers issue together.  alpha = 0 while r~ is fresh from an update
 // (nothing pending) and beta = 0 when the direction restarts; the setup
-// zero-fills the first p and q buffers, so 0 * p and 0 * q are exact zeros.
+// zero-fills the first record's q and p, so 0 * p and 0 * q are exact zeros.
 struct XMixDir {
-  const float *r, *q, *p;
+  const float4 *rc;
   float alpha, beta;
-  __device__ __forceinline__ float rnew(int k) const { return r[k] - alpha * q[k]; }
-  __device__ __forceinline__ float operator()(int k) const { return rnew(k) + beta * p[k]; }
+  __device__ __forceinline__ static float rnew(const float4 &c, float alpha) { return c.x - alpha * c.y; }
+  __device__ __forceinline__ float operator()(int k) const {
+    const float4 c = rc[k];
+    return rnew(c, alpha) + beta * c.z;
+  }
 };
 
 // The operands of an MX_A row that do not alternate between buffers: the
-// FP32 matrix row, y and sqrt(a_ii).
+// FP32 matrix row and sqrt(a_ii).
 template <int F>
 struct MixRow {
   RowRegs32<F> mr;
-  float y, dsq;
+  float dsq;
   bool ok;
 };
 template <int F>
 __device__ __forceinline__ MixRow<F> mixed_load(const MixedArgs &a, int i) {
   MixRow<F> q;
   q.mr.load(a.A32, i);
-  q.y = a.y[i];
   q.dsq = a.dsq[i];
   q.ok = true;
   return q;
@@ -217,8 +221,7 @@ __device__ __forceinline__ void mixed_rows(const MixedArgs &a, int ph, const KSt
     // next MX_A reads; y = 0
     const bool setup = ph == MX_SETUP, zero = setup && a.x0_zero;
     const int j = setup ? 1 : ld_int(&S->iter) + 1;
-    const size_t oc = (j & 1) ? 0 : a.ps;
-    float *rc = a.r + oc;
+    float4 *rc = a.rec + ((j & 1) ? 0 : a.ps);
     for (int i = i0 + (int)threadIdx.x; i < i1; i += blockDim.x) {
       double ax;
       if (zero) {
@@ -230,11 +233,13 @@ __device__ __forceinline__ void mixed_rows(const MixedArgs &a, int ph, const KSt
       const double bi = a.b[i];
       const double ri = bi - ax;
       const float rt = (float)(ri * a.s[i]);
-      rc[i] = rt;
-      a.y[i] = 0.0f;
       if (setup) {
-        a.p[oc + i] = 0.0f;
-        a.q[oc + i] = 0.0f;
+        rc[i] = make_float4(rt, 0.0f, 0.0f, 0.0f);
+      } else {
+        // the refreshed residual; q and p stay (the direction keeps its history)
+        float *f = reinterpret_cast<float *>(rc + i);
+        f[0] = rt;
+        f[3] = 0.0f;
       }
       v[0] += (double)rt * (double)rt;
       v[1] += ri * ri;
@@ -243,21 +248,20 @@ __device__ __forceinline__ void mixed_rows(const MixedArgs &a, int ph, const KSt
   } else if (ph == MX_A) {
     const int j = ld_int(&S->iter) + 1;
     const size_t oc = (j & 1) ? 0 : a.ps, on = a.ps - oc;
-    const float *rc = a.r + oc, *pc = a.p + oc, *qc = a.q + oc;
-    float *rn = a.r + on, *pn = a.p + on, *qn = a.q + on;
+    const float4 *rc = a.rec + oc;
+    float4 *rn = a.rec + on;
     // a fresh residual has nothing pending (alpha = 0), a restart no history
     // (beta = 0)
     const float alpha = ld_int(&S->step_i) ? 0.0f : (float)ld_dbl(&S->alpha);
     const float beta = ld_int(&S->ni) ? 0.0f : (float)ld_dbl(&S->beta);
-    const XMixDir xf{rc, qc, pc, alpha, beta};
+    const XMixDir xf{rc, alpha, beta};
     auto row = [&](int i, const MixRow<F> &w) {
-      a.y[i] = w.y + xf.alpha * pc[i];
-      const float ri = xf.rnew(i);
-      const float pi = xf(i);
+      const float4 c = rc[i];
+      const float yi = c.w + xf.alpha * c.z;
+      const float ri = XMixDir::rnew(c, xf.alpha);
+      const float pi = ri + xf.beta * c.z;
       const float qi = w.mr.apply(a.A32, i, xf);
-      rn[i] = ri;
-      pn[i] = pi;
-      qn[i] = qi;
+      rn[i] = make_float4(ri, qi, pi, yi);
       const double rd = ri, qd = qi;
       v[0] += (double)pi * qd;
       v[1] += rd * qd;
@@ -276,10 +280,11 @@ __device__ __forceinline__ void mixed_rows(const MixedArgs &a, int ph, const KSt
     // complete the last step and fold the FP32 increment into the FP64
     // solution: x += S (y + alpha p)
     const int j = ld_int(&S->iter);
-    const float *pj = a.p + ((j & 1) ? a.ps : 0);
+    const float4 *rj = a.rec + ((j & 1) ? a.ps : 0);
     const float alpha = (float)ld_dbl(&S->alpha);
     for (int i = i0 + (int)threadIdx.x; i < i1; i += blockDim.x) {
-      const float yi = a.y[i] + alpha * pj[i];
+      const float4 c = rj[i];
+      const float yi = c.w + alpha * c.z;
       a.x[i] = a.x[i] + a.s[i] * (double)yi;
     }
   } else {

@@ -628,7 +628,7 @@ int run_mixed(DevState &d, const SolveIn &in, fvc_stats *stats, hipStream_t s) {
     }
     size_t bytes = align_up((kCtrWords + (size_t)G) * sizeof(unsigned)) + align_up(sizeof(KState)) +
                    align_up(kMaxK * (size_t)G * sizeof(double)) + align_up(nv * sizeof(float)) +
-                   align_up(n * sizeof(double)) + 8 * align_up(n * sizeof(float)) +
+                   align_up(n * sizeof(double)) + align_up(n * sizeof(float)) + 2 * align_up(n * sizeof(float4)) +
                    align_up(2 * sizeof(KState)) + align_up(2 * kMaxK * (size_t)G * sizeof(double)) + 4096;
     if ((rc = ensure_arena(d, bytes))) return rc;
     Carve cv{d.arena};
@@ -640,11 +640,8 @@ int run_mixed(DevState &d, const SolveIn &in, fvc_stats *stats, hipStream_t s) {
     double *sc = cv.take<double>(n);
     float *dsq = cv.take<float>(n);
     MixedArgs a{};
-    a.y = cv.take<float>(n);
-    a.ps = align_up(n * sizeof(float)) / sizeof(float);
-    a.r = cv.take<float>(2 * a.ps);
-    a.p = cv.take<float>(2 * a.ps);
-    a.q = cv.take<float>(2 * a.ps);
+    a.ps = align_up(n * sizeof(float4)) / sizeof(float4);
+    a.rec = cv.take<float4>(2 * a.ps);
     a.A = in.A;
     a.A32.neq = neq;
     a.A32.w = in.A.w;
@@ -1178,7 +1175,7 @@ int run_batch(DevState &d, int nsys, const std::vector<int> &row_off, int w, con
   size_t bytes = align_up(((size_t)wave * kCtrWords + cap) * sizeof(unsigned)) + align_up((size_t)wave * sizeof(KState)) +
                  align_up((size_t)kMaxK * cap * sizeof(double)) + align_up((size_t)wave * sizeof(SysDesc)) +
                  align_up((size_t)cap * sizeof(int)) + 4096;
-  if (mixed) bytes += 7 * align_up(n * sizeof(float)) + align_up(n * w * sizeof(float)) +
+  if (mixed) bytes += 2 * align_up(n * sizeof(float4)) + align_up(n * w * sizeof(float)) +
                       align_up(n * sizeof(double)) + align_up(n * sizeof(float));
   else bytes += 5 * align_up(n * sizeof(double));
   if ((rc = ensure_arena(d, bytes))) return rc;
@@ -1204,9 +1201,8 @@ int run_batch(DevState &d, int nsys, const std::vector<int> &row_off, int w, con
   B.delta = o.refresh_drop_e3 > 0 ? o.refresh_drop_e3 * 1e-3 : kRefreshDrop;
   if (B.delta >= 1.0) return fail(FVC_BAD_ARGUMENT, "refresh_drop_e3 %d must be below 1000", o.refresh_drop_e3);
   if (mixed) {
-    B.fs = align_up(n * sizeof(float)) / sizeof(float);
-    B.f[0] = cv.take<float>(n);
-    for (int k = 1; k < 4; ++k) B.f[k] = cv.take<float>(2 * B.fs);
+    B.fs = align_up(n * sizeof(float4)) / sizeof(float4);
+    B.rec = cv.take<float4>(2 * B.fs);
     B.av32 = cv.take<float>(n * w);
     B.s = cv.take<double>(n);
     B.dsq = cv.take<float>(n);
