@@ -287,17 +287,44 @@ int read_state(DevState &d, KState *st, hipStream_t s) {
 void fill_stats(fvc_stats *stats, const KState &hs, int code, bool persistent, float ms, int G, int bs);
 int finish(const KState &hs, unsigned *ctr);
 
+// The size of the next batch: doubling up to kMaxBatch, capped near the end
+// of the solve by a convergence-rate model.  From the first observed state
+// (it0, l0) and the latest one, the residual measure's geometric rate per
+// iteration predicts the iterations still needed; the batch covers them, less
+// those already enqueued, plus one (at least one, so that the GPU always has
+// a batch while the host waits).  Without a usable rate it just doubles; a
+// wrong prediction costs host round trips, never results.
+int next_batch(const KState &h, int launched, int batch, int &it0, double &l0) {
+  constexpr int kMaxBatch = 8;
+  const int nb = std::min(batch * 2, kMaxBatch);
+  const double l = h.last, c = h.conv;
+  if (!(l > 0.0) || !(c > 0.0) || !std::isfinite(l) || !std::isfinite(c)) return nb;
+  if (it0 < 0) {
+    it0 = h.iter;
+    l0 = l;
+    return nb;
+  }
+  if (h.iter <= it0 || !(l < l0)) return nb;
+  const double rate = std::pow(l / l0, 1.0 / (h.iter - it0));
+  const double remaining = std::log(c / l) / std::log(rate);
+  if (!std::isfinite(remaining)) return nb;
+  const double need = remaining - (launched - h.iter);
+  const int want = need <= 0.0 ? 1 : (int)std::ceil(need) + 1;
+  return std::max(1, std::min(nb, want));
+}
+
 // Kernel-per-phase driver: enqueues iterations in batches and checks for
 // convergence one batch behind, so the GPU never idles on the host.  After
 // each batch the latest state goes to pinned host memory behind an event;
 // the host then waits only for the previous batch's event, while the GPU
 // runs the current one.  Launches after convergence find `done` set and
-// return at once (at most one batch of them).  `iteration()` enqueues one
-// iteration; `state()` is the device state the last launch wrote.
+// return at once (at most one batch of them, kept short by next_batch).
+// `iteration()` enqueues one iteration; `state()` is the device state the
+// last launch wrote.
 template <class Iter, class State>
 int drive_batches(DevState &d, hipStream_t s, int maxit, Iter &&iteration, State &&state) {
-  constexpr int kMaxBatch = 8;
-  int launched = 0, batch = 2, k = 0;
+  int launched = 0, batch = 2, k = 0, it0 = -1;
+  double l0 = 0.0;
   bool waiting[2] = {false, false};
   while (launched < maxit) {
     const int nb = std::min(batch, maxit - launched);
@@ -310,13 +337,15 @@ int drive_batches(DevState &d, hipStream_t s, int maxit, Iter &&iteration, State
     HIPCHK(hipMemcpyAsync(d.host_ring + slot, state(), sizeof(KState), hipMemcpyDeviceToHost, s));
     HIPCHK(hipEventRecord(d.evq[slot], s));
     waiting[slot] = true;
+    ++k;
     if (waiting[prev]) {
       HIPCHK(hipEventSynchronize(d.evq[prev]));
       waiting[prev] = false;
       if (d.host_ring[prev].done) break;
+      batch = next_batch(d.host_ring[prev], launched, batch, it0, l0);
+    } else {
+      batch = std::min(batch * 2, 8);
     }
-    ++k;
-    batch = std::min(batch * 2, kMaxBatch);
   }
   return FVC_OK;
 }
