@@ -176,6 +176,28 @@ def tri_mesh(n: int, jitter: float = 0.0, seed: int = SEED_BASE) -> Mesh:
     return Mesh(2 * n * n, 3, q.nodes, cell_nodes, nbr, tag)
 
 
+def ref_mesh_arrays(mesh: Mesh, wall_bc=(3, 6, 1, 3)) -> dict:
+    """The mesh in the reference's arrays, as C arrays with the first Fortran
+    index fastest (src/Rdata_mvf.f, src/Adjacency.f): x(2, nnode) ->
+    (nnode, 2); ix(nen+1, numel) -> (numel, nen+1), 1-based nodes then the
+    material (1); nelcon(nen, numel) -> (numel, nen), 1-based neighbours, -1
+    across a wall; pedge(nen+1, numel) -> (numel, nen+1), the boundary
+    condition of each wall edge by wall (S, E, N, W): 3 prescribed pressure,
+    6 zero gradient, anything else extrapolated (src/CellibPressure.f:
+    443-458).  cell_type is the material's cell type: 65 (quad) or 63
+    (triangle), the SIMPLE cells of src/Celllib.f:127-138."""
+    w = mesh.w
+    ix = np.empty((mesh.neq, w + 1), dtype=np.int32)
+    ix[:, :w] = mesh.cell_nodes + 1
+    ix[:, w] = 1
+    nelcon = np.where(mesh.nbr >= 0, mesh.nbr + 1, -1).astype(np.int32)
+    pedge = np.zeros((mesh.neq, w + 1), dtype=np.int32)
+    bc = np.asarray(wall_bc, dtype=np.int32)
+    pedge[:, :w] = np.where(mesh.boundary_tag >= 0, bc[np.maximum(mesh.boundary_tag, 0)], 0)
+    return dict(x=np.ascontiguousarray(mesh.nodes, dtype=np.float64), ix=ix, nelcon=nelcon, pedge=pedge,
+                cell_type=65 if w == 4 else 63)
+
+
 # ------------------------------------------------------------------ systems --
 
 @dataclass
