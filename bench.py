@@ -140,6 +140,59 @@ def cpu_baseline(fmt, method, tol, budget_s=10.0, threads=None):
             "sample": f"1 full {method} solve, sequential C restatement", "seconds_per_solve": dt}
 
 
+def gform_bytes(numel: int, nnode: int, nen: int = 4) -> int:
+    """Algorithmic bytes of the pressure-correction gradient: per cell ix
+    (nen+1 int32), nelcon (nen), pedge (nen+1), u, the old and new gradient
+    and fluxl; each node's coordinates once (neighbour gathers are cache
+    hits)."""
+    return numel * (4 * (3 * nen + 2) + 8 + 32 + 8) + nnode * 16
+
+
+def bench_post(F, n: int, reps: int = 50) -> dict:
+    """fvc_gform_dev and fvc_simple_update_dev on the n x n jittered quad mesh
+    (HIP events on the launch stream)."""
+    import ctypes as C
+    from fortranfvcell_amd import problems as P
+    from fortranfvcell_amd.post import DeviceMesh, simple_update
+    m = P.quad_mesh(n, jitter=0.1, seed=P.SEED_BASE + 6)
+    a = P.ref_mesh_arrays(m)
+    dm = DeviceMesh(a["x"], a["ix"], a["nelcon"], a["pedge"])
+    rng = np.random.default_rng(1)
+    u = F.DeviceArray.from_host(rng.normal(size=m.neq))
+    g = F.DeviceArray.from_host(np.zeros((m.neq, 2)))
+    fl = F.DeviceArray.empty(m.neq)
+    L = F.load()
+    e0, e1 = C.c_void_p(), C.c_void_p()
+    L.fvc_event_create(C.byref(e0))
+    L.fvc_event_create(C.byref(e1))
+    dm.gform(u, g, fl)
+    L.fvc_synchronize()
+    L.fvc_event_record(e0, None)
+    for _ in range(reps):
+        dm.gform(u, g, fl)
+    L.fvc_event_record(e1, None)
+    ms = C.c_float()
+    L.fvc_event_elapsed_ms(e0, e1, C.byref(ms))
+    t_g = ms.value * 1e-3 / reps
+    vecs = [F.DeviceArray.from_host(rng.normal(size=s)) for s in ((m.neq, 2), m.neq, m.neq, m.neq, m.neq,
+                                                                   (m.neq, 2), (m.neq, 2))]
+    w, pre, u1, u2, pc, gp, d = vecs
+    simple_update(m.neq, w, pre, u1, u2, pc, gp, d, 0.7, 0.3, False)
+    L.fvc_synchronize()
+    L.fvc_event_record(e0, None)
+    for _ in range(reps):
+        simple_update(m.neq, w, pre, u1, u2, pc, gp, d, 0.7, 0.3, False)
+    L.fvc_event_record(e1, None)
+    L.fvc_event_elapsed_ms(e0, e1, C.byref(ms))
+    t_s = ms.value * 1e-3 / reps
+    L.fvc_event_destroy(e0)
+    L.fvc_event_destroy(e1)
+    gb = gform_bytes(m.neq, len(a["x"]))
+    sb = m.neq * (16 + 8 + 8 + 8 + 8 + 16 + 16 + 8)     # w out, pre r/w, u1, u2, preC, gradPc, d
+    return {"cells": m.neq, "gform_us": t_g * 1e6, "gform_GBs": gb / t_g / 1e9, "gform_bytes": gb,
+            "simple_update_us": t_s * 1e6, "simple_update_GBs": sb / t_s / 1e9}
+
+
 def calibrate_copy(mib: int, reps: int) -> None:
     """A plain device copy of a known size: rocprofv3's FETCH_SIZE and
     WRITE_SIZE for it give the scale of those counters (bytes each reads per
@@ -372,6 +425,10 @@ def main():
         bres["one_at_a_time_fp64"] = {"ms": single * 1e3, "solves_per_s": len(fmts) / single}
         extra["batch256"] = bres
         log("batch", bres)
+        # the post-solve step on the 1M-cell quad mesh: the pressure-correction
+        # gradient (gform) and simpleUpdate, on resident arrays
+        extra["post_solve"] = bench_post(F, n)
+        log("post", extra["post_solve"])
         # the drop-in as the Fortran program calls it: host arrays in, x out
         # (H2D of the matrix and vectors and D2H of x inside every solve)
         F.solve(ell, F.BICGSTAB, tol=tol, **opts)

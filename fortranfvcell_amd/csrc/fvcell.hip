@@ -17,6 +17,7 @@
 #include "fvc_mixed.h"
 #include "fvc_batch.h"
 #include "fvc_adjoint.h"
+#include "fvc_post.h"
 
 using namespace fvc;
 
@@ -1441,6 +1442,37 @@ int fvc_assemble_heat_dev(int32_t neq, int32_t w, const int32_t *ja, const doubl
     return fail(FVC_BAD_ARGUMENT, "bad assembly arguments");
   hipLaunchKernelGGL(assemble_heat_kernel, dim3((neq + 255) / 256), dim3(256), 0, (hipStream_t)stream, neq, w,
                      ja, k, q, geo, alpha, conv, conv_f, wall_g, wall_t, ad, al, b);
+  HIPCHK(hipGetLastError());
+  return FVC_OK;
+}
+
+int fvc_gform_dev(int32_t numel, int32_t nnode, int32_t nen, const double *x, const int32_t *ix,
+                  const int32_t *nelcon, const int32_t *pedge, const double *u, double *grad,
+                  double *fluxl, void *stream) {
+  if (numel <= 0 || nnode <= 0 || !x || !ix || !nelcon || !pedge || !u || !grad || !fluxl)
+    return fail(FVC_BAD_ARGUMENT, "bad gform arguments");
+  const dim3 grid((numel + 255) / 256), block(256);
+  if (nen == 4)
+    hipLaunchKernelGGL(gform_p_kernel<4>, grid, block, 0, (hipStream_t)stream, numel, nnode, x, ix, nelcon,
+                       pedge, u, grad, fluxl);
+  else if (nen == 3)
+    hipLaunchKernelGGL(gform_p_kernel<3>, grid, block, 0, (hipStream_t)stream, numel, nnode, x, ix, nelcon,
+                       pedge, u, grad, fluxl);
+  else
+    return fail(FVC_UNSUPPORTED, "gform: cells of %d edges (triangles and quads only)", nen);
+  HIPCHK(hipGetLastError());
+  return FVC_OK;
+}
+
+int fvc_simple_update_dev(int32_t numel, int32_t ndm, double *w, double *pre, const double *u1,
+                          const double *u2, const double *prec, const double *gradpc, const double *d,
+                          double under_u, double under_pc, int32_t simple_r, void *stream) {
+  (void)under_u;
+  if (numel <= 0 || !w || !pre || !u1 || !u2 || !prec || !gradpc || !d)
+    return fail(FVC_BAD_ARGUMENT, "bad simpleUpdate arguments");
+  if (ndm != 2) return fail(FVC_UNSUPPORTED, "simpleUpdate: ndm = %d (2-D only)", ndm);
+  hipLaunchKernelGGL(simple_update_kernel, dim3((numel + 255) / 256), dim3(256), 0, (hipStream_t)stream, numel,
+                     w, pre, u1, u2, prec, gradpc, d, under_pc, simple_r ? 1 : 0);
   HIPCHK(hipGetLastError());
   return FVC_OK;
 }

@@ -227,6 +227,36 @@ int fvc_assemble_heat_dev(int32_t neq, int32_t w, const int32_t *ja, const doubl
                           double *ad, double *al, double *b, void *stream);
 
 /* ----------------------------------------------------------------------------
+ * The post-solve step of a SIMPLE iteration on device memory, so that p', its
+ * gradient and the corrected velocities stay in HBM after the solve.
+ *
+ * fvc_gform_dev replaces
+ *   call gform(pC,gradPc,fluxlPc,x,sedge,e,ls,ie,nelcon,pedge,ix,
+ *              numel,ndm,nen,nshared,1,ndf,2,2,2)
+ * of src/Simple.f:601,635 (src/Pform.f:466-676) for the cells that take
+ * cell_si_p_i_ggl (types 63/65/73/75, src/Celllib.f:127-157): the Green-Gauss
+ * gradient over the neighbours' centres, with ghost cells across walls
+ * (pedge 3: 0, 6: the cell's value, otherwise extrapolated with the cell's
+ * old gradient; src/CellibPressure.f:432-463,611-650).  Arrays are the
+ * reference's: x(2,nnode), ix(nen+1,numel), nelcon(nen,numel) (negative
+ * across a wall), pedge(nen+1,numel), u(numel); grad(2,numel) holds the old
+ * gradient on entry and the new one on return; fluxl(numel) = 1.  nen = 3 or
+ * 4 (nshared = nen).  Bit-identical to the reference.  Indices are checked on
+ * the device: a cell with one out of range gets NaN. */
+int fvc_gform_dev(int32_t numel, int32_t nnode, int32_t nen, const double *x, const int32_t *ix,
+                  const int32_t *nelcon, const int32_t *pedge, const double *u, double *grad,
+                  double *fluxl, void *stream);
+
+/* fvc_simple_update_dev replaces
+ *   subroutine simpleUpdate(w,pre,u1,u2,preC,gradPc,d,numel,ndm,underU,underPc,simpleR)
+ * of src/Simple.f:871-898 (called at :642): w(:,i) = (u1(i),u2(i)) - d(:,i)*gradPc(:,i); and
+ * unless simpleR (SIMPLEC), pre(i) = pre(i) + underPc*preC(i).  w, d and
+ * gradPc are (2,numel); ndm must be 2.  Bit-identical to the reference. */
+int fvc_simple_update_dev(int32_t numel, int32_t ndm, double *w, double *pre, const double *u1,
+                          const double *u2, const double *prec, const double *gradpc, const double *d,
+                          double under_u, double under_pc, int32_t simple_r, void *stream);
+
+/* ----------------------------------------------------------------------------
  * SpMV y = A x.  Replaces the matvec_* routines of src/Matvec.f:21-300 and
  * src/Matvec_omp.f (matvec_csrd[_ilu2]_omp, matvec_csrd_sym_omp family,
  * matvec_ellpack_omp).  Each row is summed in the reference's sequential
