@@ -242,3 +242,24 @@ def test_pcg_csr_full_storage(fvc, orc):
                       oracle._p(b), oracle._p(x), 1e-10, 20000, 0, 1, C.byref(st))
     xg, sg = fvc.solve(f, 1)
     assert sg["status"] == 0 and _rel(xg, x) <= X_TOL
+
+
+@pytest.mark.parametrize("code,prec", [(1, 0), (2, 0), (1, 1)], ids=["pcg", "bicgstab", "pcg_mixed"])
+def test_more_partials_than_loaded_ahead(fvc, code, prec):
+    """A kernel-per-phase launch loads each thread's terms of the previous
+    phase's partial sums ahead, up to four; with more blocks than four times
+    the block size it sums them in a loop instead.  That path must converge
+    like the other and repeat bit for bit."""
+    s = P.config_poisson(384) if code == 1 else P.config_advdiff(384, peclet_max=60.0)
+    f = P.to_ell(s)
+    print(f"many blocks code={code} precision={prec} neq={s.neq}", flush=True)
+    opts = dict(tol=1e-10, maxit=20000, stop_rule=fvc.STOP_RELRES, x0_rule=fvc.X0_ZERO, precision=prec,
+                engine=fvc.ENGINE_KERNELS, block_size=256)
+    xa, sa = fvc.solve(f, code, grid_blocks=2048, **opts)
+    xb, sb = fvc.solve(f, code, grid_blocks=2048, **opts)
+    xc, sc = fvc.solve(f, code, grid_blocks=1024, **opts)
+    assert sa["grid_blocks"] == 2048 and sc["grid_blocks"] == 1024
+    assert sa["status"] == sc["status"] == 0
+    assert np.array_equal(xa, xb) and sa["iterations"] == sb["iterations"]
+    assert sa["true_relres"] <= 1e-10 and sc["true_relres"] <= 1e-10
+    assert _iters_close(sa["iterations"], sc["iterations"])
