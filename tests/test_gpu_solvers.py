@@ -98,11 +98,14 @@ def test_recurrences_vs_oracle(fvc, orc, case, rec):
     assert abs(st["energy"] - so["energy"]) <= 1e-8 * abs(so["energy"])
 
 
-@pytest.mark.parametrize("rec", [0, 1], ids=["fused", "reference_order"])
-@pytest.mark.parametrize("code,lay", [(1, "ell"), (1, "csrd"), (2, "ell"), (2, "csrd"), (3, "ell"), (3, "csrd")])
+# GMRES has a single recurrence
+ENGINE_CASES = [(code, lay, rec) for code in (1, 2, 3) for lay in ("ell", "csrd") for rec in (0, 1)
+                if not (code == 3 and rec == 1)]
+
+
+@pytest.mark.parametrize("code,lay,rec", ENGINE_CASES,
+                         ids=[f"{c}-{l}-{'fused' if r == 0 else 'reference_order'}" for c, l, r in ENGINE_CASES])
 def test_engines_bit_identical(fvc, code, lay, rec):
-    if code == 3 and rec == 1:
-        pytest.skip("GMRES has a single recurrence")
     s = P.config_poisson(40) if code == 1 else P.config_advdiff(40, peclet_max=60.0)
     f = LAYOUT[lay](s)
     print(f"engines code={code} {lay} recurrence={rec}", flush=True)
