@@ -65,7 +65,7 @@ def test_oracle_replays_1024_bicgstab_statistics(orc):
 
 
 # --------------------------------------------- the hot-path configurations --
-CONFIGS = np.load(os.path.join(HERE, "ref_configs.npz"), allow_pickle=False)
+REF_FIXTURES = np.load(os.path.join(HERE, "ref_configs.npz"), allow_pickle=False)
 
 
 def config_system(key):
@@ -85,40 +85,40 @@ def config_system(key):
     raise KeyError(key)
 
 
-CONFIG_KEYS = [str(k) for k in CONFIGS["keys"]]
+CONFIG_KEYS = [str(k) for k in REF_FIXTURES["keys"]]
 
 
 @pytest.mark.parametrize("key", CONFIG_KEYS)
 def test_oracle_replays_config_fixture(orc, key):
     i = CONFIG_KEYS.index(key)
     s, f = config_system(key)
-    tol = float(CONFIGS["tol"][i])
+    tol = float(REF_FIXTURES["tol"][i])
     if key.endswith("gmres30"):
         x, _, st = orc.krylov("gmres", f, k=30, tol=tol)
     else:
         code = 1 if key.endswith("pcg") else 2
         x, _, st = orc.solver(f, code, unsym=True, omp=False, tol=tol)
-    assert np.array_equal(x, CONFIGS[key + "_x"])
-    assert st["iterations"] == CONFIGS["iterations"][i]
-    assert abs(st["energy"] - CONFIGS["energy"][i]) <= 1e-9 * abs(CONFIGS["energy"][i])
+    assert np.array_equal(x, REF_FIXTURES[key + "_x"])
+    assert st["iterations"] == REF_FIXTURES["iterations"][i]
+    assert abs(st["energy"] - REF_FIXTURES["energy"][i]) <= 1e-9 * abs(REF_FIXTURES["energy"][i])
     # the reference stopped where its own rule reached true relative residual 1e-10
     assert np.linalg.norm(s.b - P.scipy_csr(s) @ x) <= 1e-10 * np.linalg.norm(s.b)
 
 
 def test_oracle_replays_adjoint_fixture(orc):
     op, k, subset = P.heat_problem(32, subset_frac=0.05)
-    assert np.array_equal(subset, CONFIGS["c5_subset"])
+    assert np.array_equal(subset, REF_FIXTURES["c5_subset"])
     s = op.assemble(k)
     T, _, _ = orc.krylov("bicgstab", P.to_ell(s), tol=1e-15)
-    assert np.array_equal(T, CONFIGS["c5_T"])
+    assert np.array_equal(T, REF_FIXTURES["c5_T"])
     g = np.zeros(s.neq)
     g[subset] = 2.0 * T[subset] / len(subset)
     lam, _, _ = orc.krylov("bicgstab", P.to_csrd(P.transpose_system(s)), b=g, tol=1e-15)
-    assert np.array_equal(lam, CONFIGS["c5_lam"])
+    assert np.array_equal(lam, REF_FIXTURES["c5_lam"])
     gk = op.grad_k(k, T, lam)
-    assert np.array_equal(gk, CONFIGS["c5_gk"])
-    cells = CONFIGS["c5_cells"]
-    assert np.all(np.abs(CONFIGS["c5_fd"] - gk[cells]) <= 1e-5 * np.abs(CONFIGS["c5_fd"]))
+    assert np.array_equal(gk, REF_FIXTURES["c5_gk"])
+    cells = REF_FIXTURES["c5_cells"]
+    assert np.all(np.abs(REF_FIXTURES["c5_fd"] - gk[cells]) <= 1e-5 * np.abs(REF_FIXTURES["c5_fd"]))
 
 
 @pytest.mark.parametrize("name", ["advdiff1024_ell_gmres30", "advdiff1024_csrd_bicgstab"])
@@ -128,7 +128,7 @@ def test_oracle_replays_1024_config_statistics(orc, name):
         x, _, st = orc.krylov("gmres", P.to_ell(s), k=30, tol=1e-10)
     else:
         x, _, st = orc.solver(P.to_csrd(s), 2, omp=False, tol=1e-10)
-    assert st["iterations"] == CONFIGS[name + "_iterations"]
-    assert abs(st["energy"] - CONFIGS[name + "_energy"]) <= 1e-9 * abs(CONFIGS[name + "_energy"])
-    assert np.array_equal(x[CONFIGS[name + "_idx"]], CONFIGS[name + "_xsample"])
-    assert x.sum() == CONFIGS[name + "_xsum"]
+    assert st["iterations"] == REF_FIXTURES[name + "_iterations"]
+    assert abs(st["energy"] - REF_FIXTURES[name + "_energy"]) <= 1e-9 * abs(REF_FIXTURES[name + "_energy"])
+    assert np.array_equal(x[REF_FIXTURES[name + "_idx"]], REF_FIXTURES[name + "_xsample"])
+    assert x.sum() == REF_FIXTURES[name + "_xsum"]

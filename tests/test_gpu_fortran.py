@@ -39,7 +39,7 @@ def _run(tmp_path, f, code, unsym, omp, tol=1e-10, maxit=20000, x0=None):
     u = np.frombuffer(raw[8 * neq + 16: 16 * neq + 16], np.float64)
     timers = np.frombuffer(raw[16 * neq + 16: 16 * neq + 48], np.float64)
     _run.timers = dict(zip(("time", "matvectime", "dottime", "precontime"), timers))
-    # the driver's second solve: solverKeepMatrix, then the same A with 2 b
+    # the Fortran caller's second solve: solverKeepMatrix, then the same A with 2 b
     _run.x_kept = np.frombuffer(raw[16 * neq + 48: 24 * neq + 48], np.float64)
     return x, int(status), int(iters), energy, u, open(flog).read(), r.stdout
 

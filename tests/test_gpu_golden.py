@@ -9,7 +9,7 @@ import numpy as np
 import pytest
 
 from fortranfvcell_amd import problems as P
-from test_golden import CONFIG_KEYS, CONFIGS, config_system
+from test_golden import CONFIG_KEYS, REF_FIXTURES, config_system
 
 pytestmark = pytest.mark.gpu
 
@@ -22,17 +22,17 @@ def _rel(x, y):
 def test_gpu_matches_reference_config(fvc, key):
     i = CONFIG_KEYS.index(key)
     s, f = config_system(key)
-    tol = float(CONFIGS["tol"][i])
+    tol = float(REF_FIXTURES["tol"][i])
     print(f"config {key} neq={s.neq} tol={tol:g}", flush=True)
     if key.endswith("gmres30"):
         x, st = fvc.solve(f, fvc.GMRES, tol=tol, gmres_k=30)
     else:
         x, st = fvc.solve(f, 1 if key.endswith("pcg") else 2, tol=tol, x0_rule=fvc.X0_ZERO)
     assert st["status"] == 0
-    assert _rel(x, CONFIGS[key + "_x"]) <= 1e-8, _rel(x, CONFIGS[key + "_x"])
-    it_ref = int(CONFIGS["iterations"][i])
+    assert _rel(x, REF_FIXTURES[key + "_x"]) <= 1e-8, _rel(x, REF_FIXTURES[key + "_x"])
+    it_ref = int(REF_FIXTURES["iterations"][i])
     assert abs(st["iterations"] - it_ref) <= max(2, int(0.03 * it_ref)), (st["iterations"], it_ref)
-    assert abs(st["energy"] - CONFIGS["energy"][i]) <= 1e-8 * abs(CONFIGS["energy"][i])
+    assert abs(st["energy"] - REF_FIXTURES["energy"][i]) <= 1e-8 * abs(REF_FIXTURES["energy"][i])
 
 
 @pytest.mark.parametrize("precision", [0, 1], ids=["fp64", "mixed"])
@@ -41,7 +41,7 @@ def test_gpu_batch_matches_reference_config(fvc, precision):
     systems = [config_system(k) for k in keys]
     batch = fvc.concat_ell([f for _, f in systems])
     ro = batch["row_offsets"]
-    tol = float(CONFIGS["tol"][CONFIG_KEYS.index(keys[0])])
+    tol = float(REF_FIXTURES["tol"][CONFIG_KEYS.index(keys[0])])
     if precision == 0:
         # FP64: the reference's own stopping rule at the fixture's tolerance
         x, sts = fvc.solve_batch(batch, tol=tol, stop_rule=fvc.STOP_REFERENCE)
@@ -57,7 +57,7 @@ def test_gpu_batch_matches_reference_config(fvc, precision):
         xi = x[ro[i]:ro[i + 1]]
         s = systems[i][0]
         assert np.linalg.norm(s.b - P.scipy_csr(s) @ xi) <= 1.01e-10 * np.linalg.norm(s.b)
-        err = _rel(xi, CONFIGS[k + "_x"])
+        err = _rel(xi, REF_FIXTURES[k + "_x"])
         print(f"  {k}: |x - x_ref|/|x_ref| = {err:.2e}", flush=True)
         assert err <= bar, (k, err)
 
@@ -67,11 +67,11 @@ def test_gpu_adjoint_matches_reference_config(fvc):
     op, k, subset = P.heat_problem(32, subset_frac=0.05)
     solver = HeatSolver(op, tol=1e-15)
     L0, gk, lam, T = loss_and_grad(solver, k, subset)
-    assert _rel(T, CONFIGS["c5_T"]) <= 1e-8
-    assert _rel(lam, CONFIGS["c5_lam"]) <= 1e-8
-    assert _rel(gk, CONFIGS["c5_gk"]) <= 1e-8
-    cells = CONFIGS["c5_cells"]
-    fd = CONFIGS["c5_fd"]
+    assert _rel(T, REF_FIXTURES["c5_T"]) <= 1e-8
+    assert _rel(lam, REF_FIXTURES["c5_lam"]) <= 1e-8
+    assert _rel(gk, REF_FIXTURES["c5_gk"]) <= 1e-8
+    cells = REF_FIXTURES["c5_cells"]
+    fd = REF_FIXTURES["c5_fd"]
     assert np.all(np.abs(gk[cells] - fd) <= 1e-5 * np.abs(fd))
 
 
@@ -83,13 +83,13 @@ def test_gpu_1024_config_statistics(fvc, name):
         x, st = fvc.solve(P.to_ell(s), fvc.GMRES, tol=1e-10, gmres_k=30)
     else:
         x, st = fvc.solve(P.to_csrd(s), fvc.BICGSTAB, tol=1e-10, x0_rule=fvc.X0_ZERO)
-    it_ref = int(CONFIGS[name + "_iterations"])
+    it_ref = int(REF_FIXTURES[name + "_iterations"])
     print(f"  gpu {st['iterations']} it, reference {it_ref}, {st['seconds'] * 1e3:.1f} ms", flush=True)
     assert st["status"] == 0
     assert abs(st["iterations"] - it_ref) <= max(1, int(0.03 * it_ref))
-    e_ref = float(CONFIGS[name + "_energy"])
+    e_ref = float(REF_FIXTURES[name + "_energy"])
     assert abs(st["energy"] - e_ref) <= 1e-8 * abs(e_ref)
-    xsq = float(CONFIGS[name + "_xsq"])
+    xsq = float(REF_FIXTURES[name + "_xsq"])
     assert abs((x * x).sum() - xsq) <= 1e-8 * xsq
-    idx = CONFIGS[name + "_idx"]
-    assert np.max(np.abs(x[idx] - CONFIGS[name + "_xsample"])) <= 1e-7 * np.sqrt(xsq / s.neq)
+    idx = REF_FIXTURES[name + "_idx"]
+    assert np.max(np.abs(x[idx] - REF_FIXTURES[name + "_xsample"])) <= 1e-7 * np.sqrt(xsq / s.neq)
