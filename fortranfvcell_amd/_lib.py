@@ -41,7 +41,7 @@ class Options(C.Structure):
                 ("true_residual", C.c_int32), ("engine", C.c_int32), ("block_size", C.c_int32),
                 ("grid_blocks", C.c_int32), ("precision", C.c_int32), ("batch_work_exp10", C.c_int32),
                 ("barrier", C.c_int32), ("recurrence", C.c_int32), ("refresh_every", C.c_int32),
-                ("refresh_drop_e3", C.c_int32), ("reserved", C.c_int32 * 3)]
+                ("refresh_drop_e3", C.c_int32), ("row_order", C.c_int32), ("reserved", C.c_int32 * 2)]
 
 
 class Stats(C.Structure):
@@ -125,6 +125,9 @@ def check(rc: int, allow=(FVC_OK,)) -> int:
 def default_options(**kw) -> Options:
     o = Options()
     load().fvc_default_options(C.byref(o))
+    names = {f for f, _ in Options._fields_ if f != "reserved"}
     for k, v in kw.items():
+        if k not in names:
+            raise TypeError(f"unknown solver option {k!r}")
         setattr(o, k, v)
     return o

@@ -50,7 +50,8 @@ struct BatchArgs {
 
 __device__ __forceinline__ void batch_grid(const BatchArgs &B, int sys, const SysDesc &D, Grid &g) {
   g.G = D.g_count;
-  g.R = (D.neq + D.g_count - 1) / D.g_count;
+  g.tiles = 0;
+  g.R = grid_rows(D.neq, D.g_count, 0);
   g.neq = D.neq;
   g.ctr = B.ctr + (size_t)sys * kCtrWords;
   g.partials = B.partials + (size_t)kMaxK * D.g_first;

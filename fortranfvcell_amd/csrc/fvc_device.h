@@ -300,8 +300,9 @@ __device__ __forceinline__ void fence_acq_rel() { __builtin_amdgcn_fence(__ATOMI
 
 struct Grid {
   int G;                 // blocks = row chunks
-  int R;                 // rows per chunk
+  int R;                 // rows per chunk (tiles: rows per group of blocks)
   int neq;
+  int tiles;             // row order: 0 contiguous chunk per block, 1 XCD tiles
   unsigned *ctr;
   double *partials;      // kMaxK * G
   bool persistent;
@@ -317,12 +318,36 @@ struct Grid {
     const int b = bid();
     return ((G & 7) == 0) ? (b & 7) * (G >> 3) + (b >> 3) : b;
   }
-  __device__ __forceinline__ int row_begin() const { return chunk() * R; }
+  // This thread's rows are first(), first() + stride(), ... below row_end().
+  // Chunks: block b's thread t takes chunk() R + t, + bs, ...  A block then
+  // meets a row's neighbours one grid row (nx rows) later in its own sweep,
+  // which stays in L2 only while nx is a few block sizes.
+  // Tiles: the 8 groups of blocks (one per XCD) own [x R, (x+1) R) each,
+  // and a group's G/8 blocks sweep it together, bs rows per block: block j
+  // of group x takes x R + j bs + t, + (G/8) bs, ...  The group's rows in
+  // flight are then (G/8) bs consecutive ones, so the neighbours nx rows
+  // away are read by the XCD at about the same time at any nx below that.
+  __device__ __forceinline__ int ngroups() const { return ((G & 7) == 0) ? 8 : 1; }
+  __device__ __forceinline__ int first() const {
+    const int t = (int)threadIdx.x;
+    if (!tiles) return chunk() * R + t;
+    const int b = bid(), ng = ngroups();
+    return (b % ng) * R + (b / ng) * (int)blockDim.x + t;
+  }
+  __device__ __forceinline__ int stride() const {
+    return tiles ? (G / ngroups()) * (int)blockDim.x : (int)blockDim.x;
+  }
   __device__ __forceinline__ int row_end() const {
-    const long e = (long)(chunk() + 1) * R;
+    const long e = (long)((tiles ? bid() % ngroups() : chunk()) + 1) * R;
     return e < neq ? (int)e : neq;
   }
 };
+
+// Rows per chunk (per group of blocks with tiles) for a grid of G blocks
+__host__ __device__ inline int grid_rows(int neq, int G, int tiles) {
+  const int parts = tiles ? (((G & 7) == 0) ? 8 : 1) : G;
+  return (int)(((long)neq + parts - 1) / parts);
+}
 
 __device__ __forceinline__ int group_size(int G, int g) { return (G - g + 7) / 8; }
 

@@ -168,12 +168,12 @@ struct PcgArgs {
 // src/Iterativos.f:29-106 / src/Iterativos_omp.f:30-127.
 template <int F>
 __device__ bool pcg_phase(const PcgArgs &a, int ph, double *lds) {
-  const int i0 = a.g.row_begin(), i1 = a.g.row_end();
+  const int i0 = a.g.first(), i1 = a.g.row_end(), is = a.g.stride();
   KState *st = a.st;
   if (ph == PCG_SETUP) {
     // z = A x; r = b - z; z = r/m; p = z; d = r.z; conv = tol*sqrt|d|
     double v[3] = {0.0, 0.0, 0.0};
-    for (int i = i0 + (int)threadIdx.x; i < i1; i += blockDim.x) {
+    for (int i = i0; i < i1; i += is) {
       double ax;
       if (a.x0_zero) {
         a.x[i] = 0.0;
@@ -218,7 +218,7 @@ __device__ bool pcg_phase(const PcgArgs &a, int ph, double *lds) {
     const double *po = (j & 1) ? a.p0 : a.p1;
     double v[1] = {0.0};
     if (j == 1) {
-      for (int i = i0 + (int)threadIdx.x; i < i1; i += blockDim.x) {
+      for (int i = i0; i < i1; i += is) {
         const double qi = row_apply<F>(a.A, i, XVec{a.z});
         const double pi = a.z[i];
         pn[i] = pi;
@@ -227,7 +227,7 @@ __device__ bool pcg_phase(const PcgArgs &a, int ph, double *lds) {
       }
     } else {
       const XPcgDir xf{a.z, po, ld_dbl(&st->beta)};
-      for (int i = i0 + (int)threadIdx.x; i < i1; i += blockDim.x) {
+      for (int i = i0; i < i1; i += is) {
         const double qi = row_apply<F>(a.A, i, xf);
         const double pi = xf(i);
         pn[i] = pi;
@@ -248,7 +248,7 @@ __device__ bool pcg_phase(const PcgArgs &a, int ph, double *lds) {
     const double *pn = (j & 1) ? a.p1 : a.p0;
     const double alpha = ld_dbl(&st->alpha);
     double v[2] = {0.0, 0.0};
-    for (int i = i0 + (int)threadIdx.x; i < i1; i += blockDim.x) {
+    for (int i = i0; i < i1; i += is) {
       const double xi = a.x[i] + alpha * pn[i];
       const double ri = a.r[i] - alpha * a.q[i];
       const double zi = ri / a.m[i];
@@ -280,7 +280,7 @@ __device__ bool pcg_phase(const PcgArgs &a, int ph, double *lds) {
   }
   // PCG_FINAL: energy norm x.Ax (src/Iterativos.f:82-83) and ||b - Ax||^2
   double v[2] = {0.0, 0.0};
-  for (int i = i0 + (int)threadIdx.x; i < i1; i += blockDim.x) {
+  for (int i = i0; i < i1; i += is) {
     const double ax = row_apply<F>(a.A, i, XVec{a.x});
     const double xi = a.x[i];
     const double ri = a.b[i] - ax;
@@ -383,10 +383,10 @@ __device__ __forceinline__ Cg1Row<F> cg1_load(const Cg1Args &a, int i) {
 template <int F>
 __device__ void cg1_rows(const Cg1Args &a, int ph, const KState *S, double *v,
                          const Cg1Row<F> *pre = nullptr) {
-  const int i0 = a.g.row_begin(), i1 = a.g.row_end();
+  const int i0 = a.g.first(), i1 = a.g.row_end(), is = a.g.stride();
   if (ph == CG1_SETUP) {
     // r = b - A x (or r = b for x0 = 0); d = r.M^-1 r; ||b||^2, ||r||^2
-    for (int i = i0 + (int)threadIdx.x; i < i1; i += blockDim.x) {
+    for (int i = i0; i < i1; i += is) {
       double ax;
       if (a.x0_zero) {
         a.x[i] = 0.0;
@@ -404,7 +404,7 @@ __device__ void cg1_rows(const Cg1Args &a, int ph, const KState *S, double *v,
   } else if (ph == CG1_W0) {
     // w = A u, u = r/m on the fly; -> w.u
     const XDiv xf{a.r0, a.m};
-    for (int i = i0 + (int)threadIdx.x; i < i1; i += blockDim.x) {
+    for (int i = i0; i < i1; i += is) {
       const double wi = row_apply<F>(a.A, i, xf);
       a.w0[i] = wi;
       v[0] += wi * xf(i);
@@ -432,15 +432,15 @@ __device__ void cg1_rows(const Cg1Args &a, int ph, const KState *S, double *v,
       v[1] += wi * un;
       v[2] += ri * ri;
     };
-    int i = i0 + (int)threadIdx.x;
+    int i = i0;
     if (pre && pre->ok && i < i1) {
       row(i, *pre);
-      i += blockDim.x;
+      i += is;
     }
-    for (; i < i1; i += blockDim.x) row(i, cg1_load<F>(a, i));
+    for (; i < i1; i += is) row(i, cg1_load<F>(a, i));
   } else {
     // CG1_FINAL: energy norm x.Ax (src/Iterativos.f:82-83) and ||b - Ax||^2
-    for (int i = i0 + (int)threadIdx.x; i < i1; i += blockDim.x) {
+    for (int i = i0; i < i1; i += is) {
       const double ax = row_apply<F>(a.A, i, XVec{a.x});
       const double ri = a.b[i] - ax;
       v[0] += a.x[i] * ax;
@@ -542,7 +542,7 @@ __global__ __launch_bounds__(1024) void cg1_kernel(Cg1Args a) {
     __device__ bool skip(int ph, const KState &s) const { return (ph == CG1_W0 || ph == CG1_ITER) && s.done; }
     __device__ Pre prefetch(int ph, const KState &) const {
       Pre q{};
-      const int i = a.g.row_begin() + (int)threadIdx.x;
+      const int i = a.g.first();
       if (ph == CG1_ITER && i < a.g.row_end()) q = cg1_load<F>(a, i);
       return q;
     }
@@ -600,12 +600,12 @@ struct XBiRes {
 
 template <int F>
 __device__ bool bicg_phase(const BicgArgs &a, int ph, double *lds) {
-  const int i0 = a.g.row_begin(), i1 = a.g.row_end();
+  const int i0 = a.g.first(), i1 = a.g.row_end(), is = a.g.stride();
   KState *st = a.st;
   if (ph == BI_SETUP) {
     // z = Ax; r = b - z; p = r; b = p; z = p/m; d = r.z
     double v[3] = {0.0, 0.0, 0.0};
-    for (int i = i0 + (int)threadIdx.x; i < i1; i += blockDim.x) {
+    for (int i = i0; i < i1; i += is) {
       double ax;
       if (a.x0_zero) {
         a.x[i] = 0.0;
@@ -655,7 +655,7 @@ __device__ bool bicg_phase(const BicgArgs &a, int ph, double *lds) {
   if (ph == BI_F1) {
     const XBiDir zf{bbc, pc, vc, a.m, ld_dbl(&st->beta), ld_dbl(&st->omega), j == 1};
     double v[1] = {0.0};
-    for (int i = i0 + (int)threadIdx.x; i < i1; i += blockDim.x) {
+    for (int i = i0; i < i1; i += is) {
       const double pi = (j == 1) ? pc[i] : bbc[i] + zf.beta * (pc[i] - zf.w * vc[i]);
       const double zi = pi / a.m[i];
       const double vi = row_apply<F>(a.A, i, zf);
@@ -674,7 +674,7 @@ __device__ bool bicg_phase(const BicgArgs &a, int ph, double *lds) {
   if (ph == BI_F2) {
     const XBiRes zf{bbc, vn, a.m, ld_dbl(&st->alpha)};
     double v[2] = {0.0, 0.0};
-    for (int i = i0 + (int)threadIdx.x; i < i1; i += blockDim.x) {
+    for (int i = i0; i < i1; i += is) {
       const double bi = bbc[i] - zf.alpha * vn[i];
       const double ti = row_apply<F>(a.A, i, zf);
       bbn[i] = bi;
@@ -693,7 +693,7 @@ __device__ bool bicg_phase(const BicgArgs &a, int ph, double *lds) {
   if (ph == BI_F3) {
     const double alpha = ld_dbl(&st->alpha), w = ld_dbl(&st->omega);
     double v[3] = {0.0, 0.0, 0.0};
-    for (int i = i0 + (int)threadIdx.x; i < i1; i += blockDim.x) {
+    for (int i = i0; i < i1; i += is) {
       const double z2i = a.z2[i];
       a.x[i] = (a.x[i] + alpha * a.z[i]) + w * z2i;
       const double bi = bbn[i] - w * a.t[i];
@@ -724,7 +724,7 @@ __device__ bool bicg_phase(const BicgArgs &a, int ph, double *lds) {
   }
   // BI_FINAL: energy x.Ax (src/Iterativos.f:204-205) and ||b - Ax||^2
   double v[2] = {0.0, 0.0};
-  for (int i = i0 + (int)threadIdx.x; i < i1; i += blockDim.x) {
+  for (int i = i0; i < i1; i += is) {
     const double ax = row_apply<F>(a.A, i, XVec{a.x});
     const double ri = a.b[i] - ax;
     v[0] += a.x[i] * ax;
@@ -824,10 +824,10 @@ __device__ __forceinline__ Bi2Row<F> bicg2_load(const Bicg2Args &a, int i) {
 template <int F>
 __device__ void bicg2_rows(const Bicg2Args &a, int ph, const KState *S, double *v,
                            const Bi2Row<F> *pre = nullptr) {
-  const int i0 = a.g.row_begin(), i1 = a.g.row_end();
+  const int i0 = a.g.first(), i1 = a.g.row_end(), is = a.g.stride();
   if (ph == B2_SETUP) {
     // r = b - Ax; rhat = r; p = r; b = r; d = r.(r/m); rho = b.rhat = r.r
-    for (int i = i0 + (int)threadIdx.x; i < i1; i += blockDim.x) {
+    for (int i = i0; i < i1; i += is) {
       double ax;
       if (a.x0_zero) {
         a.x[i] = 0.0;
@@ -848,7 +848,7 @@ __device__ void bicg2_rows(const Bicg2Args &a, int ph, const KState *S, double *
   }
   if (ph == B2_FINAL) {
     // energy x.Ax (src/Iterativos.f:204-205) and ||b - Ax||^2
-    for (int i = i0 + (int)threadIdx.x; i < i1; i += blockDim.x) {
+    for (int i = i0; i < i1; i += is) {
       const double ax = row_apply<F>(a.A, i, XVec{a.x});
       const double ri = a.b[i] - ax;
       v[0] += a.x[i] * ax;
@@ -863,7 +863,7 @@ __device__ void bicg2_rows(const Bicg2Args &a, int ph, const KState *S, double *
   if (ph == B2_P) {
     if (j == 1) {
       const XDiv xf{pc, a.m};
-      for (int i = i0 + (int)threadIdx.x; i < i1; i += blockDim.x) {
+      for (int i = i0; i < i1; i += is) {
         const double vi = row_apply<F>(a.A, i, xf);
         pn[i] = pc[i];
         vn[i] = vi;
@@ -887,12 +887,12 @@ __device__ void bicg2_rows(const Bicg2Args &a, int ph, const KState *S, double *
         v[1] += bi * z2i;
         v[2] += bi * bi;
       };
-      int i = i0 + (int)threadIdx.x;
+      int i = i0;
       if (pre && pre->ok && i < i1) {
         row(i, *pre);
-        i += blockDim.x;
+        i += is;
       }
-      for (; i < i1; i += blockDim.x) row(i, bicg2_load<F>(a, i));
+      for (; i < i1; i += is) row(i, bicg2_load<F>(a, i));
     }
     return;
   }
@@ -909,12 +909,12 @@ __device__ void bicg2_rows(const Bicg2Args &a, int ph, const KState *S, double *
     v[2] += ri * si;
     v[3] += ri * ti;
   };
-  int i = i0 + (int)threadIdx.x;
+  int i = i0;
   if (pre && pre->ok && i < i1) {
     row(i, *pre);
-    i += blockDim.x;
+    i += is;
   }
-  for (; i < i1; i += blockDim.x) row(i, bicg2_load<F>(a, i));
+  for (; i < i1; i += is) row(i, bicg2_load<F>(a, i));
 }
 
 __device__ inline void bicg2_epi(const Bicg2Args &a, int ph, KState *st, const double *t) {
@@ -1016,7 +1016,7 @@ __global__ __launch_bounds__(1024) void bicg2_kernel(Bicg2Args a) {
     __device__ bool skip(int ph, const KState &s) const { return (ph == B2_P || ph == B2_Q) && s.done; }
     __device__ Pre prefetch(int ph, const KState &) const {
       Pre q{};
-      const int i = a.g.row_begin() + (int)threadIdx.x;
+      const int i = a.g.first();
       if ((ph == B2_P || ph == B2_Q) && i < a.g.row_end()) q = bicg2_load<F>(a, i);
       return q;
     }
@@ -1058,7 +1058,7 @@ struct GmresArgs {
 // g(i,c)/norm.
 template <int F>
 __device__ bool gmres_phase(const GmresArgs &a, int ph, double *lds) {
-  const int i0 = a.g.row_begin(), i1 = a.g.row_end();
+  const int i0 = a.g.first(), i1 = a.g.row_end(), is = a.g.stride();
   KState *st = a.st;
   const int k = a.k;
   auto col = [&](int c) { return a.gv + (size_t)(c - 1) * a.ld; };
@@ -1068,7 +1068,7 @@ __device__ bool gmres_phase(const GmresArgs &a, int ph, double *lds) {
     const int first = ld_int(&st->cycle) == 0;
     double *g1 = col(1);
     double v[2] = {0.0, 0.0};
-    for (int i = i0 + (int)threadIdx.x; i < i1; i += blockDim.x) {
+    for (int i = i0; i < i1; i += is) {
       double ax;
       const double bi = a.b[i];
       if (first) {
@@ -1122,7 +1122,7 @@ __device__ bool gmres_phase(const GmresArgs &a, int ph, double *lds) {
     double *gn = col(i + 1);
     const XScaled xf{gi, ni};
     double v[1] = {0.0};
-    for (int r = i0 + (int)threadIdx.x; r < i1; r += blockDim.x) {
+    for (int r = i0; r < i1; r += is) {
       const double w = row_apply<F>(a.A, r, xf) / a.m[r];
       gn[r] = w;
       const double g1r = (i == 1) ? g1[r] / n1 : g1[r];
@@ -1144,7 +1144,7 @@ __device__ bool gmres_phase(const GmresArgs &a, int ph, double *lds) {
     const double *gj1 = col(j + 1);
     const double ni = ld_dbl(&a.nrm[i]);
     double v[1] = {0.0};
-    for (int r = i0 + (int)threadIdx.x; r < i1; r += blockDim.x) {
+    for (int r = i0; r < i1; r += is) {
       if (j == 1) gi[r] = gi[r] / ni;      // store column i normalised
       const double w = gn[r] - hj * gj[r];
       gn[r] = w;
@@ -1198,7 +1198,7 @@ __device__ bool gmres_phase(const GmresArgs &a, int ph, double *lds) {
     for (int j = (int)threadIdx.x + 1; j <= nI; j += blockDim.x) ysh[j] = ld_dbl(&a.y[j]);
     __syncthreads();
     double v[1] = {0.0};
-    for (int r = i0 + (int)threadIdx.x; r < i1; r += blockDim.x) {
+    for (int r = i0; r < i1; r += is) {
       double xr = a.x[r];
       for (int j = 1; j <= nI; ++j) xr = xr + ysh[j] * a.gv[(size_t)(j - 1) * a.ld + r];
       a.x[r] = xr;
@@ -1215,7 +1215,7 @@ __device__ bool gmres_phase(const GmresArgs &a, int ph, double *lds) {
   }
   // GM_FINAL: energy x.b (src/Iterativos.f:412) and ||b - Ax||^2
   double v[2] = {0.0, 0.0};
-  for (int i = i0 + (int)threadIdx.x; i < i1; i += blockDim.x) {
+  for (int i = i0; i < i1; i += is) {
     const double ax = row_apply<F>(a.A, i, XVec{a.x});
     const double bi = a.b[i];
     const double ri = bi - ax;
@@ -1271,9 +1271,9 @@ __global__ __launch_bounds__(256) void spmv_kernel(Mat A, const double *__restri
 __global__ __launch_bounds__(1024) void dot_kernel(Grid g, const double *a, const double *b,
                                                    double *out) {
   __shared__ double lds[kMaxK * kMaxWaves];
-  const int i0 = g.row_begin(), i1 = g.row_end();
+  const int i0 = g.first(), i1 = g.row_end(), is = g.stride();
   double v[1] = {0.0};
-  for (int i = i0 + (int)threadIdx.x; i < i1; i += blockDim.x) v[0] += a[i] * b[i];
+  for (int i = i0; i < i1; i += is) v[0] += a[i] * b[i];
   grid_reduce<1>(g, v, lds, [&](const double *t) { *out = t[0]; });
 }
 

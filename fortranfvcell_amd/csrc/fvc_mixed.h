@@ -215,14 +215,14 @@ __device__ __forceinline__ MixRow<F> mixed_load(const MixedArgs &a, int i) {
 template <int F>
 __device__ __forceinline__ void mixed_rows(const MixedArgs &a, int ph, const KState *S, double *v,
                                            const MixRow<F> *pre = nullptr) {
-  const int i0 = a.g.row_begin(), i1 = a.g.row_end();
+  const int i0 = a.g.first(), i1 = a.g.row_end(), is = a.g.stride();
   if (ph == MX_SETUP || ph == MX_R) {
     // r64 = b - A x (FP64); r~ = S r64 rounded to FP32 into the buffer the
     // next MX_A reads; y = 0
     const bool setup = ph == MX_SETUP, zero = setup && a.x0_zero;
     const int j = setup ? 1 : ld_int(&S->iter) + 1;
     float4 *rc = a.rec + ((j & 1) ? 0 : a.ps);
-    for (int i = i0 + (int)threadIdx.x; i < i1; i += blockDim.x) {
+    for (int i = i0; i < i1; i += is) {
       double ax;
       if (zero) {
         a.x[i] = 0.0;
@@ -270,26 +270,26 @@ __device__ __forceinline__ void mixed_rows(const MixedArgs &a, int ph, const KSt
       const double ru = rd * (double)w.dsq;
       v[4] += ru * ru;
     };
-    int i = i0 + (int)threadIdx.x;
+    int i = i0;
     if (pre && pre->ok && i < i1) {
       row(i, *pre);
-      i += blockDim.x;
+      i += is;
     }
-    for (; i < i1; i += blockDim.x) row(i, mixed_load<F>(a, i));
+    for (; i < i1; i += is) row(i, mixed_load<F>(a, i));
   } else if (ph == MX_X) {
     // complete the last step and fold the FP32 increment into the FP64
     // solution: x += S (y + alpha p)
     const int j = ld_int(&S->iter);
     const float4 *rj = a.rec + ((j & 1) ? a.ps : 0);
     const float alpha = (float)ld_dbl(&S->alpha);
-    for (int i = i0 + (int)threadIdx.x; i < i1; i += blockDim.x) {
+    for (int i = i0; i < i1; i += is) {
       const float4 c = rj[i];
       const float yi = c.w + alpha * c.z;
       a.x[i] = a.x[i] + a.s[i] * (double)yi;
     }
   } else {
     // MX_FINAL: energy x.Ax and ||b - Ax||^2 in FP64
-    for (int i = i0 + (int)threadIdx.x; i < i1; i += blockDim.x) {
+    for (int i = i0; i < i1; i += is) {
       const double ax = row_apply<F>(a.A, i, XVec{a.x});
       const double ri = a.b[i] - ax;
       v[0] += a.x[i] * ax;
@@ -407,7 +407,7 @@ __global__ __launch_bounds__(1024) void mixed_kernel(MixedArgs a) {
     // A only at a reliable update), so A's operands are loaded ahead
     __device__ Pre prefetch(int ph, const KState &stored) const {
       Pre q{};
-      const int i = a.g.row_begin() + (int)threadIdx.x;
+      const int i = a.g.first();
       const bool a_next = ph == MX_A || (ph == kPhaseNext && stored.pend - 1 != MX_X);
       if (a_next && i < a.g.row_end()) q = mixed_load<F>(a, i);
       return q;

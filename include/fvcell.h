@@ -106,7 +106,10 @@ typedef struct fvc_options {
   int32_t refresh_drop_e3; /* mixed: recompute when the FP32 (Jacobi-scaled) residual
                              norm fell by this factor (thousandths) since the last
                              one; 0 = 300                                            */
-  int32_t reserved[3];
+  int32_t row_order;      /* rows to blocks: 0 tuned default, 1 one contiguous chunk
+                             per block, 2 XCD tiles (each eighth of the rows swept
+                             by one XCD's blocks together)                            */
+  int32_t reserved[2];
 } fvc_options;
 
 typedef struct fvc_stats {

@@ -8,6 +8,7 @@ The two GPU engines (persistent kernel, kernel per phase) share chunking and
 reduction trees and must agree bit for bit.
 """
 import numpy as np
+import scipy.sparse.linalg as sla
 import pytest
 
 import oracle
@@ -116,6 +117,44 @@ def test_engines_bit_identical(fvc, code, lay, rec):
     assert np.array_equal(xp, xk)
     assert sp["iterations"] == sk["iterations"]
     assert sp["energy"] == sk["energy"]
+
+
+ROW_CASES = [(1, "ell", 0), (2, "ell", 0), (3, "csrd", 0), (1, "csrd_sym", 0), (2, "csrc", 0), (1, "ell", 1)]
+
+
+@pytest.mark.parametrize("G", [64, 60], ids=["8_groups", "1_group"])
+@pytest.mark.parametrize("code,lay,prec", ROW_CASES,
+                         ids=[f"{c}-{l}-{'mixed' if p else 'fp64'}" for c, l, p in ROW_CASES])
+def test_row_orders(fvc, orc, code, lay, prec, G):
+    """Rows to blocks as one contiguous chunk per block (row_order 1) or as
+    XCD tiles (row_order 2: 64 blocks make 8 groups of 8 sweeping an eighth
+    of the rows each, 60 blocks one group sweeping all of them).  With
+    64-thread blocks every block holds several rows per thread, so the two
+    orders give the rows to different threads.  Both reach the oracle's
+    solution, and under tiles the two engines agree bit for bit."""
+    s = P.config_poisson(96) if code == 1 else P.config_advdiff(96, peclet_max=60.0)
+    f = LAYOUT[lay](s)
+    unsym = not lay.endswith("sym")
+    print(f"row orders code={code} {lay} precision={prec} G={G} neq={s.neq}", flush=True)
+    opts = dict(grid_blocks=G, block_size=64)
+    if prec:
+        A = P.scipy_csr(s)
+        xd = sla.spsolve(A.tocsc(), s.b)
+        runs = [fvc.solve(f, code, tol=1e-10, precision=fvc.PREC_MIXED, row_order=ro, engine=e, **opts)
+                for ro, e in ((1, fvc.ENGINE_AUTO), (2, fvc.ENGINE_PERSISTENT), (2, fvc.ENGINE_KERNELS))]
+        for x, st in runs:
+            assert st["status"] == 0 and st["grid_blocks"] == G
+            assert np.linalg.norm(x - xd) <= 1e-8 * np.linalg.norm(xd)
+    else:
+        tol, xo, so = tol_for_relres(orc, s, f, code, unsym)
+        runs = [fvc.solve(f, code, tol=tol, maxit=20000, row_order=ro, engine=e, **opts)
+                for ro, e in ((1, fvc.ENGINE_AUTO), (2, fvc.ENGINE_PERSISTENT), (2, fvc.ENGINE_KERNELS))]
+        for x, st in runs:
+            assert st["status"] == 0 and st["grid_blocks"] == G
+            assert _rel(x, xo) <= X_TOL, _rel(x, xo)
+            assert _iters_close(st["iterations"], so["iterations"]), (st["iterations"], so["iterations"])
+    (xp, sp), (xk, sk) = runs[1], runs[2]
+    assert np.array_equal(xp, xk) and sp["iterations"] == sk["iterations"]
 
 
 def test_repeat_solves_bit_identical(fvc):
