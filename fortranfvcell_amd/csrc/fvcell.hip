@@ -656,17 +656,20 @@ int run_mixed(DevState &d, const SolveIn &in, fvc_stats *stats, hipStream_t s) {
     const int neq = in.A.neq;
     // AUTO runs one kernel per phase, as for FP64 PCG
     const bool persistent = in.engine == FVC_ENGINE_PERSISTENT;
-    // one-launch inner iterations: four blocks of 256 per CU, as for FP64
-    // PCG, measured best on MI355X (1M cells: 19.3 us per inner iteration
-    // against 21.8 at two blocks of 512 and 29.0 at four of 512)
+    // one-launch inner iterations: blocks of 256, four per CU as for FP64
+    // PCG, measured best on MI355X at 1M cells (19.3 us per inner iteration
+    // against 21.8 at two blocks of 512 and 29.0 at four of 512).  From 16
+    // rows per thread on, three per CU: 60.3 against 62.6 us at 4M cells,
+    // 237 against 253 at 16.7M, 418 against 454 at 33.5M (1.5 % slower at 1M)
     const int bs = in.bs > 0 ? in.bs : persistent ? kDefaultBlock : kernels_block(FVC_PCG, true);
+    const int per_cu = (long)neq >= 16L * 4 * d.cus * bs ? 3 : 4;
     if (bs % 64 != 0 || bs > 1024) return fail(FVC_BAD_ARGUMENT, "block size %d must be a multiple of 64 <= 1024", bs);
     const void *pfn = (const void *)&mixed_persistent<F>;
     const void *kfn = (const void *)&mixed_kernel<F>;
     int occ = 0;
     int rc = occupancy(pfn, bs, &occ);
     if (rc) return rc;
-    const int G = persistent ? choose_blocks(neq, d.cus, occ, in.G_req) : kernels_blocks(neq, d.cus, in.G_req);
+    const int G = persistent ? choose_blocks(neq, d.cus, occ, in.G_req) : kernels_blocks(neq, d.cus, in.G_req, per_cu);
     if (persistent && G > occ * d.cus)
       return fail(FVC_BAD_ARGUMENT, "%d blocks exceed the %d co-resident blocks of a persistent launch", G, occ * d.cus);
     const size_t n = (size_t)neq;
