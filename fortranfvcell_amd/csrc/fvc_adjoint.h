@@ -75,7 +75,7 @@ __global__ void assemble_heat_kernel(int neq, int w, const int *__restrict__ ja,
                                      const double *__restrict__ conv_f, const double *__restrict__ wall_g,
                                      const double *__restrict__ wall_T, double *__restrict__ ad,
                                      double *__restrict__ al, double *__restrict__ b) {
-  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  const int c = xcd_block() * (int)blockDim.x + (int)threadIdx.x;
   if (c >= neq) return;
   const double kc = k[c];
   double sa = 0.0, sw = 0.0, sb = 0.0;

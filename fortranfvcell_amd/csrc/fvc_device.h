@@ -351,6 +351,20 @@ __host__ __device__ inline int grid_rows(int neq, int G, int tiles) {
 
 __device__ __forceinline__ int group_size(int G, int g) { return (G - g + 7) / 8; }
 
+// One-item-per-thread launches over cells or rows: this block's place when
+// each XCD (the blocks b = x mod 8 under round-robin placement) takes a
+// contiguous eighth of the blocks, so the neighbours a grid row away are
+// gathered through the same L2.  A block count that is not a multiple of 8
+// keeps the identity; launches round up with xcd_blocks.
+__device__ __forceinline__ int xcd_block() {
+  const int b = (int)blockIdx.x, nb = (int)gridDim.x;
+  return (nb & 7) ? b : (b & 7) * (nb >> 3) + (b >> 3);
+}
+inline unsigned xcd_blocks(long items, int bs) {
+  const long nb = (items + bs - 1) / bs;
+  return (unsigned)((nb + 7) & ~7L);
+}
+
 // Reduce K per-thread values over the whole grid and run `epi(tot)` once, on
 // thread 0 of the last block to arrive.  In persistent mode every block then
 // waits for the epilogue to be published; in kernel-per-phase mode the other

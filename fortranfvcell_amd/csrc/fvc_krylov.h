@@ -2,13 +2,13 @@
 // as sequences of row-parallel phases separated by grid-wide reductions.
 //
 // Each method is one phase function, pcg_phase / bicg_phase / gmres_phase,
-// that runs one phase over this block's chunk of rows and ends with a
+// that runs one phase over this block's rows (Grid::first) and ends with a
 // grid_reduce whose epilogue (run once, by the last block) advances the
 // scalar recurrences held in KState.  Two engines drive the same functions:
 //   persistent : one cooperative launch loops over the phases with grid
 //                barriers (two barriers per PCG iteration);
 //   kernels    : one launch per phase; the next launch sees the epilogue.
-// Both engines use the same chunking and reduction trees, so they produce
+// Both engines use the same row order and reduction trees, so they produce
 // bit-identical results.
 #pragma once
 #include "fvc_device.h"
@@ -1263,7 +1263,7 @@ __global__ __launch_bounds__(1024) void gmres_kernel(GmresArgs a) {
 template <int F>
 __global__ __launch_bounds__(256) void spmv_kernel(Mat A, const double *__restrict__ x,
                                                    double *__restrict__ y) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int i = xcd_block() * (int)blockDim.x + (int)threadIdx.x;
   if (i < A.neq) y[i] = row_apply<F>(A, i, XVec{x});
 }
 

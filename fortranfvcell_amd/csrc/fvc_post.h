@@ -56,7 +56,7 @@ __global__ __launch_bounds__(256) void gform_p_kernel(int numel, int nnode, cons
                                                       const int *ix, const int *nelcon,
                                                       const int *pedge, const double *u,
                                                       double *grad, double *fluxl) {
-  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  const int c = xcd_block() * (int)blockDim.x + (int)threadIdx.x;
   if (c >= numel) return;
   bool ok = true;
   const int *ixc = ix + (size_t)(W + 1) * c;

@@ -1079,7 +1079,7 @@ int solve_impl(int code, int matriz, int unsym, int neq, int nad, const double *
 
 template <int F>
 int spmv_launch(const Mat &A, const double *x, double *y, hipStream_t s) {
-  hipLaunchKernelGGL(spmv_kernel<F>, dim3((A.neq + 255) / 256), dim3(256), 0, s, A, x, y);
+  hipLaunchKernelGGL(spmv_kernel<F>, dim3(xcd_blocks(A.neq, 256)), dim3(256), 0, s, A, x, y);
   HIPCHK(hipGetLastError());
   return FVC_OK;
 }
@@ -1467,7 +1467,7 @@ int fvc_assemble_heat_dev(int32_t neq, int32_t w, const int32_t *ja, const doubl
   if (neq <= 0 || w < 1 || !ja || !k || !geo || !alpha || !conv || !conv_f || !wall_g || !wall_t || !ad ||
       !al || !b)
     return fail(FVC_BAD_ARGUMENT, "bad assembly arguments");
-  hipLaunchKernelGGL(assemble_heat_kernel, dim3((neq + 255) / 256), dim3(256), 0, (hipStream_t)stream, neq, w,
+  hipLaunchKernelGGL(assemble_heat_kernel, dim3(xcd_blocks(neq, 256)), dim3(256), 0, (hipStream_t)stream, neq, w,
                      ja, k, q, geo, alpha, conv, conv_f, wall_g, wall_t, ad, al, b);
   HIPCHK(hipGetLastError());
   return FVC_OK;
@@ -1478,7 +1478,7 @@ int fvc_gform_dev(int32_t numel, int32_t nnode, int32_t nen, const double *x, co
                   double *fluxl, void *stream) {
   if (numel <= 0 || nnode <= 0 || !x || !ix || !nelcon || !pedge || !u || !grad || !fluxl)
     return fail(FVC_BAD_ARGUMENT, "bad gform arguments");
-  const dim3 grid((numel + 255) / 256), block(256);
+  const dim3 grid(xcd_blocks(numel, 256)), block(256);
   if (nen == 4)
     hipLaunchKernelGGL(gform_p_kernel<4>, grid, block, 0, (hipStream_t)stream, numel, nnode, x, ix, nelcon,
                        pedge, u, grad, fluxl);

@@ -4,7 +4,7 @@ Bar (stated by the path's contract): solutions agree with the FP64 oracle to
 ||x - x_ref|| / ||x_ref|| <= 1e-8 at relative residual ~1e-10.  Iteration
 counts agree within a small band: the GPU's dot products sum in a fixed
 tree rather than left to right, which moves the last bits of every scalar.
-The two GPU engines (persistent kernel, kernel per phase) share chunking and
+The two GPU engines (persistent kernel, kernel per phase) share the row order and
 reduction trees and must agree bit for bit.
 """
 import numpy as np
