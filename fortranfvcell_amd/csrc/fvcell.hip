@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <climits>
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
@@ -227,6 +228,14 @@ Grid make_grid(int neq, int G, unsigned *ctr, double *partials, bool persistent,
   return g;
 }
 
+// A thread's row index runs up to the last row plus one stride (at most
+// G bs), which must stay an int.
+int check_rows(int neq, int G, int bs) {
+  if ((long)neq + (long)G * bs > (long)INT_MAX)
+    return fail(FVC_UNSUPPORTED, "%d rows with %d blocks of %d threads overflow 32-bit row indices", neq, G, bs);
+  return FVC_OK;
+}
+
 // Row order of a solve (Grid::first).  Default: tiles once every thread of
 // the grid has a row; below that, chunks give every block some rows.  On
 // MI355X tiles took BiCGSTAB at 1M cells from 2.80 to 2.71 ms, PCG from 32.0
@@ -410,6 +419,7 @@ int run_solve(DevState &d, const SolveIn &in, fvc_stats *stats, hipStream_t s) {
       return fail(FVC_BAD_ARGUMENT, "%d blocks exceed the %d co-resident blocks of a persistent launch", G, occ * d.cus);
     persistent = false;
   }
+  if ((rc = check_rows(neq, G, bs))) return rc;
   const int k = in.gmres_k;
   if (in.code == FVC_GMRES && (k < 1 || k > kMaxGmresK))
     return fail(FVC_BAD_ARGUMENT, "GMRES restart k = %d outside [1, %d]", k, kMaxGmresK);
@@ -672,6 +682,7 @@ int run_mixed(DevState &d, const SolveIn &in, fvc_stats *stats, hipStream_t s) {
     const int G = persistent ? choose_blocks(neq, d.cus, occ, in.G_req) : kernels_blocks(neq, d.cus, in.G_req, per_cu);
     if (persistent && G > occ * d.cus)
       return fail(FVC_BAD_ARGUMENT, "%d blocks exceed the %d co-resident blocks of a persistent launch", G, occ * d.cus);
+    if ((rc = check_rows(neq, G, bs))) return rc;
     const size_t n = (size_t)neq;
     const size_t nval = (F == F_CSRD) ? (size_t)0 : (size_t)in.A.w * n;
     size_t nv = nval;
@@ -1079,6 +1090,7 @@ int solve_impl(int code, int matriz, int unsym, int neq, int nad, const double *
 
 template <int F>
 int spmv_launch(const Mat &A, const double *x, double *y, hipStream_t s) {
+  if (A.neq > INT_MAX - 2048) return fail(FVC_UNSUPPORTED, "%d rows overflow 32-bit row indices", A.neq);
   hipLaunchKernelGGL(spmv_kernel<F>, dim3(xcd_blocks(A.neq, 256)), dim3(256), 0, s, A, x, y);
   HIPCHK(hipGetLastError());
   return FVC_OK;
@@ -1339,6 +1351,7 @@ int batch_impl(int code, int nsys, const int32_t *row_offsets, int w, const doub
   if (row_off[0] != 0) return fail(FVC_BAD_ARGUMENT, "row_offsets[0] must be 0");
   for (int i = 0; i < nsys; ++i)
     if (row_off[i + 1] <= row_off[i]) return fail(FVC_BAD_ARGUMENT, "system %d is empty", i);
+  if (row_off[nsys] > INT_MAX - 1024) return fail(FVC_UNSUPPORTED, "%d rows overflow 32-bit row indices", row_off[nsys]);
   const size_t n = (size_t)row_off[nsys];
   Owned own;
   const double *dad = ad, *dav = al, *db = b;
@@ -1467,6 +1480,7 @@ int fvc_assemble_heat_dev(int32_t neq, int32_t w, const int32_t *ja, const doubl
   if (neq <= 0 || w < 1 || !ja || !k || !geo || !alpha || !conv || !conv_f || !wall_g || !wall_t || !ad ||
       !al || !b)
     return fail(FVC_BAD_ARGUMENT, "bad assembly arguments");
+  if (neq > INT_MAX - 2048) return fail(FVC_UNSUPPORTED, "%d cells overflow 32-bit indices", neq);
   hipLaunchKernelGGL(assemble_heat_kernel, dim3(xcd_blocks(neq, 256)), dim3(256), 0, (hipStream_t)stream, neq, w,
                      ja, k, q, geo, alpha, conv, conv_f, wall_g, wall_t, ad, al, b);
   HIPCHK(hipGetLastError());
@@ -1478,6 +1492,7 @@ int fvc_gform_dev(int32_t numel, int32_t nnode, int32_t nen, const double *x, co
                   double *fluxl, void *stream) {
   if (numel <= 0 || nnode <= 0 || !x || !ix || !nelcon || !pedge || !u || !grad || !fluxl)
     return fail(FVC_BAD_ARGUMENT, "bad gform arguments");
+  if (numel > INT_MAX - 2048) return fail(FVC_UNSUPPORTED, "%d cells overflow 32-bit indices", numel);
   const dim3 grid(xcd_blocks(numel, 256)), block(256);
   if (nen == 4)
     hipLaunchKernelGGL(gform_p_kernel<4>, grid, block, 0, (hipStream_t)stream, numel, nnode, x, ix, nelcon,

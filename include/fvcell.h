@@ -108,7 +108,8 @@ typedef struct fvc_options {
                              one; 0 = 300                                            */
   int32_t row_order;      /* rows to blocks: 0 tuned default, 1 one contiguous chunk
                              per block, 2 XCD tiles (each eighth of the rows swept
-                             by one XCD's blocks together)                            */
+                             by one XCD's blocks together); single-system solves
+                             (batches use chunks)                                    */
   int32_t reserved[2];
 } fvc_options;
 

@@ -50,3 +50,8 @@ def test_options_and_stats_layout_match_header():
     assert C.sizeof(_lib.Stats) == 4 * 4 + 5 * 8 + 2 * 4 + 4 * 8
     o = _lib.default_options()
     assert o.x0_rule == _lib.X0_AS_OPENMP and o.gmres_k == 15 and o.engine == _lib.ENGINE_AUTO
+    assert o.row_order == 0
+    assert _lib.default_options(row_order=2).row_order == 2
+    # a misspelt option is an error, not a silently ignored attribute
+    with pytest.raises(TypeError, match="row_ordr"):
+        _lib.default_options(row_ordr=2)

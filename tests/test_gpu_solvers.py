@@ -255,6 +255,8 @@ def test_status_codes(fvc):
         fvc.solve(dict(f, matriz=7), 1, raise_on_failure=True)
     with pytest.raises(fvc.FvcError):
         fvc.solve(P.to_csr(s), 3, gmres_k=1000, raise_on_failure=True)
+    with pytest.raises(fvc.FvcError, match="row_order"):
+        fvc.solve(f, 1, row_order=3)
 
 
 def test_device_resident_solve_matches_host(fvc):
