@@ -240,10 +240,13 @@ int check_rows(int neq, int G, int bs) {
 // the grid has a row; below that, chunks give every block some rows.  On
 // MI355X tiles took BiCGSTAB at 1M cells from 2.80 to 2.71 ms, PCG from 32.0
 // to 29.6 us per iteration, and at 33.5M cells PCG from 1353 to 1017 us per
-// iteration (FP32 inner iterations: 649 to 458).
-int use_tiles(int row_order, int neq, int G, int bs) {
+// iteration (FP32 inner iterations: 649 to 458).  GMRES, whose Arnoldi phases
+// are mostly dot products over whole vectors, keeps chunks (GMRES(30) at 1M
+// cells: 17.1 ms against 17.5 with tiles).
+int use_tiles(int row_order, int code, int neq, int G, int bs) {
   if (row_order == 1) return 0;
   if (row_order == 2) return 1;
+  if (code == FVC_GMRES) return 0;
   return (long)neq >= (long)G * bs ? 1 : 0;
 }
 
@@ -447,7 +450,7 @@ int run_solve(DevState &d, const SolveIn &in, fvc_stats *stats, hipStream_t s) {
   if (may_fall_back && !in.x0_zero)
     HIPCHK(hipMemcpyAsync(xbackup, in.x, n * sizeof(double), hipMemcpyDeviceToDevice, s));
 
-  Grid g = make_grid(neq, G, ctr, partials, persistent, in.barrier, use_tiles(in.row_order, neq, G, bs));
+  Grid g = make_grid(neq, G, ctr, partials, persistent, in.barrier, use_tiles(in.row_order, in.code, neq, G, bs));
   HIPCHK(hipEventRecord(d.ev0, s));
   hipLaunchKernelGGL(reset_kernel, dim3(1), dim3(256), 0, s, ctr, (int)kCtrWords + G, st, 1);
   HIPCHK(hipGetLastError());
@@ -713,7 +716,7 @@ int run_mixed(DevState &d, const SolveIn &in, fvc_stats *stats, hipStream_t s) {
     a.A32.av = av32;
     a.A32.ia = in.A.ia;
     a.A32.ja = in.A.ja;
-    a.g = make_grid(neq, G, ctr, partials, persistent, in.barrier, use_tiles(in.row_order, neq, G, bs));
+    a.g = make_grid(neq, G, ctr, partials, persistent, in.barrier, use_tiles(in.row_order, FVC_PCG, neq, G, bs));
     a.st = st;
     a.m = in.m;
     a.s = sc;

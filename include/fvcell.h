@@ -106,7 +106,8 @@ typedef struct fvc_options {
   int32_t refresh_drop_e3; /* mixed: recompute when the FP32 (Jacobi-scaled) residual
                              norm fell by this factor (thousandths) since the last
                              one; 0 = 300                                            */
-  int32_t row_order;      /* rows to blocks: 0 tuned default, 1 one contiguous chunk
+  int32_t row_order;      /* rows to blocks: 0 tuned default (tiles for PCG and
+                             BiCGSTAB from one row per thread), 1 one contiguous chunk
                              per block, 2 XCD tiles (each eighth of the rows swept
                              by one XCD's blocks together); single-system solves
                              (batches use chunks)                                    */
