@@ -22,6 +22,8 @@ struct Mat32 {
   const float *av;   // scaled off-diagonal values, same layout as the FP64 matrix
   const int *ia;     // shared with the FP64 matrix
   const int *ja;
+  const short *jo;   // ELLPACK widths 3 and 4: column offsets j - i (0-based), or
+                     // null (then ja); col_offsets_kernel
 };
 
 template <int F, class X>
@@ -38,20 +40,26 @@ struct RowRegs32 {
     return row_apply32<F>(A, i, xf);
   }
 };
+// c0.. hold 0-based columns.
 template <>
 struct RowRegs32<F_ELL4> {
   float a0, a1, a2, a3;
   int c0, c1, c2, c3;
   __device__ __forceinline__ void load(const Mat32 &A, int i) {
     const float *a = A.av + 4 * (size_t)i;
-    const int *c = A.ja + 4 * (size_t)i;
     a0 = a[0]; a1 = a[1]; a2 = a[2]; a3 = a[3];
-    c0 = c[0]; c1 = c[1]; c2 = c[2]; c3 = c[3];
+    if (A.jo) {
+      const short4 o = *reinterpret_cast<const short4 *>(A.jo + 4 * (size_t)i);
+      c0 = i + o.x; c1 = i + o.y; c2 = i + o.z; c3 = i + o.w;
+    } else {
+      const int *c = A.ja + 4 * (size_t)i;
+      c0 = c[0] - 1; c1 = c[1] - 1; c2 = c[2] - 1; c3 = c[3] - 1;
+    }
   }
   template <class X>
   __device__ __forceinline__ float apply(const Mat32 &, int i, const X &xf) const {
-    const float t1 = a0 * xf(c0 - 1), t2 = a1 * xf(c1 - 1);
-    const float t3 = a2 * xf(c2 - 1), t4 = a3 * xf(c3 - 1);
+    const float t1 = a0 * xf(c0), t2 = a1 * xf(c1);
+    const float t3 = a2 * xf(c2), t4 = a3 * xf(c3);
     return (((t1 + t2) + t3) + t4) + xf(i);
   }
 };
@@ -61,13 +69,18 @@ struct RowRegs32<F_ELL3> {
   int c0, c1, c2;
   __device__ __forceinline__ void load(const Mat32 &A, int i) {
     const float *a = A.av + 3 * (size_t)i;
-    const int *c = A.ja + 3 * (size_t)i;
     a0 = a[0]; a1 = a[1]; a2 = a[2];
-    c0 = c[0]; c1 = c[1]; c2 = c[2];
+    if (A.jo) {
+      const short *o = A.jo + 3 * (size_t)i;
+      c0 = i + o[0]; c1 = i + o[1]; c2 = i + o[2];
+    } else {
+      const int *c = A.ja + 3 * (size_t)i;
+      c0 = c[0] - 1; c1 = c[1] - 1; c2 = c[2] - 1;
+    }
   }
   template <class X>
   __device__ __forceinline__ float apply(const Mat32 &, int i, const X &xf) const {
-    const float t1 = a0 * xf(c0 - 1), t2 = a1 * xf(c1 - 1), t3 = a2 * xf(c2 - 1);
+    const float t1 = a0 * xf(c0), t2 = a1 * xf(c1), t3 = a2 * xf(c2);
     return ((t1 + t2) + t3) + xf(i);
   }
 };
@@ -91,6 +104,30 @@ __device__ __forceinline__ float row_apply32(const Mat32 &A, int i, const X &xf)
     for (int k = k1; k < k2; ++k) t = t + A.av[k] * xf(A.ja[k] - 1);
     return t;
   }
+}
+
+// Column offsets j - i of a width-W ELLPACK matrix as int16, read by the
+// FP32 inner iterations instead of the int32 columns (8 instead of 16 bytes
+// per row at width 4).  A slot whose scaled FP32 value is zero (ELLPACK
+// padding: column 1, value 0) gets offset 0, the row itself, and still adds
+// a zero.  A nonzero slot whose offset does not fit sets *wide, and the
+// caller then keeps the int32 columns.
+template <int W>
+__global__ void col_offsets_kernel(int neq, const int *ja, const float *av32, short *jo, int *wide) {
+  const int i = xcd_block() * (int)blockDim.x + (int)threadIdx.x;
+  if (i >= neq) return;
+  bool fits = true;
+  for (int k = 0; k < W; ++k) {
+    const size_t e = (size_t)W * i + k;
+    const int d = (ja[e] - 1) - i;
+    short o = 0;
+    if (av32[e] != 0.0f) {
+      if (d >= -32768 && d <= 32767) o = (short)d;
+      else fits = false;
+    }
+    jo[e] = o;
+  }
+  if (!fits) *wide = 1;
 }
 
 // Builds S A S in FP32: av32 = a_ij * s_i * s_j, s_i = 1/sqrt(a_ii); also

@@ -694,17 +694,24 @@ int run_mixed(DevState &d, const SolveIn &in, fvc_stats *stats, hipStream_t s) {
       HIPCHK(hipMemcpy(&last, in.A.ia + neq, sizeof(int), hipMemcpyDeviceToHost));
       nv = (size_t)std::max(last - 1, 0);
     }
-    size_t bytes = align_up((kCtrWords + (size_t)G) * sizeof(unsigned)) + align_up(sizeof(KState)) +
+    // int16 column offsets for ELLPACK widths 3 and 4 (col_offsets_kernel)
+    constexpr int kW16 = (F == F_ELL4) ? 4 : (F == F_ELL3) ? 3 : 0;
+    const size_t n16 = kW16 ? (size_t)kW16 * n : 0;
+    size_t bytes = align_up((kCtrWords + (size_t)G + 1) * sizeof(unsigned)) + align_up(sizeof(KState)) +
                    align_up(kMaxK * (size_t)G * sizeof(double)) + align_up(nv * sizeof(float)) +
                    align_up(n * sizeof(double)) + align_up(n * sizeof(float)) + 2 * align_up(n * sizeof(float4)) +
-                   align_up(2 * sizeof(KState)) + align_up(2 * kMaxK * (size_t)G * sizeof(double)) + 4096;
+                   align_up(2 * sizeof(KState)) + align_up(2 * kMaxK * (size_t)G * sizeof(double)) +
+                   align_up(std::max<size_t>(n16, 1) * sizeof(short)) + 4096;
     if ((rc = ensure_arena(d, bytes))) return rc;
     Carve cv{d.arena};
-    unsigned *ctr = cv.take<unsigned>(kCtrWords + (size_t)G);
+    // barrier counters, G arrival flags, then the offsets' "does not fit" flag
+    unsigned *ctr = cv.take<unsigned>(kCtrWords + (size_t)G + 1);
+    int *wide = reinterpret_cast<int *>(ctr + kCtrWords + G);
     KState *st = cv.take<KState>(1);
     double *partials = cv.take<double>(kMaxK * (size_t)G);
     Defer dfr{cv.take<KState>(2), cv.take<double>(2 * kMaxK * (size_t)G), 0};
     float *av32 = cv.take<float>(std::max<size_t>(nv, 1));
+    short *jo = cv.take<short>(std::max<size_t>(n16, 1));
     double *sc = cv.take<double>(n);
     float *dsq = cv.take<float>(n);
     MixedArgs a{};
@@ -729,10 +736,19 @@ int run_mixed(DevState &d, const SolveIn &in, fvc_stats *stats, hipStream_t s) {
     a.x0_zero = in.x0_zero;
     a.max_inner = in.refresh_every;
     HIPCHK(hipEventRecord(d.ev0, s));
-    hipLaunchKernelGGL(reset_kernel, dim3(1), dim3(256), 0, s, ctr, (int)kCtrWords + G, st, 1);
+    hipLaunchKernelGGL(reset_kernel, dim3(1), dim3(256), 0, s, ctr, (int)kCtrWords + G + 1, st, 1);
     hipLaunchKernelGGL(scale_to_f32_kernel<F>, dim3((neq + 255) / 256), dim3(256), 0, s, in.A, in.m,
                        av32, sc, dsq);
     HIPCHK(hipGetLastError());
+    if constexpr (kW16 > 0) {
+      hipLaunchKernelGGL(col_offsets_kernel<kW16>, dim3(xcd_blocks(neq, 256)), dim3(256), 0, s, neq, in.A.ja,
+                         av32, jo, wide);
+      HIPCHK(hipGetLastError());
+      int h_wide = 1;
+      HIPCHK(hipMemcpyAsync(&h_wide, wide, sizeof(int), hipMemcpyDeviceToHost, s));
+      HIPCHK(hipStreamSynchronize(s));
+      if (!h_wide) a.A32.jo = jo;
+    }
     const KState *final_state = st;
     if (persistent) {
       a.phase = -1;

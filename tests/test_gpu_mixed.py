@@ -52,6 +52,39 @@ def test_mixed_engines_bit_identical(fvc):
     assert sp["inner_iterations"] == sk["inner_iterations"]
 
 
+@pytest.mark.parametrize("triangles", [False, True], ids=["ell4", "ell3"])
+def test_mixed_column_offsets_int16_and_int32(fvc, triangles):
+    """The FP32 inner iterations read ELLPACK columns as int16 offsets j - i
+    when every nonzero slot's offset fits, else the int32 columns.  The
+    mesh's own numbering (offsets up to a grid row) takes the first path; a
+    random renumbering of the same system (offsets up to its size, above
+    32,767) takes the second.  Both meet the bar of the FP64 path."""
+    s = P.config_pressure_lognormal(200 if not triangles else 140, triangles=triangles)
+    n, w = s.neq, s.w
+    rng = np.random.default_rng(11)
+    p = rng.permutation(n)
+    q = np.empty(n, np.int64)
+    q[p] = np.arange(n)
+    nb = s.nbr[p]
+    inner = nb >= 0
+    ja = np.where(inner, q[np.where(inner, nb, 0)] + 1, 1).astype(np.int32)
+    al = np.where(inner, s.off[p], 0.0)
+    renumbered = dict(matriz=4, neq=n, nad=int(inner.sum()), iax=np.array([w, n + 1], np.int32),
+                      ja=np.ascontiguousarray(ja.ravel()), ad=s.ad[p].copy(),
+                      al=np.ascontiguousarray(al.ravel()), au=None, b=s.b[p].copy(), w=w)
+    offsets = np.abs(ja.astype(np.int64) - 1 - np.arange(n)[:, None])[inner]
+    assert offsets.max() > 32767
+    own = np.abs(s.nbr - np.arange(n)[:, None])[s.nbr >= 0]
+    assert own.max() <= 32767
+    A = P.scipy_csr(s)
+    xd = sla.spsolve(A.tocsc(), s.b)
+    print(f"column offsets neq={n} w={w} renumbered max offset {offsets.max()}", flush=True)
+    for f, xref in ((P.to_ell(s), xd), (renumbered, xd[p])):
+        x, st = fvc.solve(f, fvc.PCG, tol=1e-10, precision=fvc.PREC_MIXED)
+        assert st["status"] == 0, st
+        assert np.linalg.norm(x - xref) <= 1e-8 * np.linalg.norm(xref)
+
+
 def test_mixed_maxit_and_warm_start(fvc):
     # maxit stops the FP32 iteration count; a warm start from the converged
     # solution needs no iteration at all
