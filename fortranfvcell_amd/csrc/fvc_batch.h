@@ -40,6 +40,9 @@ struct BatchArgs {
   float4 *rec;                      // mixed: {r~, q, p, y} records, two buffers,
   size_t fs;                        // the second at +fs
   float *av32;                      // FP32 scaled values (mixed)
+  short *jo;                        // mixed, ELLPACK widths 3 and 4: local column
+                                    // offsets j - i, or null (then ja)
+  int *wide;                        // set when an offset does not fit int16
   double *s;                        // 1/sqrt(a_ii) (mixed)
   float *dsq;                       // sqrt(a_ii) (mixed)
   double tol;
@@ -128,6 +131,7 @@ __global__ __launch_bounds__(1024) void mixed_batch_persistent(BatchArgs B) {
   a.A32.av = B.av32 + off * B.w;
   a.A32.ia = nullptr;
   a.A32.ja = a.A.ja;
+  a.A32.jo = B.jo ? B.jo + off * B.w : nullptr;
   batch_grid(B, sys, D, a.g);
   a.g.ep = &s_ep;
   a.st = B.st + sys;
@@ -147,7 +151,8 @@ __global__ __launch_bounds__(1024) void mixed_batch_persistent(BatchArgs B) {
 }
 
 // S A S in FP32 for every system of the batch (rows are independent; the
-// column indices are local to each system).
+// column indices are local to each system), and with B.jo the int16 column
+// offsets of col_offsets_kernel (local row and column, the same rules).
 template <int F>
 __global__ void batch_scale_kernel(BatchArgs B, int nsys, const int *row_off) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -164,11 +169,25 @@ __global__ void batch_scale_kernel(BatchArgs B, int nsys, const int *row_off) {
   B.s[i] = si;
   B.dsq[i] = (float)sqrt(B.ad[i]);
   const int w = (F == F_ELL4) ? 4 : (F == F_ELL3) ? 3 : B.w;
+  bool fits = true;
   for (int k = 0; k < w; ++k) {
     const size_t e = (size_t)w * i + k;
     const int j = B.ja[e] - 1;
-    B.av32[e] = (float)(B.av[e] * si * (1.0 / sqrt(B.ad[off + j])));
+    const float v = (float)(B.av[e] * si * (1.0 / sqrt(B.ad[off + j])));
+    B.av32[e] = v;
+    if constexpr (F == F_ELL4 || F == F_ELL3) {
+      if (B.jo) {
+        const int d = j - (i - (int)off);
+        short o = 0;
+        if (v != 0.0f) {
+          if (d >= -32768 && d <= 32767) o = (short)d;
+          else fits = false;
+        }
+        B.jo[e] = o;
+      }
+    }
   }
+  if (!fits) *B.wide = 1;
 }
 
 }  // namespace fvc

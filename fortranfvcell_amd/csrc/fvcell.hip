@@ -1263,8 +1263,10 @@ int run_batch(DevState &d, int nsys, const std::vector<int> &row_off, int w, con
   size_t bytes = align_up(((size_t)wave * kCtrWords + cap) * sizeof(unsigned)) + align_up((size_t)wave * sizeof(KState)) +
                  align_up((size_t)kMaxK * cap * sizeof(double)) + align_up((size_t)wave * sizeof(SysDesc)) +
                  align_up((size_t)cap * sizeof(int)) + 4096;
+  constexpr bool kOff16 = (F == F_ELL4 || F == F_ELL3);
   if (mixed) bytes += 2 * align_up(n * sizeof(float4)) + align_up(n * w * sizeof(float)) +
-                      align_up(n * sizeof(double)) + align_up(n * sizeof(float));
+                      align_up(n * sizeof(double)) + align_up(n * sizeof(float)) +
+                      (kOff16 ? align_up(n * w * sizeof(short)) + align_up(sizeof(int)) : 0);
   else bytes += 5 * align_up(n * sizeof(double));
   if ((rc = ensure_arena(d, bytes))) return rc;
   Carve cv{d.arena};
@@ -1294,6 +1296,10 @@ int run_batch(DevState &d, int nsys, const std::vector<int> &row_off, int w, con
     B.av32 = cv.take<float>(n * w);
     B.s = cv.take<double>(n);
     B.dsq = cv.take<float>(n);
+    if (kOff16) {
+      B.jo = cv.take<short>(n * w);
+      B.wide = cv.take<int>(1);
+    }
   } else {
     for (double *&v : B.v) v = cv.take<double>(n);
   }
@@ -1306,8 +1312,15 @@ int run_batch(DevState &d, int nsys, const std::vector<int> &row_off, int w, con
   B.block_sys = dbsys;
   HIPCHK(hipEventRecord(d.ev0, s));
   if (mixed) {
+    if (B.jo) hipLaunchKernelGGL(reset_kernel, dim3(1), dim3(64), 0, s, reinterpret_cast<unsigned *>(B.wide), 1, st, 0);
     hipLaunchKernelGGL(batch_scale_kernel<F>, dim3((total + 255) / 256), dim3(256), 0, s, B, nsys, d_row_off);
     HIPCHK(hipGetLastError());
+    if (B.jo) {
+      int h_wide = 1;
+      HIPCHK(hipMemcpyAsync(&h_wide, B.wide, sizeof(int), hipMemcpyDeviceToHost, s));
+      HIPCHK(hipStreamSynchronize(s));
+      if (h_wide) B.jo = nullptr;   // keep the int32 columns
+    }
   }
   std::vector<KState> host_st(nsys);
   for (int s0 = 0; s0 < nsys; s0 += wave) {

@@ -42,6 +42,49 @@ def test_batch_matches_single_solves(fvc, orc, precision):
         assert np.linalg.norm(xi - xo) <= 1e-6 * np.linalg.norm(xo)
 
 
+def _renumbered(s, seed):
+    """The ELLPACK arrays of s with its cells numbered in random order, and the
+    order (new row r is old row p[r])."""
+    n, w = s.neq, s.w
+    p = np.random.default_rng(seed).permutation(n)
+    q = np.empty(n, np.int64)
+    q[p] = np.arange(n)
+    nb = s.nbr[p]
+    inner = nb >= 0
+    ja = np.where(inner, q[np.where(inner, nb, 0)] + 1, 1).astype(np.int32)
+    al = np.where(inner, s.off[p], 0.0)
+    f = dict(matriz=4, neq=n, nad=int(inner.sum()), iax=np.array([w, n + 1], np.int32),
+             ja=np.ascontiguousarray(ja.ravel()), ad=s.ad[p].copy(),
+             al=np.ascontiguousarray(al.ravel()), au=None, b=s.b[p].copy(), w=w)
+    return f, p
+
+
+def test_batch_mixed_column_offsets_int16_and_int32(fvc):
+    """Mixed batches read the FP32 inner iterations' columns as int16 offsets
+    j - i when every slot of every system fits, else the int32 columns.  A
+    batch in mesh numbering takes the first path; the same batch with a
+    40,000-cell system renumbered at random (offsets above 32,767) the second."""
+    small = _systems(6, 16, 48, 3)
+    big = P.config_pressure_lognormal(200)
+    renum, p = _renumbered(big, 11)
+    for with_big in (False, True):
+        systems = small + ([big] if with_big else [])
+        fmts = [P.to_ell(s) for s in small] + ([renum] if with_big else [])
+        batch = fvc.concat_ell(fmts)
+        print(f"mixed batch of {len(fmts)} systems, renumbered member: {with_big}", flush=True)
+        x, sts = fvc.solve_batch(batch, tol=1e-10, precision=1, stop_rule=fvc.STOP_RELRES)
+        ro = batch["row_offsets"]
+        for i, s in enumerate(systems):
+            assert sts[i]["status"] == 0, (i, sts[i])
+            xi = x[ro[i]:ro[i + 1]]
+            if s is big:
+                xo = np.empty(s.neq)
+                xo[p] = xi
+                xi = xo
+            r = np.linalg.norm(s.b - P.scipy_csr(s) @ xi) / np.linalg.norm(s.b)
+            assert r <= 1e-10 * 1.01, (i, r)
+
+
 def test_batch_uneven_sizes_and_waves(fvc):
     # more systems than half the co-resident blocks: solved in waves
     systems = _systems(300, 4, 12, 11)
