@@ -416,7 +416,11 @@ int run_solve(DevState &d, const SolveIn &in, fvc_stats *stats, hipStream_t s) {
   if (rc) return rc;
   if (occ < 1) return fail(FVC_DEVICE_ERROR, "persistent kernel does not fit a CU at block size %d", bs);
   const int G_req = (gmres_p && in.G_req == 0) ? std::max(8, (3 * d.cus / 4) & ~7) : in.G_req;
-  const int G = persistent ? choose_blocks(neq, d.cus, occ, G_req) : kernels_blocks(neq, d.cus, G_req);
+  // Fused BiCGSTAB from 24 rows per thread on: two blocks per CU (MI355X,
+  // us per iteration at 1024 -> 512 blocks: 8.4M cells 491 -> 464, 16.7M
+  // 1040 -> 945; equal at 4M).  PCG stays at four (16.7M: 553 against 722).
+  const int per_cu = (in.code == FVC_BICGSTAB && fused && (long)neq >= 24L * 4 * d.cus * bs) ? 2 : 4;
+  const int G = persistent ? choose_blocks(neq, d.cus, occ, G_req) : kernels_blocks(neq, d.cus, G_req, per_cu);
   if (persistent && G > occ * d.cus) {
     if (in.engine == FVC_ENGINE_PERSISTENT)
       return fail(FVC_BAD_ARGUMENT, "%d blocks exceed the %d co-resident blocks of a persistent launch", G, occ * d.cus);
