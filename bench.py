@@ -276,8 +276,8 @@ def main():
     if args.c3b:
         return bench_c3b(args.c3b, max(args.steps, 1))
     if args.gpus != 1:
-        print(f"bench.py: this build runs on one GPU only (asked for --gpus {args.gpus}); "
-              "the solver path is a single-GPU design, see the sharding notes", file=sys.stderr)
+        print(f"bench.py: the solver runs on one GPU only (asked for --gpus {args.gpus}); "
+              "there is no multi-GPU path to measure", file=sys.stderr)
         sys.exit(2)
 
     import fortranfvcell_amd as F
