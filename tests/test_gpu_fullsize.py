@@ -35,3 +35,29 @@ def test_fullsize_vs_reference_statistics(fvc, name, build, code):
     scale = np.sqrt(STATS[name + "_xsq"] / s.neq)
     assert np.max(np.abs(x[idx] - xs)) <= 1e-8 * scale * np.sqrt(s.neq) ** 0 * 10
     assert abs(np.sqrt((x * x).sum()) - np.sqrt(STATS[name + "_xsq"])) <= 1e-8 * np.sqrt(STATS[name + "_xsq"])
+
+
+def test_large_bicgstab_grid_and_row_order(fvc):
+    """Above 24 rows per thread at four blocks of 256 per CU (6.76M cells
+    here), BiCGSTAB runs on two blocks per CU, with XCD row tiles.  The solve
+    reaches the true residual bar and agrees with the four-per-CU grid and
+    with one contiguous chunk of rows per block."""
+    s = P.config_advdiff(2600)
+    f = P.to_ell(s)
+    A = P.scipy_csr(s)
+    ds = fvc.DeviceSystem(f)
+    opts = dict(tol=1e-10, stop_rule=fvc.STOP_RELRES, x0_rule=fvc.X0_ZERO)
+    runs = []
+    for extra in ({}, {"grid_blocks": 1024}, {"row_order": 1}):
+        st = ds.solve(fvc.BICGSTAB, **opts, **extra)
+        x = ds.x.get()
+        r = np.linalg.norm(s.b - A @ x) / np.linalg.norm(s.b)
+        print(f"  neq={s.neq} {extra or 'default'}: G={st['grid_blocks']} {st['iterations']} it "
+              f"relres={r:.2e} {st['seconds'] * 1e3:.1f} ms", flush=True)
+        assert st["status"] == 0 and r <= 1e-10, (extra, r)
+        runs.append((x, st))
+    cus = fvc.device_info()["compute_units"]
+    per_cu = 2 if s.neq >= 24 * 4 * cus * 256 else 4
+    assert runs[0][1]["grid_blocks"] == per_cu * cus and runs[0][1]["block_size"] == 256
+    for x, _ in runs[1:]:
+        assert np.linalg.norm(x - runs[0][0]) <= 1e-8 * np.linalg.norm(runs[0][0])
