@@ -61,3 +61,29 @@ def test_large_bicgstab_grid_and_row_order(fvc):
     assert runs[0][1]["grid_blocks"] == per_cu * cus and runs[0][1]["block_size"] == 256
     for x, _ in runs[1:]:
         assert np.linalg.norm(x - runs[0][0]) <= 1e-8 * np.linalg.norm(runs[0][0])
+
+
+def test_large_mixed_pcg_grid(fvc):
+    """From 16 rows per thread at four blocks of 256 per CU (the 2048x2048
+    lognormal pressure system), mixed PCG runs on three blocks per CU, reads
+    int16 column offsets and reaches the FP64 true residual bar; its
+    solution agrees with FP64 PCG's to what that residual leaves free."""
+    s = P.config_pressure_lognormal(2048)
+    f = P.to_ell(s)
+    A = P.scipy_csr(s)
+    ds = fvc.DeviceSystem(f)
+    opts = dict(tol=1e-10, stop_rule=fvc.STOP_RELRES, x0_rule=fvc.X0_ZERO)
+    st64 = ds.solve(fvc.PCG, **opts)
+    x64 = ds.x.get()
+    stm = ds.solve(fvc.PCG, precision=fvc.PREC_MIXED, **opts)
+    xm = ds.x.get()
+    r = np.linalg.norm(s.b - A @ xm) / np.linalg.norm(s.b)
+    d = np.linalg.norm(xm - x64) / np.linalg.norm(x64)
+    print(f"  neq={s.neq}: fp64 {st64['iterations']} it {st64['seconds']:.2f} s, mixed {stm['iterations']} it "
+          f"{stm['seconds']:.2f} s G={stm['grid_blocks']}, relres {r:.2e}, vs fp64 {d:.2e}", flush=True)
+    assert st64["status"] == 0 and stm["status"] == 0
+    assert r <= 1e-10
+    cus = fvc.device_info()["compute_units"]
+    per_cu = 3 if s.neq >= 16 * 4 * cus * 256 else 4
+    assert stm["grid_blocks"] == per_cu * cus
+    assert d <= 1e-7
