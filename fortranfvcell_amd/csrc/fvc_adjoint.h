@@ -23,7 +23,7 @@ namespace fvc {
 __global__ void ell_transpose_kernel(int neq, int w, const int *__restrict__ ja,
                                      const int *__restrict__ back, const double *__restrict__ a,
                                      double *__restrict__ at) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int i = xcd_block() * (int)blockDim.x + (int)threadIdx.x;
   if (i >= neq) return;
   for (int s = 0; s < w; ++s) {
     const size_t e = (size_t)w * i + s;
@@ -37,7 +37,7 @@ __global__ void grad_k_kernel(int neq, int w, const int *__restrict__ ja, const 
                               const double *__restrict__ geo, const double *__restrict__ alpha,
                               const double *__restrict__ wall_g, const double *__restrict__ wall_T,
                               double *__restrict__ grad) {
-  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  const int c = xcd_block() * (int)blockDim.x + (int)threadIdx.x;
   if (c >= neq) return;
   const double kc = k[c], Tc = T[c], lc = lam[c];
   double acc = 0.0;

@@ -1492,7 +1492,8 @@ int fvc_solve_batch_dev(int32_t code, int32_t nsys, const int32_t *row_offsets, 
 int fvc_ell_transpose_dev(int32_t neq, int32_t w, const int32_t *ja, const int32_t *back,
                           const double *al, double *alt, void *stream) {
   if (neq <= 0 || w < 1 || !ja || !back || !al || !alt) return fail(FVC_BAD_ARGUMENT, "bad transpose arguments");
-  hipLaunchKernelGGL(ell_transpose_kernel, dim3((neq + 255) / 256), dim3(256), 0, (hipStream_t)stream,
+  if (neq > INT_MAX - 2048) return fail(FVC_UNSUPPORTED, "%d rows overflow 32-bit row indices", neq);
+  hipLaunchKernelGGL(ell_transpose_kernel, dim3(xcd_blocks(neq, 256)), dim3(256), 0, (hipStream_t)stream,
                      neq, w, ja, back, al, alt);
   HIPCHK(hipGetLastError());
   return FVC_OK;
@@ -1503,7 +1504,8 @@ int fvc_grad_k_dev(int32_t neq, int32_t w, const int32_t *ja, const double *k, c
                    const double *wall_t, double *grad_k, void *stream) {
   if (neq <= 0 || w < 1 || !ja || !k || !t || !lam || !geo || !alpha || !wall_g || !wall_t || !grad_k)
     return fail(FVC_BAD_ARGUMENT, "bad gradient arguments");
-  hipLaunchKernelGGL(grad_k_kernel, dim3((neq + 255) / 256), dim3(256), 0, (hipStream_t)stream, neq, w,
+  if (neq > INT_MAX - 2048) return fail(FVC_UNSUPPORTED, "%d cells overflow 32-bit indices", neq);
+  hipLaunchKernelGGL(grad_k_kernel, dim3(xcd_blocks(neq, 256)), dim3(256), 0, (hipStream_t)stream, neq, w,
                      ja, k, t, lam, geo, alpha, wall_g, wall_t, grad_k);
   HIPCHK(hipGetLastError());
   return FVC_OK;
