@@ -18,7 +18,8 @@ import numpy as np
 
 from . import _lib
 from ._lib import (BICGSTAB, CSR, CSRC, CSRD, ELLPACK, ENGINE_AUTO, ENGINE_KERNELS,
-                   ENGINE_PERSISTENT, GMRES, PCG, PREC_FP64, PREC_MIXED, REC_FUSED, REC_REFERENCE,
+                   ENGINE_PERSISTENT, FLAG_COL16, FLAG_TILES, GMRES, PCG, PREC_FP64, PREC_MIXED,
+                   REC_FUSED, REC_REFERENCE,
                    STOP_REFERENCE,
                    STOP_RELRES, X0_AS_OPENMP,
                    X0_AS_SEQUENTIAL, X0_GIVEN, X0_ZERO, FvcError, Options, Stats, check,

@@ -22,6 +22,7 @@ STOP_REFERENCE, STOP_RELRES = 0, 1
 ENGINE_AUTO, ENGINE_KERNELS, ENGINE_PERSISTENT = 0, 1, 2
 PREC_FP64, PREC_MIXED = 0, 1
 REC_FUSED, REC_REFERENCE = 0, 1
+FLAG_COL16, FLAG_TILES = 1, 2
 
 # Every symbol include/fvcell.h declares.
 EXPORTS = [
@@ -48,10 +49,11 @@ class Stats(C.Structure):
     _fields_ = [("status", C.c_int32), ("iterations", C.c_int32), ("inner_iterations", C.c_int32),
                 ("engine", C.c_int32), ("energy", C.c_double), ("conv", C.c_double),
                 ("last", C.c_double), ("true_relres", C.c_double), ("seconds", C.c_double),
-                ("grid_blocks", C.c_int32), ("block_size", C.c_int32), ("reserved", C.c_double * 4)]
+                ("grid_blocks", C.c_int32), ("block_size", C.c_int32), ("flags", C.c_int32),
+                ("reserved_i", C.c_int32), ("reserved", C.c_double * 3)]
 
     def as_dict(self) -> dict:
-        return {k: getattr(self, k) for k, _ in self._fields_ if k != "reserved"}
+        return {k: getattr(self, k) for k, _ in self._fields_ if not k.startswith("reserved")}
 
 
 class FvcError(RuntimeError):

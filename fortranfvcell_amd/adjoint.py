@@ -104,6 +104,14 @@ class HeatSolver:
             raise RuntimeError(f"correction solve did not converge: {st}")
         return self.lam.get()
 
+    def adjoint_at(self, k: np.ndarray, q: np.ndarray | None, T: np.ndarray, gT: np.ndarray):
+        """(dL/dk, dL/dq) at the forward state (k, q, T) given dL/dT: A(k) is
+        assembled again and T uploaded first, so the result does not depend
+        on which forward solve ran last on this solver."""
+        self.load(k, q)
+        self.T.upload(np.asarray(T, dtype=np.float64))
+        return self.adjoint(gT)
+
     def adjoint(self, gT: np.ndarray):
         """(dL/dk, dL/dq) for the last forward solve, given dL/dT."""
         L = load()
@@ -123,17 +131,22 @@ def make_fvsolve():
     import torch
 
     class FVSolve(torch.autograd.Function):
-        """T = FVSolve.apply(k, q, solver): differentiable in k and q."""
+        """T = FVSolve.apply(k, q, solver): differentiable in k and q.  Each
+        application keeps its own k, q and T, so one solver may be applied
+        several times (time steps, several loss terms) before a backward."""
 
         @staticmethod
         def forward(ctx, k, q, solver: HeatSolver):
-            T = solver.forward(k.detach().cpu().numpy(), q.detach().cpu().numpy())
+            T = torch.from_numpy(solver.forward(k.detach().cpu().numpy(), q.detach().cpu().numpy()).copy())
             ctx.solver = solver
-            return torch.from_numpy(T.copy())
+            ctx.save_for_backward(k, q, T)
+            return T
 
         @staticmethod
         def backward(ctx, gT):
-            gk, gq = ctx.solver.adjoint(gT.detach().cpu().numpy())
+            k, q, T = ctx.saved_tensors
+            gk, gq = ctx.solver.adjoint_at(k.detach().cpu().numpy(), q.detach().cpu().numpy(),
+                                           T.detach().cpu().numpy(), gT.detach().cpu().numpy())
             return torch.from_numpy(gk), torch.from_numpy(gq), None
 
     return FVSolve

@@ -1048,6 +1048,7 @@ struct GmresArgs {
   double *nrm;     // normaliser of each basis column (k+2 entries)
   double tol;
   int k, maxit, phase;
+  int stop_rule;   // 0 the reference's test on the preconditioned residual, 1 the true residual
   size_t ld;       // leading dimension of gv (= neq)
 };
 
@@ -1056,6 +1057,15 @@ struct GmresArgs {
 // first MGS phase of the step that uses it; until then its readers divide by
 // its norm on the fly, which is the same rounding as the reference's stored
 // g(i,c)/norm.
+//
+// Stopping.  The reference ends a cycle, and the solve, when the
+// preconditioned residual |e(i+1)| = ||M^-1 (b - A x)|| falls to tol ||b||
+// (src/Iterativos.f:295-296,376,402): a mixed norm, which at 1M cells leaves
+// the true relative residual ~75x above tol.  With stop_rule 1 the start of
+// every cycle, which forms r = b - A x anyway, also reduces ||r||^2: the solve
+// ends there when ||r|| <= tol ||b||.  Otherwise the cycle runs until the
+// preconditioned residual has dropped by the factor the true one still needs,
+// |e(i+1)| <= e(1) tol ||b|| / ||r||, and the next cycle's start checks again.
 template <int F>
 __device__ bool gmres_phase(const GmresArgs &a, int ph, double *lds) {
   const int i0 = a.g.first(), i1 = a.g.row_end(), is = a.g.stride();
@@ -1067,7 +1077,7 @@ __device__ bool gmres_phase(const GmresArgs &a, int ph, double *lds) {
     // g1 = (b - A x)/m; e(1) = ||g1||; cycle 1 starts from x = 0 and sets econv
     const int first = ld_int(&st->cycle) == 0;
     double *g1 = col(1);
-    double v[2] = {0.0, 0.0};
+    double v[3] = {0.0, 0.0, 0.0};
     for (int i = i0; i < i1; i += is) {
       double ax;
       const double bi = a.b[i];
@@ -1078,11 +1088,13 @@ __device__ bool gmres_phase(const GmresArgs &a, int ph, double *lds) {
       } else {
         ax = row_apply<F>(a.A, i, XVec{a.x});
       }
-      const double gi = (bi - ax) / a.m[i];
+      const double ri = bi - ax;
+      const double gi = ri / a.m[i];
       g1[i] = gi;
       v[0] += gi * gi;
+      v[2] += ri * ri;
     }
-    return grid_sync<2>(a.g, v, lds, [&](const double *t) {
+    return grid_sync<3>(a.g, v, lds, [&](const double *t) {
       if (first) {
         st->bnorm2 = t[1];
         st->econv = a.tol * sqrt(t[1]);
@@ -1091,8 +1103,24 @@ __device__ bool gmres_phase(const GmresArgs &a, int ph, double *lds) {
         st->status = S_MAXIT;
         st->done = 0;
       }
-      st->cycle = st->cycle + 1;
       const double e1 = sqrt(t[0]);
+      if (a.stop_rule == 1) {
+        const double rt = sqrt(t[2]);
+        st->rnorm2 = t[2];
+        st->last = rt;
+        st->iter = st->cycle;
+        const bool stop = rt <= st->conv || !finite(rt) || !finite(e1) || st->cycle >= a.maxit;
+        if (rt <= st->conv) st->status = S_OK;
+        else if (!finite(rt) || !finite(e1)) st->status = S_NONFINITE;
+        if (stop) {
+          st->done = 1;
+          st->inner_done = 1;
+          return;
+        }
+        // this cycle's target for the preconditioned residual
+        st->econv = e1 * (st->conv / rt);
+      }
+      st->cycle = st->cycle + 1;
       a.e[1] = e1;
       a.nrm[1] = e1;
       st->step_i = 1;
@@ -1207,8 +1235,9 @@ __device__ bool gmres_phase(const GmresArgs &a, int ph, double *lds) {
       const double en = a.e[nI + 1];
       st->last = fabs(en);
       st->iter = st->cycle;
-      if (fabs(en) <= st->econv) { st->status = S_OK; st->done = 1; }   // :402
-      else if (!finite(en)) { st->status = S_NONFINITE; st->done = 1; }
+      if (!finite(en)) { st->status = S_NONFINITE; st->done = 1; }
+      else if (a.stop_rule == 1) {}   // the next cycle's start tests the true residual
+      else if (fabs(en) <= st->econv) { st->status = S_OK; st->done = 1; }   // :402
       else if (st->cycle >= a.maxit) { st->status = S_MAXIT; st->done = 1; }
       st->inner_done = 0;
     });

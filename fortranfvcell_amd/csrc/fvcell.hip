@@ -311,7 +311,8 @@ int read_state(DevState &d, KState *st, hipStream_t s) {
   return FVC_OK;
 }
 
-void fill_stats(fvc_stats *stats, const KState &hs, int code, bool persistent, float ms, int G, int bs);
+void fill_stats(fvc_stats *stats, const KState &hs, int code, bool persistent, float ms, int G, int bs,
+                int flags);
 int finish(const KState &hs, unsigned *ctr);
 
 // The size of the next batch: doubling up to kMaxBatch, capped near the end
@@ -593,14 +594,15 @@ int run_solve(DevState &d, const SolveIn &in, fvc_stats *stats, hipStream_t s) {
     a.e = sm; sm += k + 2;
     a.y = sm; sm += k + 2;
     a.nrm = sm; sm += k + 2;
-    a.tol = in.tol; a.k = k; a.maxit = in.maxit;
+    a.tol = in.tol; a.k = k; a.maxit = in.maxit; a.stop_rule = in.stop_rule;
     if (persistent) {
       a.phase = -1;
       if ((rc = launch(pfn, true, G, bs, a, s))) return rc;
       HIPCHK(hipEventRecord(d.ev1, s));
       if ((rc = read_state(d, st, s))) return rc;
     } else {
-      for (int cyc = 0; cyc < in.maxit; ++cyc) {
+      // under the true-residual rule the start of cycle maxit + 1 decides
+      for (int cyc = 0; cyc <= in.maxit; ++cyc) {
         a.phase = GM_START;
         if ((rc = launch(kfn, false, G, bs, a, s))) return rc;
         for (int i = 1; i <= k; ++i) {
@@ -632,20 +634,22 @@ int run_solve(DevState &d, const SolveIn &in, fvc_stats *stats, hipStream_t s) {
   }
   float ms = 0.f;
   HIPCHK(hipEventElapsedTime(&ms, d.ev0, d.ev1));
-  fill_stats(stats, *d.host_st, in.code, persistent, ms, G, bs);
+  fill_stats(stats, *d.host_st, in.code, persistent, ms, G, bs, g.tiles ? FVC_FLAG_TILES : 0);
   // A bounded grid wait that expired is a device error, never a result.
   return finish(*d.host_st, ctr);
 }
 
-void fill_stats(fvc_stats *stats, const KState &hs, int code, bool persistent, float ms, int G, int bs) {
+void fill_stats(fvc_stats *stats, const KState &hs, int code, bool persistent, float ms, int G, int bs,
+                int flags) {
   if (!stats) return;
   std::memset(stats, 0, sizeof *stats);
   stats->status = hs.status;
   stats->iterations = hs.iter;
   stats->inner_iterations = code == FVC_GMRES ? hs.inner : hs.iter;
   stats->engine = persistent ? FVC_ENGINE_PERSISTENT : FVC_ENGINE_KERNELS;
+  stats->flags = flags;
   stats->energy = hs.energy;
-  stats->conv = code == FVC_GMRES ? hs.econv : hs.conv;
+  stats->conv = hs.conv;
   stats->last = hs.last;
   stats->true_relres = hs.bnorm2 > 0 ? std::sqrt(hs.res2 / hs.bnorm2) : std::sqrt(hs.res2);
   stats->seconds = ms * 1e-3;
@@ -779,7 +783,8 @@ int run_mixed(DevState &d, const SolveIn &in, fvc_stats *stats, hipStream_t s) {
     if ((rc = read_state(d, const_cast<KState *>(final_state), s))) return rc;
     float ms = 0.f;
     HIPCHK(hipEventElapsedTime(&ms, d.ev0, d.ev1));
-    fill_stats(stats, *d.host_st, FVC_PCG, persistent, ms, G, bs);
+    fill_stats(stats, *d.host_st, FVC_PCG, persistent, ms, G, bs,
+               (a.g.tiles ? FVC_FLAG_TILES : 0) | (a.A32.jo ? FVC_FLAG_COL16 : 0));
     if (stats) stats->inner_iterations = d.host_st->cycle;   // FP64 residual recomputations
     return finish(*d.host_st, ctr);
   }
@@ -927,7 +932,7 @@ bool is_device_pointer(const void *p) {
 int build_device_matrix(int matriz, int unsym, int neq, int nad, const double *ad,
                         const double *au, const double *al, const int32_t *iax,
                         const int32_t *ja, bool on_device, Owned &own, MatDesc &md,
-                        const double **m_out) {
+                        const double **m_out, hipStream_t s) {
   int w = 0;
   std::vector<int> h_ia;
   const int *ia_host = nullptr;
@@ -1007,7 +1012,8 @@ int build_device_matrix(int matriz, int unsym, int neq, int nad, const double *a
   if (matriz == FVC_CSR) {
     double *dm = nullptr;
     if ((rc = own.up(&dm, (const double *)nullptr, (size_t)neq))) return rc;
-    hipLaunchKernelGGL(csr_diag_kernel, dim3((neq + 255) / 256), dim3(256), 0, 0, neq,
+    // on the solve's stream, so the diagonal is written before the solve reads it
+    hipLaunchKernelGGL(csr_diag_kernel, dim3((neq + 255) / 256), dim3(256), 0, s, neq,
                        md.m.ia, md.m.ja, md.m.av, dm);
     HIPCHK(hipGetLastError());
     *m_out = dm;
@@ -1035,7 +1041,7 @@ int solve_impl(int code, int matriz, int unsym, int neq, int nad, const double *
   double *dx = x;
   const double *db = b;
   if (on_device) {
-    if ((rc = build_device_matrix(matriz, unsym, neq, nad, ad, au, al, iax, ja, true, own, md, &m))) {
+    if ((rc = build_device_matrix(matriz, unsym, neq, nad, ad, au, al, iax, ja, true, own, md, &m, s))) {
       if (stats) stats->status = rc;
       return rc;
     }
@@ -1059,7 +1065,7 @@ int solve_impl(int code, int matriz, int unsym, int neq, int nad, const double *
       d->mat_bufs.next = 0;
       Owned kept;
       kept.persist = &d->mat_bufs;
-      if ((rc = build_device_matrix(matriz, unsym, neq, nad, ad, au, al, iax, ja, false, kept, md, &m))) {
+      if ((rc = build_device_matrix(matriz, unsym, neq, nad, ad, au, al, iax, ja, false, kept, md, &m, s))) {
         if (stats) stats->status = rc;
         return rc;
       }
@@ -1103,8 +1109,8 @@ int solve_impl(int code, int matriz, int unsym, int neq, int nad, const double *
   if (in.refresh_drop >= 1.0) return fail(FVC_BAD_ARGUMENT, "refresh_drop_e3 %d must be below 1000", o.refresh_drop_e3);
   if (o.recurrence != FVC_REC_FUSED && o.recurrence != FVC_REC_REFERENCE)
     return fail(FVC_BAD_ARGUMENT, "recurrence %d (0 fused, 1 reference order)", o.recurrence);
-  if (code == FVC_GMRES && o.stop_rule != FVC_STOP_REFERENCE)
-    return fail(FVC_UNSUPPORTED, "GMRES supports only the reference stopping rule");
+  if (o.stop_rule != FVC_STOP_REFERENCE && o.stop_rule != FVC_STOP_RELRES)
+    return fail(FVC_BAD_ARGUMENT, "stop_rule %d (0 reference, 1 true relative residual)", o.stop_rule);
   rc = dispatch_solve(*d, in, stats, s);
   if (!on_device && (rc == FVC_OK || rc == FVC_MAXIT || rc == FVC_BREAKDOWN || rc == FVC_NONFINITE))
     HIPCHK(hipMemcpy(x, dx, (size_t)neq * sizeof(double), hipMemcpyDeviceToHost));
@@ -1129,7 +1135,7 @@ int matvec_impl(int matriz, int unsym, int neq, int nad, const double *ad, const
   Owned own;
   MatDesc md;
   const double *m = nullptr;
-  if ((rc = build_device_matrix(matriz, unsym, neq, nad, ad, au, al, iax, ja, on_device, own, md, &m))) return rc;
+  if ((rc = build_device_matrix(matriz, unsym, neq, nad, ad, au, al, iax, ja, on_device, own, md, &m, s))) return rc;
   const double *dx = x;
   double *dy = y;
   if (!on_device) {
@@ -1361,7 +1367,7 @@ int run_batch(DevState &d, int nsys, const std::vector<int> &row_off, int w, con
   int worst = S_OK;
   for (int i = 0; i < nsys; ++i) {
     if (stats) {
-      fill_stats(&stats[i], host_st[i], FVC_PCG, true, ms, 0, bs);
+      fill_stats(&stats[i], host_st[i], FVC_PCG, true, ms, 0, bs, B.jo ? FVC_FLAG_COL16 : 0);
       if (mixed) stats[i].inner_iterations = host_st[i].cycle;
     }
     if (host_st[i].status != S_OK && worst == S_OK) worst = host_st[i].status;

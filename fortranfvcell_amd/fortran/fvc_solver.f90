@@ -23,8 +23,8 @@ module fvc_capi
   type, bind(C) :: fvc_stats_t
     integer(c_int32_t) :: status, iterations, inner_iterations, engine
     real(c_double) :: energy, conv, last, true_relres, seconds
-    integer(c_int32_t) :: grid_blocks, block_size
-    real(c_double) :: reserved(4)
+    integer(c_int32_t) :: grid_blocks, block_size, flags, reserved_i
+    real(c_double) :: reserved(3)
   end type fvc_stats_t
 
   interface

@@ -62,11 +62,15 @@ extern "C" {
 
 /* Stopping rules. */
 #define FVC_STOP_REFERENCE 0    /* each method's own test (src/Iterativos.f:73,190,376)     */
-#define FVC_STOP_RELRES 1       /* recursive ||r||_2 <= tol*||b||_2; PCG and BiCGSTAB then check
-                                   the true FP64 residual and restart from x while it misses
-                                   the tolerance and each restart at least halves it
-                                   (iterations include the restarts; true_relres reports
-                                   what was reached)                                         */
+#define FVC_STOP_RELRES 1       /* ||b - A x||_2 <= tol*||b||_2.  PCG and BiCGSTAB test the
+                                   recursive residual, then check the true FP64 residual and
+                                   restart from x while it misses the tolerance and each
+                                   restart at least halves it (iterations include the
+                                   restarts; true_relres reports what was reached).  GMRES
+                                   tests the true residual at the start of every cycle, and
+                                   runs a cycle until the preconditioned residual has dropped
+                                   by the factor the true one still needs; maxit counts
+                                   cycles                                                    */
 
 /* Engines. */
 #define FVC_ENGINE_AUTO 0        /* KERNELS for PCG and BiCGSTAB, PERSISTENT for GMRES   */
@@ -128,8 +132,15 @@ typedef struct fvc_stats {
   double seconds;           /* device time of the solve (HIP events)                 */
   int32_t grid_blocks;      /* row chunks used                                       */
   int32_t block_size;
-  double reserved[4];
+  int32_t flags;            /* FVC_FLAG_* bits: which variant of the kernels ran      */
+  int32_t reserved_i;
+  double reserved[3];
 } fvc_stats;
+
+/* fvc_stats.flags */
+#define FVC_FLAG_COL16 1        /* mixed precision read int16 column offsets (else the int32
+                                   columns: an offset did not fit)                           */
+#define FVC_FLAG_TILES 2        /* rows went to blocks as XCD tiles (else contiguous chunks) */
 
 /* Fills *opt with the defaults listed above. */
 void fvc_default_options(fvc_options *opt);

@@ -9,8 +9,10 @@ energy norm it writes to its unit-13 log (src/Iterativos.f:91-93,216-218,
 statistics (iterations, energy, sum and sum of squares of x, 64 sampled
 entries).  Run in the development container (needs the reference build):
 
-    python tests/golden/make_golden.py           # ref_solver_small.npz, ref_solver_1024_stats.npz
-    python tests/golden/make_golden.py configs   # ref_configs.npz
+    python tests/golden/make_golden.py             # ref_solver_small.npz, ref_solver_1024_stats.npz
+    python tests/golden/make_golden.py configs     # ref_configs.npz
+    python tests/golden/make_golden.py ladder1024  # ref_ladder_1024.npz
+    python tests/golden/make_golden.py steady      # ref_steady.npz (tens of minutes)
 """
 import os
 import re
@@ -212,9 +214,66 @@ def configs():
                         **arrays, **stats)
 
 
+def _stats(name, s, x, it, en, tol, rng):
+    idx = np.sort(rng.choice(s.neq, 64, replace=False))
+    return {f"{name}_iterations": np.int64(it), f"{name}_energy": np.float64(en),
+            f"{name}_xsum": np.float64(x.sum()), f"{name}_xsq": np.float64((x * x).sum()),
+            f"{name}_idx": idx, f"{name}_xsample": x[idx], f"{name}_tol": np.float64(tol),
+            f"{name}_relres": np.float64(_relres(s, x))}
+
+
+def ladder1024():
+    """1024x1024 statistics at the ladder tolerance, where the reference's own
+    rule leaves a true relative residual <= 1e-10: GMRES(30) and BiCGSTAB on
+    the advection-diffusion momentum system (ELLPACK, sequential paths).
+    Written to ref_ladder_1024.npz."""
+    rng = np.random.default_rng(13)
+    s = P.config_advdiff(1024)
+    f = P.to_ell(s)
+    stats = {}
+    for name, run in (("advdiff1024_ell_gmres30",
+                       lambda t: _ref_call(lambda: oracle.krylov("gmres", f, k=30, tol=t, use_ref=True)[:2])),
+                      ("advdiff1024_ell_bicgstab",
+                       lambda t: _ref_call(lambda: oracle.ref_solver(f, 2, unsym=True, omp=False, tol=t)))):
+        tol, ((x, _), label, it, en) = _ladder(s, run)
+        stats.update(_stats(name, s, x, it, en, tol, rng))
+        print(name, it, en, tol, _relres(s, x), label, flush=True)
+    np.savez_compressed(os.path.join(HERE, "ref_ladder_1024.npz"), **stats)
+
+
+def steady(n_bicg: int = 1024, n_gmres: int = 256):
+    """The advection-diffusion system at the reference's defaults: no
+    under-relaxation (underU = 1, src/Main.f:219) and no transient term, so
+    a_P = sum a_nb + sum F_f.  BiCGSTAB at n_bicg^2 and GMRES(30) at
+    n_gmres^2 (the largest sizes the sequential reference finishes here in
+    minutes), at the ladder tolerance.  Written to ref_steady.npz."""
+    rng = np.random.default_rng(17)
+    stats = {}
+    for name, n, run in (
+            (f"steady{n_gmres}_ell_gmres30", n_gmres,
+             lambda f, t: _ref_call(lambda: oracle.krylov("gmres", f, k=30, tol=t, maxit=100000,
+                                                          use_ref=True)[:2])),
+            (f"steady{n_bicg}_ell_bicgstab", n_bicg,
+             lambda f, t: _ref_call(lambda: oracle.ref_solver(f, 2, unsym=True, omp=False, tol=t,
+                                                              maxit=500000)))):
+        s = P.config_advdiff(n, dt=None, under_relax=1.0)
+        f = P.to_ell(s)
+        tol, ((x, _), label, it, en) = _ladder(s, lambda t: run(f, t))
+        stats.update(_stats(name, s, x, it, en, tol, rng))
+        print(name, it, en, tol, _relres(s, x), label, flush=True)
+    np.savez_compressed(os.path.join(HERE, "ref_steady.npz"), **stats)
+
+
 if __name__ == "__main__":
-    if len(sys.argv) > 1 and sys.argv[1] == "configs":
+    cmd = sys.argv[1] if len(sys.argv) > 1 else ""
+    if cmd == "configs":
         oracle.build(ref=True)
         configs()
+    elif cmd == "ladder1024":
+        oracle.build(ref=True)
+        ladder1024()
+    elif cmd == "steady":
+        oracle.build(ref=True)
+        steady(*(int(a) for a in sys.argv[2:4]))
     else:
         main()

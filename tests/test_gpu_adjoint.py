@@ -111,6 +111,33 @@ def test_torch_autograd_function(fvc):
         assert abs(gq[c] - fd) <= 1e-6 * abs(fd) + 1e-15
 
 
+def test_torch_two_applications_before_backward(fvc):
+    """One solver applied to two conductivities (two time levels, two loss
+    terms) before one backward: each application's gradient is taken at its
+    own k, q and T, not at the solver's last forward state."""
+    torch = pytest.importorskip("torch")
+    from fortranfvcell_amd.adjoint import HeatSolver, make_fvsolve
+    FVSolve = make_fvsolve()
+    op, k1, subset = P.heat_problem(32, subset_frac=0.05)
+    k2 = k1 * np.exp(0.3 * np.sin(np.arange(op.mesh.neq)))
+    q2 = 1e-2 * np.cos(np.arange(op.mesh.neq))
+    solver = HeatSolver(op, tol=1e-14)
+    kt1 = torch.tensor(k1, dtype=torch.float64, requires_grad=True)
+    kt2 = torch.tensor(k2, dtype=torch.float64, requires_grad=True)
+    qt1 = torch.zeros(op.mesh.neq, dtype=torch.float64, requires_grad=True)
+    qt2 = torch.tensor(q2, dtype=torch.float64, requires_grad=True)
+    idx = torch.from_numpy(subset)
+    T1 = FVSolve.apply(kt1, qt1, solver)
+    T2 = FVSolve.apply(kt2, qt2, solver)
+    ((T1[idx] ** 2).mean() + (T2[idx] ** 2).mean()).backward()
+    from fortranfvcell_amd.adjoint import loss_and_grad
+    _, gk1, gq1, _ = loss_and_grad(solver, k1, subset)
+    _, gk2, gq2, _ = loss_and_grad(solver, k2, subset, q2)
+    assert np.abs(gk1 - gk2).max() > 1e-3 * np.abs(gk1).max(), "the two applications must differ"
+    for g, ref in ((kt1.grad, gk1), (kt2.grad, gk2), (qt1.grad, gq1), (qt2.grad, gq2)):
+        assert np.allclose(g.numpy(), ref, rtol=1e-8, atol=1e-12 * np.abs(ref).max())
+
+
 def test_adjoint_full_config_512(fvc):
     """The differentiation config: 512x512 jittered quads, lognormal k, 1%
     subset; 32 random cells against central finite differences, and all cells

@@ -76,6 +76,8 @@ def test_batch_mixed_column_offsets_int16_and_int32(fvc):
         ro = batch["row_offsets"]
         for i, s in enumerate(systems):
             assert sts[i]["status"] == 0, (i, sts[i])
+            # one system whose offsets do not fit sends the whole batch to int32
+            assert bool(sts[i]["flags"] & fvc.FLAG_COL16) == (not with_big), (i, sts[i])
             xi = x[ro[i]:ro[i + 1]]
             if s is big:
                 xo = np.empty(s.neq)

@@ -82,6 +82,7 @@ def test_large_mixed_pcg_grid(fvc):
     print(f"  neq={s.neq}: fp64 {st64['iterations']} it {st64['seconds']:.2f} s, mixed {stm['iterations']} it "
           f"{stm['seconds']:.2f} s G={stm['grid_blocks']}, relres {r:.2e}, vs fp64 {d:.2e}", flush=True)
     assert st64["status"] == 0 and stm["status"] == 0
+    assert stm["flags"] & fvc.FLAG_COL16 and stm["flags"] & fvc.FLAG_TILES, stm
     assert r <= 1e-10
     cus = fvc.device_info()["compute_units"]
     per_cu = 3 if s.neq >= 16 * 4 * cus * 256 else 4

@@ -1,0 +1,130 @@
+"""GMRES(k) under the true-residual stopping rule (FVC_STOP_RELRES): the
+start of every restart cycle checks ||b - A x|| <= tol ||b|| in FP64.  The
+reference's own rule compares the preconditioned residual with ||b||
+(src/Iterativos.f:295-296,376,402) and leaves the true relative residual
+~75x above tol at 1M cells; its solution at the ladder tolerance (the
+largest tol at which its rule does reach true 1e-10, written by
+tests/golden/make_golden.py) is what the GPU's true-residual solve must
+agree with: ||x - x_ref|| / ||x_ref|| <= 1e-8, both at residual <= 1e-10."""
+import os
+
+import numpy as np
+import pytest
+
+from fortranfvcell_amd import problems as P
+from test_golden import CONFIG_KEYS, REF_FIXTURES, config_system
+
+pytestmark = pytest.mark.gpu
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+
+
+def _relres(s, x):
+    return np.linalg.norm(s.b - P.scipy_csr(s) @ x) / np.linalg.norm(s.b)
+
+
+@pytest.mark.parametrize("key", [k for k in CONFIG_KEYS if k.endswith("gmres30")])
+@pytest.mark.parametrize("layout", ["ell", "csrd"])
+def test_gmres_true_residual_small(fvc, key, layout):
+    s, f = config_system(key)
+    if layout == "csrd":
+        f = P.to_csrd(s)
+    xref = REF_FIXTURES[key + "_x"]
+    print(f"gmres relres {key} {layout} neq={s.neq}", flush=True)
+    out = {}
+    for engine in (fvc.ENGINE_PERSISTENT, fvc.ENGINE_KERNELS):
+        x, st = fvc.solve(f, fvc.GMRES, tol=1e-10, gmres_k=30, stop_rule=fvc.STOP_RELRES, engine=engine)
+        r = _relres(s, x)
+        print(f"  engine {engine}: {st['iterations']} cycles {st['inner_iterations']} steps relres {r:.2e} "
+              f"(reported {st['true_relres']:.2e})", flush=True)
+        assert st["status"] == 0, st
+        assert r <= 1e-10
+        assert abs(st["true_relres"] - r) <= 1e-3 * r
+        assert np.linalg.norm(x - xref) <= 1e-8 * np.linalg.norm(xref)
+        out[engine] = (x, st)
+    # both engines run the same phases with the same reduction trees
+    (xp, sp), (xk, sk) = out[fvc.ENGINE_PERSISTENT], out[fvc.ENGINE_KERNELS]
+    assert np.array_equal(xp, xk) and sp["inner_iterations"] == sk["inner_iterations"]
+
+
+def test_gmres_true_residual_needs_more_than_the_reference_rule(fvc):
+    """At the same tol the reference rule stops with the true residual above
+    it; the true-residual rule runs further and meets it."""
+    s = P.config_advdiff(128)
+    f = P.to_ell(s)
+    x0, s0 = fvc.solve(f, fvc.GMRES, tol=1e-10, gmres_k=30)
+    x1, s1 = fvc.solve(f, fvc.GMRES, tol=1e-10, gmres_k=30, stop_rule=fvc.STOP_RELRES)
+    assert _relres(s, x0) > 1e-10 >= _relres(s, x1)
+    assert s1["inner_iterations"] > s0["inner_iterations"]
+
+
+def test_gmres_true_residual_limits(fvc):
+    s = P.config_advdiff(48)
+    f = P.to_ell(s)
+    for engine in (fvc.ENGINE_PERSISTENT, fvc.ENGINE_KERNELS):
+        # maxit counts cycles; the start of cycle maxit + 1 reports the status
+        x, st = fvc.solve(f, fvc.GMRES, tol=1e-14, gmres_k=5, maxit=3, stop_rule=fvc.STOP_RELRES,
+                          engine=engine)
+        assert st["status"] == 1 and st["iterations"] == 3, st
+        assert st["inner_iterations"] == 15
+        # b = 0: x = 0 at once, no cycle
+        x, st = fvc.solve(f, fvc.GMRES, b=np.zeros(s.neq), tol=1e-10, gmres_k=5, stop_rule=fvc.STOP_RELRES,
+                          engine=engine)
+        assert st["status"] == 0 and st["iterations"] == 0 and not x.any(), st
+
+
+LADDER = os.path.join(HERE, "golden", "ref_ladder_1024.npz")
+
+
+@pytest.fixture(scope="module")
+def ladder():
+    return np.load(LADDER, allow_pickle=False)
+
+
+@pytest.fixture(scope="module")
+def advdiff1024():
+    s = P.config_advdiff(1024)
+    return s, fvc_device_system(s)
+
+
+def fvc_device_system(s):
+    import fortranfvcell_amd as F
+    return F.DeviceSystem(P.to_ell(s))
+
+
+def _check_stats(name, ladder, s, x, bar):
+    xsq = float(ladder[name + "_xsq"])
+    assert abs((x * x).sum() - xsq) <= bar * xsq
+    idx = ladder[name + "_idx"]
+    assert np.max(np.abs(x[idx] - ladder[name + "_xsample"])) <= bar * 10 * np.sqrt(xsq / s.neq)
+
+
+@pytest.mark.parametrize("method", ["gmres30", "bicgstab"])
+def test_1024_at_the_ladder_tolerance(fvc, ladder, advdiff1024, method):
+    """1M cells: at the ladder tolerance the GPU under the reference's rule
+    takes the reference's iterations and reaches its energy and solution; under
+    the true-residual rule at tol 1e-10 it reaches <= 1e-10 and the same
+    solution to 1e-8."""
+    s, ds = advdiff1024
+    name = f"advdiff1024_ell_{method}"
+    tol = float(ladder[name + "_tol"])
+    code = fvc.GMRES if method == "gmres30" else fvc.BICGSTAB
+    kw = dict(gmres_k=30) if method == "gmres30" else dict(x0_rule=fvc.X0_ZERO)
+    st = ds.solve(code, tol=tol, **kw)
+    x = ds.x.get()
+    it_ref = int(ladder[name + "_iterations"])
+    print(f"{name}: tol {tol:g}: gpu {st['iterations']} it, reference {it_ref}, relres {_relres(s, x):.2e}, "
+          f"{st['seconds'] * 1e3:.1f} ms", flush=True)
+    assert st["status"] == 0
+    assert abs(st["iterations"] - it_ref) <= max(1, int(0.03 * it_ref))
+    e_ref = float(ladder[name + "_energy"])
+    assert abs(st["energy"] - e_ref) <= 1e-8 * abs(e_ref)
+    assert _relres(s, x) <= 1e-10
+    _check_stats(name, ladder, s, x, 1e-8)
+    st = ds.solve(code, tol=1e-10, stop_rule=fvc.STOP_RELRES, **kw)
+    x = ds.x.get()
+    r = _relres(s, x)
+    print(f"  true-residual rule: {st['iterations']} it ({st['inner_iterations']}), relres {r:.2e}, "
+          f"{st['seconds'] * 1e3:.1f} ms", flush=True)
+    assert st["status"] == 0 and r <= 1e-10
+    _check_stats(name, ladder, s, x, 1e-8)

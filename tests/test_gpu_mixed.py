@@ -79,9 +79,10 @@ def test_mixed_column_offsets_int16_and_int32(fvc, triangles):
     A = P.scipy_csr(s)
     xd = sla.spsolve(A.tocsc(), s.b)
     print(f"column offsets neq={n} w={w} renumbered max offset {offsets.max()}", flush=True)
-    for f, xref in ((P.to_ell(s), xd), (renumbered, xd[p])):
+    for f, xref, col16 in ((P.to_ell(s), xd, True), (renumbered, xd[p], False)):
         x, st = fvc.solve(f, fvc.PCG, tol=1e-10, precision=fvc.PREC_MIXED)
         assert st["status"] == 0, st
+        assert bool(st["flags"] & fvc.FLAG_COL16) == col16, st
         assert np.linalg.norm(x - xref) <= 1e-8 * np.linalg.norm(xref)
 
 
