@@ -23,6 +23,7 @@ ENGINE_AUTO, ENGINE_KERNELS, ENGINE_PERSISTENT = 0, 1, 2
 PREC_FP64, PREC_MIXED = 0, 1
 REC_FUSED, REC_REFERENCE = 0, 1
 FLAG_COL16, FLAG_TILES = 1, 2
+ORTH_MGS, ORTH_CGS2 = 0, 1
 
 # Every symbol include/fvcell.h declares.
 EXPORTS = [
@@ -42,7 +43,8 @@ class Options(C.Structure):
                 ("true_residual", C.c_int32), ("engine", C.c_int32), ("block_size", C.c_int32),
                 ("grid_blocks", C.c_int32), ("precision", C.c_int32), ("batch_work_exp10", C.c_int32),
                 ("barrier", C.c_int32), ("recurrence", C.c_int32), ("refresh_every", C.c_int32),
-                ("refresh_drop_e3", C.c_int32), ("row_order", C.c_int32), ("reserved", C.c_int32 * 2)]
+                ("refresh_drop_e3", C.c_int32), ("row_order", C.c_int32), ("lane_exchange", C.c_int32),
+                ("gmres_orth", C.c_int32)]
 
 
 class Stats(C.Structure):

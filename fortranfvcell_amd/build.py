@@ -37,10 +37,32 @@ def _run(cmd, cwd=None):
     return r
 
 
+# translation units: the host code and small kernels, then one per family of
+# solver kernels (explicitly instantiated there, see csrc/fvc_instances.h)
+HIP_UNITS = ("fvcell", "k_cg", "k_bicg", "k_gmres", "k_mixed", "k_batch")
+
+
 def build_hip() -> str:
+    """Compiles the units in parallel (hipcc -c each), then links libfvcell.so."""
     out = os.path.join(HERE, "libfvcell.so")
+    objdir = os.path.join(HERE, "csrc", "_obj")
+    os.makedirs(objdir, exist_ok=True)
+    flags = [f for f in HIP_FLAGS if f != "-shared"]
+    procs = []
+    for u in HIP_UNITS:
+        obj = os.path.join(objdir, u + ".o")
+        cmd = ["hipcc", *flags, "-c", "-o", obj, os.path.join(HERE, "csrc", u + ".hip")]
+        procs.append((cmd, obj, subprocess.Popen(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT,
+                                                  text=True)))
+    objs = []
+    for cmd, obj, pr in procs:
+        log = pr.communicate()[0]
+        if pr.returncode != 0:
+            sys.stderr.write(log)
+            raise RuntimeError(f"build step failed: {' '.join(cmd)}")
+        objs.append(obj)
     tmp = out + ".tmp"
-    _run(["hipcc", *HIP_FLAGS, "-o", tmp, os.path.join(HERE, "csrc", "fvcell.hip")])
+    _run(["hipcc", f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp, *objs])
     os.replace(tmp, out)
     return out
 

@@ -140,6 +140,59 @@ __device__ __forceinline__ double row_apply(const Mat &A, int i, const X &xf) {
   }
 }
 
+// ------------------------------------------------- lane neighbour exchange --
+//
+// A wavefront's lanes hold consecutive rows (Grid::first), so on a mesh's
+// natural numbering a row's west and east neighbours i-1 and i+1 are the
+// rows of lanes t-1 and t+1.  A fused phase forms its SpMV operand on the
+// fly from several vectors (p/m from s, t, p, v and m in BiCGSTAB), and it
+// forms that operand for its own row anyway; the DPP wave shifts hand it to
+// the lanes on either side, so those two neighbours cost two register moves
+// instead of a gather of every vector.  The operand is the same expression on
+// the same values, so results are bit-identical.  The rows of one pass that
+// are below the block's row end are a prefix of the wavefront's lanes: lane
+// t-1 of an active lane is active; lane t+1 is when row i+1 is below the end.
+// Lane 0's west and lane 63's east neighbours, and columns that are not i+-1,
+// are gathered as before.
+constexpr int kDppWaveShl1 = 0x130;   // lane t reads lane t+1
+constexpr int kDppWaveShr1 = 0x138;   // lane t reads lane t-1
+
+template <int Ctrl>
+__device__ __forceinline__ double dpp_f64(double v) {
+  const unsigned long long u = __builtin_bit_cast(unsigned long long, v);
+  const int lo = __builtin_amdgcn_update_dpp(0, (int)(unsigned)u, Ctrl, 0xF, 0xF, false);
+  const int hi = __builtin_amdgcn_update_dpp(0, (int)(unsigned)(u >> 32), Ctrl, 0xF, 0xF, false);
+  return __builtin_bit_cast(double, ((unsigned long long)(unsigned)hi << 32) | (unsigned)lo);
+}
+
+template <class X>
+struct XLane {
+  const X &xf;
+  double own, west, east;
+  int i;
+  bool wok, eok;
+  __device__ __forceinline__ double operator()(int j) const {
+    if (j == i) return own;
+    if (wok && j == i - 1) return west;
+    if (eok && j == i + 1) return east;
+    return xf(j);
+  }
+};
+
+// Compile-time switch of a row sweep: with or without the exchange.
+template <bool B>
+struct LaneX {
+  static constexpr bool value = B;
+};
+
+// xf(own row i) = own; i1 = the row end of this pass.
+template <class X>
+__device__ __forceinline__ XLane<X> lane_exchange(const X &xf, double own, int i, int i1) {
+  const int lane = (int)(threadIdx.x & (kWave - 1));
+  return XLane<X>{xf, own, dpp_f64<kDppWaveShr1>(own), dpp_f64<kDppWaveShl1>(own), i, lane > 0,
+                  lane < kWave - 1 && i + 1 < i1};
+}
+
 // ------------------------------------------------------------- reductions --
 
 __device__ __forceinline__ double wave_sum(double v) {

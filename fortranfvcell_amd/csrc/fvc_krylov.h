@@ -348,6 +348,7 @@ struct Cg1Args {
   double *r0, *r1, *w0, *w1, *s0, *s1, *p;
   double tol;
   int maxit, x0_zero, stop_rule, phase;
+  int lanex;   // west/east neighbour operands from the adjacent lanes (lane_exchange)
   Defer d;
 };
 
@@ -415,15 +416,17 @@ __device__ void cg1_rows(const Cg1Args &a, int ph, const KState *S, double *v,
     const double *rc = odd ? a.r0 : a.r1, *wc = odd ? a.w0 : a.w1, *sc = odd ? a.s0 : a.s1;
     double *rn = odd ? a.r1 : a.r0, *wn = odd ? a.w1 : a.w0, *sn = odd ? a.s1 : a.s0;
     const XCgNext xf{rc, wc, sc, a.m, ld_dbl(&S->alpha), ld_dbl(&S->beta), j == 1};
-    auto row = [&](int i, const Cg1Row<F> &q) {
+    auto row = [&](int i, const Cg1Row<F> &q, auto lx) {
       const double mi = a.m[i];
       const double ui = rc[i] / mi;
       const double pi = xf.first ? ui : ui + xf.beta * a.p[i];
       const double si = xf.first ? wc[i] : wc[i] + xf.beta * sc[i];
       a.x[i] = a.x[i] + xf.alpha * pi;
       const double ri = rc[i] - xf.alpha * si;
-      const double un = ri / mi;
-      const double wi = q.mr.apply(a.A, i, xf);
+      const double un = ri / mi;   // = xf(i)
+      double wi;
+      if constexpr (decltype(lx)::value) wi = q.mr.apply(a.A, i, lane_exchange(xf, un, i, i1));
+      else wi = q.mr.apply(a.A, i, xf);
       a.p[i] = pi;
       sn[i] = si;
       rn[i] = ri;
@@ -432,12 +435,16 @@ __device__ void cg1_rows(const Cg1Args &a, int ph, const KState *S, double *v,
       v[1] += wi * un;
       v[2] += ri * ri;
     };
-    int i = i0;
-    if (pre && pre->ok && i < i1) {
-      row(i, *pre);
-      i += is;
-    }
-    for (; i < i1; i += is) row(i, cg1_load<F>(a, i));
+    auto sweep = [&](auto lx) {
+      int i = i0;
+      if (pre && pre->ok && i < i1) {
+        row(i, *pre, lx);
+        i += is;
+      }
+      for (; i < i1; i += is) row(i, cg1_load<F>(a, i), lx);
+    };
+    if (a.lanex) sweep(LaneX<true>{});
+    else sweep(LaneX<false>{});
   } else {
     // CG1_FINAL: energy norm x.Ax (src/Iterativos.f:82-83) and ||b - Ax||^2
     for (int i = i0; i < i1; i += is) {
@@ -795,6 +802,7 @@ struct Bicg2Args {
   double *s, *t, *r;        // half-step residual, A M^-1 s, shadow residual
   double tol;
   int maxit, x0_zero, stop_rule, phase;
+  int lanex;   // west/east neighbour operands from the adjacent lanes (lane_exchange)
   Defer d;
 };
 
@@ -872,7 +880,7 @@ __device__ void bicg2_rows(const Bicg2Args &a, int ph, const KState *S, double *
     } else {
       const double alpha = ld_dbl(&S->alpha);
       const XBi2Dir xf{a.s, a.t, pc, vc, a.m, ld_dbl(&S->beta), ld_dbl(&S->omega)};
-      auto row = [&](int i, const Bi2Row<F> &q) {
+      auto row = [&](int i, const Bi2Row<F> &q, auto lx) {
         const double mi = a.m[i];
         const double si = a.s[i];
         const double z2i = si / mi;
@@ -880,27 +888,35 @@ __device__ void bicg2_rows(const Bicg2Args &a, int ph, const KState *S, double *
         a.x[i] = (a.x[i] + alpha * (pc[i] / mi)) + xf.w * z2i;
         a.bb[i] = bi;
         const double pi = bi + xf.beta * (pc[i] - xf.w * vc[i]);
-        const double vi = q.mr.apply(a.A, i, xf);
+        double vi;
+        if constexpr (decltype(lx)::value) vi = q.mr.apply(a.A, i, lane_exchange(xf, pi / mi, i, i1));
+        else vi = q.mr.apply(a.A, i, xf);
         pn[i] = pi;
         vn[i] = vi;
         v[0] += vi * a.r[i];
         v[1] += bi * z2i;
         v[2] += bi * bi;
       };
-      int i = i0;
-      if (pre && pre->ok && i < i1) {
-        row(i, *pre);
-        i += is;
-      }
-      for (; i < i1; i += is) row(i, bicg2_load<F>(a, i));
+      auto sweep = [&](auto lx) {
+        int i = i0;
+        if (pre && pre->ok && i < i1) {
+          row(i, *pre, lx);
+          i += is;
+        }
+        for (; i < i1; i += is) row(i, bicg2_load<F>(a, i), lx);
+      };
+      if (a.lanex) sweep(LaneX<true>{});
+      else sweep(LaneX<false>{});
     }
     return;
   }
   // B2_Q
   const XBiRes xf{a.bb, vn, a.m, ld_dbl(&S->alpha)};
-  auto row = [&](int i, const Bi2Row<F> &q) {
+  auto row = [&](int i, const Bi2Row<F> &q, auto lx) {
     const double si = a.bb[i] - xf.alpha * vn[i];
-    const double ti = q.mr.apply(a.A, i, xf);
+    double ti;
+    if constexpr (decltype(lx)::value) ti = q.mr.apply(a.A, i, lane_exchange(xf, si / a.m[i], i, i1));
+    else ti = q.mr.apply(a.A, i, xf);
     const double ri = a.r[i];
     a.s[i] = si;
     a.t[i] = ti;
@@ -909,12 +925,16 @@ __device__ void bicg2_rows(const Bicg2Args &a, int ph, const KState *S, double *
     v[2] += ri * si;
     v[3] += ri * ti;
   };
-  int i = i0;
-  if (pre && pre->ok && i < i1) {
-    row(i, *pre);
-    i += is;
-  }
-  for (; i < i1; i += is) row(i, bicg2_load<F>(a, i));
+  auto sweep = [&](auto lx) {
+    int i = i0;
+    if (pre && pre->ok && i < i1) {
+      row(i, *pre, lx);
+      i += is;
+    }
+    for (; i < i1; i += is) row(i, bicg2_load<F>(a, i), lx);
+  };
+  if (a.lanex) sweep(LaneX<true>{});
+  else sweep(LaneX<false>{});
 }
 
 __device__ inline void bicg2_epi(const Bicg2Args &a, int ph, KState *st, const double *t) {
@@ -1029,8 +1049,10 @@ __global__ __launch_bounds__(1024) void bicg2_kernel(Bicg2Args a) {
 // ---------------------------------------------------------------- GMRES --
 
 constexpr int kMaxGmresK = 256;
+constexpr int kCgsWide = 32;   // CGS2: largest restart length
+constexpr int kCgsChunk = 16;  // CGS2: dot products reduced together
 
-enum { GM_START = 0, GM_MV = 1, GM_MGS = 2, GM_XUPD = 3, GM_FINAL = 4 };
+enum { GM_START = 0, GM_MV = 1, GM_MGS = 2, GM_XUPD = 3, GM_FINAL = 4, GM_C1 = 5, GM_C2 = 6, GM_C3 = 7 };
 
 struct GmresArgs {
   Mat A;
@@ -1049,25 +1071,176 @@ struct GmresArgs {
   double tol;
   int k, maxit, phase;
   int stop_rule;   // 0 the reference's test on the preconditioned residual, 1 the true residual
+  int orth;        // 0 modified Gram-Schmidt (the reference's order), 1 CGS2
+  int chunk;       // CGS2, one kernel per phase: the chunk of dot products
   size_t ld;       // leading dimension of gv (= neq)
 };
 
-// src/Iterativos.f:271-446: GMRES(k), left Jacobi, MGS, Givens.  Column c
-// (1-based) of g is gv + (c-1)*ld.  A column is stored unnormalised until the
-// first MGS phase of the step that uses it; until then its readers divide by
-// its norm on the fly, which is the same rounding as the reference's stored
-// g(i,c)/norm.
-//
-// Stopping.  The reference ends a cycle, and the solve, when the
-// preconditioned residual |e(i+1)| = ||M^-1 (b - A x)|| falls to tol ||b||
-// (src/Iterativos.f:295-296,376,402): a mixed norm, which at 1M cells leaves
-// the true relative residual ~75x above tol.  With stop_rule 1 the start of
-// every cycle, which forms r = b - A x anyway, also reduces ||r||^2: the solve
-// ends there when ||r|| <= tol ||b||.  Otherwise the cycle runs until the
-// preconditioned residual has dropped by the factor the true one still needs,
-// |e(i+1)| <= e(1) tol ||b|| / ||r||, and the next cycle's start checks again.
+// The end of Arnoldi step i once h(1..i+1, i) is known: the previous Givens
+// rotations applied to column i, the new one, e, the convergence test and,
+// when the cycle ends, the back substitution (src/Iterativos.f:363-389).
+// Runs on one thread.
+__device__ __forceinline__ void gmres_close_step(const GmresArgs &a, KState *st, int i) {
+  const int k = a.k;
+  auto H = [&](int r, int c) -> double & { return a.h[(size_t)(c - 1) * (k + 1) + (r - 1)]; };
+  for (int jj = 1; jj <= i - 1; ++jj) {
+    const double aux1 = a.cs[jj] * H(jj, i) + a.sn[jj] * H(jj + 1, i);
+    const double aux2 = -a.sn[jj] * H(jj, i) + a.cs[jj] * H(jj + 1, i);
+    H(jj, i) = aux1;
+    H(jj + 1, i) = aux2;
+  }
+  const double rr = sqrt(H(i, i) * H(i, i) + H(i + 1, i) * H(i + 1, i));
+  a.cs[i] = H(i, i) / rr;
+  a.sn[i] = H(i + 1, i) / rr;
+  H(i, i) = rr;
+  H(i + 1, i) = 0.0;
+  a.e[i + 1] = -a.sn[i] * a.e[i];
+  a.e[i] = a.cs[i] * a.e[i];
+  st->inner = st->inner + 1;
+  const bool conv = fabs(a.e[i + 1]) <= st->econv;   // src/Iterativos.f:376
+  if (conv || i == k || !finite(a.e[i + 1])) {
+    const int nI = i;
+    st->ni = nI;
+    st->inner_done = 1;
+    // back substitution h y = e (src/Iterativos.f:382-389)
+    a.y[nI] = a.e[nI] / H(nI, nI);
+    for (int ii = nI - 1; ii >= 1; --ii) {
+      double yi = 0.0;
+      for (int jj = ii + 1; jj <= nI; ++jj) yi = yi - H(ii, jj) * a.y[jj];
+      a.y[ii] = (yi + a.e[ii]) / H(ii, ii);
+    }
+  } else {
+    st->step_i = i + 1;
+    st->mgs_j = 1;
+  }
+}
+
+// Classical Gram-Schmidt with one re-orthogonalisation ("twice is enough",
+// Giraud, Langou, Rozloznik 2005): per Arnoldi step i two passes of dot
+// products against columns 1..i and one norm, instead of the i+1 sequential
+// reductions of modified Gram-Schmidt:
+//   C1  w = A (g_i / nrm_i) / m into column i+1;  h1_j = w . g_j, j <= i
+//   C2  column i stored normalised; w -= sum_j h1_j g_j;  h2_j = w . g_j
+//   C3  w -= sum_j h2_j g_j;  ||w||;  h(j,i) = h1_j + h2_j; Givens as MGS
+// The dot products of a pass are reduced kCgsChunk at a time (chunk c covers
+// columns c*kCgsChunk+1 ..), so a step takes 2 ceil(i/kCgsChunk) + 1 grid
+// reductions and no thread holds more than kCgsChunk partial sums.  Chunk 0
+// of C1 also does the SpMV and of C2 the first projection; later chunks read
+// w back (own rows).  h2 waits in y, which is free until the cycle's back
+// substitution.
 template <int F>
-__device__ bool gmres_phase(const GmresArgs &a, int ph, double *lds) {
+__device__ __forceinline__ bool gmres_cgs_phase(const GmresArgs &a, int ph, int c, double *lds) {
+  const int i0 = a.g.first(), i1 = a.g.row_end(), is = a.g.stride();
+  KState *st = a.st;
+  const int k = a.k;
+  const int i = ld_int(&st->step_i);
+  const double ni = ld_dbl(&a.nrm[i]);
+  double *gi = a.gv + (size_t)(i - 1) * a.ld;
+  double *gn = a.gv + (size_t)i * a.ld;
+  auto H = [&](int r, int col) -> double & { return a.h[(size_t)(col - 1) * (k + 1) + (r - 1)]; };
+  // Row r of columns p0+1 .. p0+kCgsLoads, loaded together: a column past i
+  // reads column i again (an L1 hit), so no load waits behind a branch.
+  constexpr int kCgsLoads = 8;
+  auto load_group = [&](double *gq, int p0, int r) {
+#pragma unroll
+    for (int q = 0; q < kCgsLoads; ++q) {
+      const int col = min(p0 + q + 1, i);
+      gq[q] = a.gv[(size_t)(col - 1) * a.ld + r];
+    }
+  };
+  const int q0 = c * kCgsChunk;
+  // w . g_col for the chunk's columns; column i is stored normalised from C2 on
+  auto dots = [&](double *v, double w, int r) {
+#pragma unroll
+    for (int u = 0; u < kCgsChunk; u += kCgsLoads) {
+      if (q0 + u >= i) break;
+      double gq[kCgsLoads];
+      load_group(gq, q0 + u, r);
+#pragma unroll
+      for (int q = 0; q < kCgsLoads; ++q) {
+        const int col = q0 + u + q + 1;
+        const double g = (ph == GM_C1 && col == i) ? gq[q] / ni : gq[q];
+        if (col <= i) v[u + q] += w * g;
+      }
+    }
+  };
+  // w - h_1 g_1 - h_2 g_2 - ... - h_i g_i, in that order; gir = g_i (normalised)
+  auto project = [&](double w, const double *h, double gir, int r) {
+    for (int p0 = 0; p0 < i; p0 += kCgsLoads) {
+      double gq[kCgsLoads];
+      load_group(gq, p0, r);
+#pragma unroll
+      for (int q = 0; q < kCgsLoads; ++q) {
+        const int col = p0 + q + 1;
+        if (col <= i) w = w - h[col - 1] * (col == i ? gir : gq[q]);
+      }
+    }
+    return w;
+  };
+  double v[kCgsChunk];
+#pragma unroll
+  for (int q = 0; q < kCgsChunk; ++q) v[q] = 0.0;
+  if (ph == GM_C1 || (ph == GM_C2 && c > 0)) {
+    if (ph == GM_C1 && c == 0) {
+      const XScaled xf{gi, ni};
+      for (int r = i0; r < i1; r += is) {
+        const double w = row_apply<F>(a.A, r, xf) / a.m[r];
+        gn[r] = w;
+        dots(v, w, r);
+      }
+    } else {
+      for (int r = i0; r < i1; r += is) dots(v, gn[r], r);
+    }
+    return grid_sync<kCgsChunk>(a.g, v, lds, [&](const double *t) {
+#pragma unroll
+      for (int q = 0; q < kCgsChunk; ++q) {
+        if (q0 + q < i) {
+          if (ph == GM_C1) H(q0 + q + 1, i) = t[q];
+          else a.y[q0 + q + 1] = t[q];
+        }
+      }
+    });
+  }
+  // the projection coefficients of C2 (h1) and C3 (h2), shared by the block
+  __shared__ double h[kCgsWide];
+  {
+    const double *src = ph == GM_C2 ? a.h + (size_t)(i - 1) * (k + 1) : a.y + 1;
+    for (int q = (int)threadIdx.x; q < i; q += (int)blockDim.x) h[q] = ld_dbl(src + q);
+    __syncthreads();
+  }
+  if (ph == GM_C2) {
+    // chunk 0: normalise column i, project, first dot products
+    for (int r = i0; r < i1; r += is) {
+      const double gir = gi[r] / ni;
+      gi[r] = gir;
+      const double w = project(gn[r], h, gir, r);
+      gn[r] = w;
+      dots(v, w, r);
+    }
+    return grid_sync<kCgsChunk>(a.g, v, lds, [&](const double *t) {
+#pragma unroll
+      for (int q = 0; q < kCgsChunk; ++q)
+        if (q < i) a.y[q + 1] = t[q];
+    });
+  }
+  // GM_C3
+  double nv[1] = {0.0};
+  for (int r = i0; r < i1; r += is) {
+    const double w = project(gn[r], h, gi[r], r);
+    gn[r] = w;
+    nv[0] += w * w;
+  }
+  return grid_sync<1>(a.g, nv, lds, [&](const double *t) {
+    for (int q = 1; q <= i; ++q) H(q, i) = H(q, i) + a.y[q];
+    const double norm = sqrt(t[0]);
+    H(i + 1, i) = norm;
+    a.nrm[i + 1] = norm;
+    gmres_close_step(a, st, i);
+  });
+}
+
+template <int F>
+__device__ __forceinline__ bool gmres_phase(const GmresArgs &a, int ph, double *lds) {
   const int i0 = a.g.first(), i1 = a.g.row_end(), is = a.g.stride();
   KState *st = a.st;
   const int k = a.k;
@@ -1187,36 +1360,7 @@ __device__ bool gmres_phase(const GmresArgs &a, int ph, double *lds) {
       const double norm = sqrt(t[0]);
       H(i + 1, i) = norm;
       a.nrm[i + 1] = norm;
-      for (int jj = 1; jj <= i - 1; ++jj) {
-        const double aux1 = a.cs[jj] * H(jj, i) + a.sn[jj] * H(jj + 1, i);
-        const double aux2 = -a.sn[jj] * H(jj, i) + a.cs[jj] * H(jj + 1, i);
-        H(jj, i) = aux1;
-        H(jj + 1, i) = aux2;
-      }
-      const double rr = sqrt(H(i, i) * H(i, i) + H(i + 1, i) * H(i + 1, i));
-      a.cs[i] = H(i, i) / rr;
-      a.sn[i] = H(i + 1, i) / rr;
-      H(i, i) = rr;
-      H(i + 1, i) = 0.0;
-      a.e[i + 1] = -a.sn[i] * a.e[i];
-      a.e[i] = a.cs[i] * a.e[i];
-      st->inner = st->inner + 1;
-      const bool conv = fabs(a.e[i + 1]) <= st->econv;   // src/Iterativos.f:376
-      if (conv || i == k || !finite(a.e[i + 1])) {
-        const int nI = i;
-        st->ni = nI;
-        st->inner_done = 1;
-        // back substitution h y = e (src/Iterativos.f:382-389)
-        a.y[nI] = a.e[nI] / H(nI, nI);
-        for (int ii = nI - 1; ii >= 1; --ii) {
-          double yi = 0.0;
-          for (int jj = ii + 1; jj <= nI; ++jj) yi = yi - H(ii, jj) * a.y[jj];
-          a.y[ii] = (yi + a.e[ii]) / H(ii, ii);
-        }
-      } else {
-        st->step_i = i + 1;
-        st->mgs_j = 1;
-      }
+      gmres_close_step(a, st, i);
     });
   }
   if (ph == GM_XUPD) {
@@ -1257,9 +1401,14 @@ __device__ bool gmres_phase(const GmresArgs &a, int ph, double *lds) {
   });
 }
 
-template <int F>
-__global__ __launch_bounds__(1024) void gmres_persistent(GmresArgs a) {
-  __shared__ double lds[kMaxK * kMaxWaves];
+// One persistent launch per solve.  Orth 0: modified Gram-Schmidt; 1: CGS2,
+// whose 16 partial sums per pass would not fit the registers of 1024 threads
+// (it spilled there), so it runs 512 per block, at most 128 registers so
+// that two blocks share a CU.
+template <int F, int Orth>
+__global__ __launch_bounds__(Orth == 1 ? 512 : 1024) __attribute__((amdgpu_waves_per_eu(Orth == 1 ? 4 : 1)))
+void gmres_persistent(GmresArgs a) {
+  __shared__ double lds[(Orth == 1 ? kCgsChunk : kMaxK) * kMaxWaves];
   __shared__ unsigned s_ep;
   if (threadIdx.x == 0) s_ep = 0u;
   __syncthreads();
@@ -1268,9 +1417,18 @@ __global__ __launch_bounds__(1024) void gmres_persistent(GmresArgs a) {
     if (!gmres_phase<F>(a, GM_START, lds)) return;
     if (ld_int(&a.st->done)) break;
     for (int i = 1; i <= a.k; ++i) {
-      if (!gmres_phase<F>(a, GM_MV, lds)) return;
-      for (int j = 1; j <= i; ++j)
-        if (!gmres_phase<F>(a, GM_MGS, lds)) return;
+      if constexpr (Orth == 1) {
+        const int nch = (i + kCgsChunk - 1) / kCgsChunk;
+        for (int c = 0; c < nch; ++c)
+          if (!gmres_cgs_phase<F>(a, GM_C1, c, lds)) return;
+        for (int c = 0; c < nch; ++c)
+          if (!gmres_cgs_phase<F>(a, GM_C2, c, lds)) return;
+        if (!gmres_cgs_phase<F>(a, GM_C3, 0, lds)) return;
+      } else {
+        if (!gmres_phase<F>(a, GM_MV, lds)) return;
+        for (int j = 1; j <= i; ++j)
+          if (!gmres_phase<F>(a, GM_MGS, lds)) return;
+      }
       if (ld_int(&a.st->inner_done)) break;
     }
     if (!gmres_phase<F>(a, GM_XUPD, lds)) return;
@@ -1278,62 +1436,19 @@ __global__ __launch_bounds__(1024) void gmres_persistent(GmresArgs a) {
   gmres_phase<F>(a, GM_FINAL, lds);
 }
 
-template <int F>
+template <int F, int Orth>
 __global__ __launch_bounds__(1024) void gmres_kernel(GmresArgs a) {
-  __shared__ double lds[kMaxK * kMaxWaves];
+  __shared__ double lds[(Orth == 1 ? kCgsChunk : kMaxK) * kMaxWaves];
   const int ph = a.phase;
   if (ph != GM_FINAL && ld_int(&a.st->done)) return;
-  if ((ph == GM_MV || ph == GM_MGS) && ld_int(&a.st->inner_done)) return;
-  gmres_phase<F>(a, ph, lds);
-}
-
-// ---------------------------------------------------------- plain kernels --
-
-template <int F>
-__global__ __launch_bounds__(256) void spmv_kernel(Mat A, const double *__restrict__ x,
-                                                   double *__restrict__ y) {
-  const int i = xcd_block() * (int)blockDim.x + (int)threadIdx.x;
-  if (i < A.neq) y[i] = row_apply<F>(A, i, XVec{x});
-}
-
-// a.b over G chunks, reduced by the last block into *out (device).
-__global__ __launch_bounds__(1024) void dot_kernel(Grid g, const double *a, const double *b,
-                                                   double *out) {
-  __shared__ double lds[kMaxK * kMaxWaves];
-  const int i0 = g.first(), i1 = g.row_end(), is = g.stride();
-  double v[1] = {0.0};
-  for (int i = i0; i < i1; i += is) v[0] += a[i] * b[i];
-  grid_reduce<1>(g, v, lds, [&](const double *t) { *out = t[0]; });
-}
-
-// Resets the barrier counters and the solver state (a kernel rather than a
-// memset, so that the sequence can be captured in a graph).
-__global__ void reset_kernel(unsigned *ctr, int nctr, KState *st, int nst = 1) {
-  for (int i = threadIdx.x; i < nctr; i += blockDim.x) ctr[i] = 0u;
-  for (int i = threadIdx.x; i < nst; i += blockDim.x) {
-    KState z{};
-    z.status = S_MAXIT;
-    st[i] = z;
+  if ((ph == GM_MV || ph == GM_MGS || ph >= GM_C1) && ld_int(&a.st->inner_done)) return;
+  if constexpr (Orth == 1) {
+    if (ph >= GM_C1) {
+      gmres_cgs_phase<F>(a, ph, a.chunk, lds);
+      return;
+    }
   }
-}
-
-// The Jacobi diagonal of a full CSR matrix (diagonal stored in the row).
-__global__ void csr_diag_kernel(int neq, const int *ia, const int *ja, const double *a,
-                                double *m) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= neq) return;
-  double d = 0.0;
-  for (int k = ia[i] - 1; k < ia[i + 1] - 1; ++k)
-    if (ja[k] - 1 == i) d = a[k];
-  m[i] = d;
-}
-
-// getRes (src/Solver.f:368-391)
-__global__ void getres_kernel(double *u, double *x, const int *num, int numel, int flag) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= numel) return;
-  if (flag) x[num[i] - 1] = u[i];
-  else u[i] = x[num[i] - 1];
+  gmres_phase<F>(a, ph, lds);
 }
 
 }  // namespace fvc

@@ -14,9 +14,8 @@
 #include <vector>
 
 #include "../../include/fvcell.h"
-#include "fvc_krylov.h"
-#include "fvc_mixed.h"
-#include "fvc_batch.h"
+#include "fvc_instances.h"   // the solver kernels, built in k_*.hip
+#include "fvc_plain.h"
 #include "fvc_adjoint.h"
 #include "fvc_post.h"
 
@@ -281,6 +280,8 @@ struct SolveIn {
   double tol;
   int maxit, x0_zero, stop_rule, gmres_k, engine, bs, G_req, precision, barrier, recurrence;
   int row_order;          // FVC options' row_order: 0 rule below, 1 chunks, 2 XCD tiles
+  int lanex;              // fused PCG / BiCGSTAB: neighbour operands from adjacent lanes
+  int orth;               // GMRES: FVC_ORTH_*
   int refresh_every;      // mixed: FP32 iterations between FP64 residual recomputations, at most
   double refresh_drop;    // mixed: residual drop that triggers one
 };
@@ -393,7 +394,10 @@ int run_solve(DevState &d, const SolveIn &in, fvc_stats *stats, hipStream_t s) {
   // GMRES(30): 17.0 ms against 22.7 at 2 x 512 per CU, 17.9 at 128 blocks,
   // 19.1 at 256).
   const bool gmres_p = persistent && in.code == FVC_GMRES;
-  const int bs = in.bs > 0 ? in.bs : gmres_p ? 1024 : persistent ? kDefaultBlock : kernels_block(in.code, fused);
+  const bool cgs = in.code == FVC_GMRES && in.orth == FVC_ORTH_CGS2;
+  // CGS2 holds 16 partial sums per thread: 512 threads per block (see gmres_persistent)
+  const int bs = in.bs > 0 ? in.bs : gmres_p ? (cgs ? 512 : 1024) : persistent ? kDefaultBlock
+                                                                   : kernels_block(in.code, fused);
   if (bs % 64 != 0 || bs > 1024) return fail(FVC_BAD_ARGUMENT, "block size %d must be a multiple of 64 <= 1024", bs);
   const void *pfn = nullptr, *kfn = nullptr;
   if (in.code == FVC_PCG && fused) {
@@ -408,15 +412,24 @@ int run_solve(DevState &d, const SolveIn &in, fvc_stats *stats, hipStream_t s) {
   } else if (in.code == FVC_BICGSTAB) {
     pfn = (const void *)&bicg_persistent<F>;
     kfn = (const void *)&bicg_kernel<F>;
+  } else if (cgs) {
+    pfn = (const void *)&gmres_persistent<F, 1>;
+    kfn = (const void *)&gmres_kernel<F, 1>;
   } else {
-    pfn = (const void *)&gmres_persistent<F>;
-    kfn = (const void *)&gmres_kernel<F>;
+    pfn = (const void *)&gmres_persistent<F, 0>;
+    kfn = (const void *)&gmres_kernel<F, 0>;
   }
   int occ = 0;
   int rc = occupancy(pfn, bs, &occ);
   if (rc) return rc;
   if (occ < 1) return fail(FVC_DEVICE_ERROR, "persistent kernel does not fit a CU at block size %d", bs);
-  const int G_req = (gmres_p && in.G_req == 0) ? std::max(8, (3 * d.cus / 4) & ~7) : in.G_req;
+  // GMRES, persistent: MGS on three quarters of the CUs (above); CGS2, with
+  // three to five reductions per Arnoldi step instead of j + 1, three blocks
+  // of 512 per two CUs, and no more blocks than rows / 512 (1M cells,
+  // GMRES(30) to true 1e-10: 19.8 ms against 20.0 at one block per CU, 21.8
+  // at two, 23.7 for MGS)
+  const int G_cgs = std::max(8, std::min(3 * d.cus / 2, (neq / 512) & ~7));
+  const int G_req = (gmres_p && in.G_req == 0) ? (cgs ? G_cgs : std::max(8, (3 * d.cus / 4) & ~7)) : in.G_req;
   // Fused BiCGSTAB from 24 rows per thread on: two blocks per CU (MI355X,
   // us per iteration at 1024 -> 512 blocks: 8.4M cells 491 -> 464, 16.7M
   // 1040 -> 945; equal at 4M).  PCG stays at four (16.7M: 553 against 722).
@@ -431,12 +444,16 @@ int run_solve(DevState &d, const SolveIn &in, fvc_stats *stats, hipStream_t s) {
   const int k = in.gmres_k;
   if (in.code == FVC_GMRES && (k < 1 || k > kMaxGmresK))
     return fail(FVC_BAD_ARGUMENT, "GMRES restart k = %d outside [1, %d]", k, kMaxGmresK);
+  if (cgs && k > kCgsWide)
+    return fail(FVC_UNSUPPORTED, "GMRES with CGS2 orthogonalisation takes k <= %d (asked %d)", kCgsWide, k);
+  // partial sums per block: CGS2 reduces kCgsChunk dot products at once
+  const size_t npart = (size_t)(cgs ? kCgsChunk : kMaxK) * G;
 
   // workspace
   const size_t n = (size_t)neq;
   int nvec = in.code == FVC_PCG ? (fused ? 7 : 5) : in.code == FVC_BICGSTAB ? (fused ? 8 : 10) : (k + 1);
   size_t bytes = align_up((kCtrWords + (size_t)G) * sizeof(unsigned)) + align_up(sizeof(KState)) +
-                 align_up(kMaxK * (size_t)G * sizeof(double)) + (size_t)nvec * align_up(n * sizeof(double)) +
+                 align_up(npart * sizeof(double)) + (size_t)nvec * align_up(n * sizeof(double)) +
                  align_up((size_t)(k + 2) * (k + 2) * sizeof(double)) * 2 + align_up(n * sizeof(double)) +
                  align_up(2 * sizeof(KState)) + align_up(2 * kMaxK * (size_t)G * sizeof(double)) + 4096;
   rc = ensure_arena(d, bytes);
@@ -444,7 +461,7 @@ int run_solve(DevState &d, const SolveIn &in, fvc_stats *stats, hipStream_t s) {
   Carve cv{d.arena};
   unsigned *ctr = cv.take<unsigned>(kCtrWords + (size_t)G);
   KState *st = cv.take<KState>(1);
-  double *partials = cv.take<double>(kMaxK * (size_t)G);
+  double *partials = cv.take<double>(npart);
   // deferred reductions (kernel-per-phase engine of the fused methods)
   Defer dfr{cv.take<KState>(2), cv.take<double>(2 * kMaxK * (size_t)G), 0};
   std::vector<double *> vec(nvec);
@@ -465,6 +482,7 @@ int run_solve(DevState &d, const SolveIn &in, fvc_stats *stats, hipStream_t s) {
     a.A = in.A; a.g = g; a.st = st; a.m = in.m; a.x = in.x; a.b = in.b;
     a.r0 = vec[0]; a.r1 = vec[1]; a.w0 = vec[2]; a.w1 = vec[3]; a.s0 = vec[4]; a.s1 = vec[5]; a.p = vec[6];
     a.tol = in.tol; a.maxit = in.maxit; a.x0_zero = in.x0_zero; a.stop_rule = in.stop_rule;
+    a.lanex = in.lanex;
     if (persistent) {
       a.phase = -1;
       if ((rc = launch(pfn, true, G, bs, a, s))) return rc;
@@ -496,6 +514,7 @@ int run_solve(DevState &d, const SolveIn &in, fvc_stats *stats, hipStream_t s) {
     a.bb = vec[0]; a.p0 = vec[1]; a.p1 = vec[2]; a.v0 = vec[3]; a.v1 = vec[4];
     a.s = vec[5]; a.t = vec[6]; a.r = vec[7];
     a.tol = in.tol; a.maxit = in.maxit; a.x0_zero = in.x0_zero; a.stop_rule = in.stop_rule;
+    a.lanex = in.lanex;
     if (persistent) {
       a.phase = -1;
       if ((rc = launch(pfn, true, G, bs, a, s))) return rc;
@@ -594,7 +613,7 @@ int run_solve(DevState &d, const SolveIn &in, fvc_stats *stats, hipStream_t s) {
     a.e = sm; sm += k + 2;
     a.y = sm; sm += k + 2;
     a.nrm = sm; sm += k + 2;
-    a.tol = in.tol; a.k = k; a.maxit = in.maxit; a.stop_rule = in.stop_rule;
+    a.tol = in.tol; a.k = k; a.maxit = in.maxit; a.stop_rule = in.stop_rule; a.orth = cgs ? 1 : 0;
     if (persistent) {
       a.phase = -1;
       if ((rc = launch(pfn, true, G, bs, a, s))) return rc;
@@ -606,6 +625,16 @@ int run_solve(DevState &d, const SolveIn &in, fvc_stats *stats, hipStream_t s) {
         a.phase = GM_START;
         if ((rc = launch(kfn, false, G, bs, a, s))) return rc;
         for (int i = 1; i <= k; ++i) {
+          if (cgs) {
+            const int nch = (i + kCgsChunk - 1) / kCgsChunk;
+            for (int ph = GM_C1; ph <= GM_C3; ++ph)
+              for (int c = 0; c < (ph == GM_C3 ? 1 : nch); ++c) {
+                a.phase = ph;
+                a.chunk = c;
+                if ((rc = launch(kfn, false, G, bs, a, s))) return rc;
+              }
+            continue;
+          }
           a.phase = GM_MV;
           if ((rc = launch(kfn, false, G, bs, a, s))) return rc;
           a.phase = GM_MGS;
@@ -1102,6 +1131,17 @@ int solve_impl(int code, int matriz, int unsym, int neq, int nad, const double *
   in.barrier = o.barrier;
   in.recurrence = o.recurrence;
   in.row_order = o.row_order;
+  if (o.lane_exchange < 0 || o.lane_exchange > 2)
+    return fail(FVC_BAD_ARGUMENT, "lane_exchange %d (0 default, 1 on, 2 off)", o.lane_exchange);
+  // Off by default: at 1M cells the exchange made BiCGSTAB 2.86 against 2.72 ms
+  // per solve and PCG 31.6 against 29.7 us per iteration on MI355X.  The
+  // west/east gathers it saves hit the L1 cache (adjacent rows), and the
+  // per-column selection splits a row's gathers into branches that no longer
+  // issue together.
+  in.lanex = o.lane_exchange == 1;
+  if (o.gmres_orth != FVC_ORTH_MGS && o.gmres_orth != FVC_ORTH_CGS2)
+    return fail(FVC_BAD_ARGUMENT, "gmres_orth %d (0 MGS, 1 CGS2)", o.gmres_orth);
+  in.orth = o.gmres_orth;
   if (o.row_order < 0 || o.row_order > 2)
     return fail(FVC_BAD_ARGUMENT, "row_order %d (0 default, 1 chunks, 2 XCD tiles)", o.row_order);
   in.refresh_every = o.refresh_every > 0 ? o.refresh_every : kRefreshEvery;

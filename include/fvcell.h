@@ -115,8 +115,22 @@ typedef struct fvc_options {
                              per block, 2 XCD tiles (each eighth of the rows swept
                              by one XCD's blocks together); single-system solves
                              (batches use chunks)                                    */
-  int32_t reserved[2];
+  int32_t lane_exchange;  /* fused PCG and BiCGSTAB: the SpMV operand of columns i-1
+                             and i+1 comes from the adjacent lanes of the wavefront
+                             (DPP wave shifts) instead of a gather of every vector it
+                             is formed from; bit-identical either way.  0 default
+                             (off: measured slower on MI355X), 1 on, 2 off           */
+  int32_t gmres_orth;     /* GMRES orthogonalisation: FVC_ORTH_MGS (default) or
+                             FVC_ORTH_CGS2                                           */
 } fvc_options;
+
+/* fvc_options.gmres_orth */
+#define FVC_ORTH_MGS 0          /* modified Gram-Schmidt in the reference's order
+                                   (src/Iterativos.f:343-349): j+1 grid reductions at
+                                   Arnoldi step j                                         */
+#define FVC_ORTH_CGS2 1         /* classical Gram-Schmidt with one re-orthogonalisation:
+                                   three grid reductions per step, the j dot products of
+                                   a pass reduced together; gmres_k <= 32                 */
 
 typedef struct fvc_stats {
   int32_t status;           /* FVC_* status                                          */

@@ -241,14 +241,17 @@ def ladder1024():
     np.savez_compressed(os.path.join(HERE, "ref_ladder_1024.npz"), **stats)
 
 
-def steady(n_bicg: int = 1024, n_gmres: int = 256):
+def steady(n_bicg: int = 256, n_gmres: int = 256):
     """The advection-diffusion system at the reference's defaults: no
     under-relaxation (underU = 1, src/Main.f:219) and no transient term, so
     a_P = sum a_nb + sum F_f.  BiCGSTAB at n_bicg^2 and GMRES(30) at
-    n_gmres^2 (the largest sizes the sequential reference finishes here in
-    minutes), at the ladder tolerance.  Written to ref_steady.npz."""
+    n_gmres^2, at the ladder tolerance.  Jacobi-BiCGSTAB breaks down on this
+    system from 512^2 on (rho or omega exactly zero after ~6e4 iterations),
+    and GMRES(30) needs ~4e3 restarts at 256^2 (~25 minutes per solve of the
+    sequential reference), so 256^2 is the largest size that pins both.
+    Each system is merged into ref_steady.npz as soon as it is done."""
     rng = np.random.default_rng(17)
-    stats = {}
+    path = os.path.join(HERE, "ref_steady.npz")
     for name, n, run in (
             (f"steady{n_gmres}_ell_gmres30", n_gmres,
              lambda f, t: _ref_call(lambda: oracle.krylov("gmres", f, k=30, tol=t, maxit=100000,
@@ -259,9 +262,10 @@ def steady(n_bicg: int = 1024, n_gmres: int = 256):
         s = P.config_advdiff(n, dt=None, under_relax=1.0)
         f = P.to_ell(s)
         tol, ((x, _), label, it, en) = _ladder(s, lambda t: run(f, t))
+        stats = dict(np.load(path, allow_pickle=False)) if os.path.exists(path) else {}
         stats.update(_stats(name, s, x, it, en, tol, rng))
+        np.savez_compressed(path, **stats)
         print(name, it, en, tol, _relres(s, x), label, flush=True)
-    np.savez_compressed(os.path.join(HERE, "ref_steady.npz"), **stats)
 
 
 if __name__ == "__main__":

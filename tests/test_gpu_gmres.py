@@ -33,7 +33,9 @@ def test_gmres_true_residual_small(fvc, key, layout):
     print(f"gmres relres {key} {layout} neq={s.neq}", flush=True)
     out = {}
     for engine in (fvc.ENGINE_PERSISTENT, fvc.ENGINE_KERNELS):
-        x, st = fvc.solve(f, fvc.GMRES, tol=1e-10, gmres_k=30, stop_rule=fvc.STOP_RELRES, engine=engine)
+        # one launch shape for both engines: the same reduction trees
+        x, st = fvc.solve(f, fvc.GMRES, tol=1e-10, gmres_k=30, stop_rule=fvc.STOP_RELRES, engine=engine,
+                          block_size=256, grid_blocks=64)
         r = _relres(s, x)
         print(f"  engine {engine}: {st['iterations']} cycles {st['inner_iterations']} steps relres {r:.2e} "
               f"(reported {st['true_relres']:.2e})", flush=True)
@@ -42,7 +44,7 @@ def test_gmres_true_residual_small(fvc, key, layout):
         assert abs(st["true_relres"] - r) <= 1e-3 * r
         assert np.linalg.norm(x - xref) <= 1e-8 * np.linalg.norm(xref)
         out[engine] = (x, st)
-    # both engines run the same phases with the same reduction trees
+    # both engines run the same phases with the same reduction trees on one grid
     (xp, sp), (xk, sk) = out[fvc.ENGINE_PERSISTENT], out[fvc.ENGINE_KERNELS]
     assert np.array_equal(xp, xk) and sp["inner_iterations"] == sk["inner_iterations"]
 
@@ -128,3 +130,62 @@ def test_1024_at_the_ladder_tolerance(fvc, ladder, advdiff1024, method):
           f"{st['seconds'] * 1e3:.1f} ms", flush=True)
     assert st["status"] == 0 and r <= 1e-10
     _check_stats(name, ladder, s, x, 1e-8)
+
+
+# ----------------------------------------------------------------- CGS2 --
+# Classical Gram-Schmidt with one re-orthogonalisation (FVC_ORTH_CGS2): three
+# grid reductions per Arnoldi step instead of j + 1.  Not the reference's
+# order, so it is held to the parity bar against the reference's solutions,
+# not to bit-identity.
+
+@pytest.mark.parametrize("key", [k for k in CONFIG_KEYS if k.endswith("gmres30")])
+def test_cgs2_small(fvc, key):
+    s, f = config_system(key)
+    xref = REF_FIXTURES[key + "_x"]
+    out = {}
+    for engine in (fvc.ENGINE_PERSISTENT, fvc.ENGINE_KERNELS):
+        x, st = fvc.solve(f, fvc.GMRES, tol=1e-10, gmres_k=30, stop_rule=fvc.STOP_RELRES, engine=engine,
+                          gmres_orth=fvc.ORTH_CGS2, block_size=256, grid_blocks=64)
+        r = _relres(s, x)
+        print(f"cgs2 {key} engine {engine}: {st['iterations']} cycles {st['inner_iterations']} steps "
+              f"relres {r:.2e}", flush=True)
+        assert st["status"] == 0 and r <= 1e-10
+        assert np.linalg.norm(x - xref) <= 1e-8 * np.linalg.norm(xref)
+        out[engine] = (x, st)
+    (xp, sp), (xk, sk) = out[fvc.ENGINE_PERSISTENT], out[fvc.ENGINE_KERNELS]
+    assert np.array_equal(xp, xk) and sp["inner_iterations"] == sk["inner_iterations"]
+    # the reference rule at the ladder tolerance: the reference's cycle count
+    i = CONFIG_KEYS.index(key)
+    tol = float(REF_FIXTURES["tol"][i])
+    x, st = fvc.solve(f, fvc.GMRES, tol=tol, gmres_k=30, gmres_orth=fvc.ORTH_CGS2)
+    it_ref = int(REF_FIXTURES["iterations"][i])
+    assert st["status"] == 0 and abs(st["iterations"] - it_ref) <= 1, (st["iterations"], it_ref)
+    assert np.linalg.norm(x - xref) <= 1e-8 * np.linalg.norm(xref)
+
+
+def test_cgs2_1024_true_residual(fvc, ladder, advdiff1024):
+    s, ds = advdiff1024
+    name = "advdiff1024_ell_gmres30"
+    st_m = ds.solve(fvc.GMRES, tol=1e-10, gmres_k=30, stop_rule=fvc.STOP_RELRES)
+    st = ds.solve(fvc.GMRES, tol=1e-10, gmres_k=30, stop_rule=fvc.STOP_RELRES, gmres_orth=fvc.ORTH_CGS2)
+    x = ds.x.get()
+    r = _relres(s, x)
+    print(f"cgs2 1024: {st['iterations']} cycles {st['inner_iterations']} steps {st['seconds'] * 1e3:.1f} ms "
+          f"relres {r:.2e}; mgs {st_m['inner_iterations']} steps {st_m['seconds'] * 1e3:.1f} ms", flush=True)
+    assert st["status"] == 0 and r <= 1e-10
+    assert abs(st["inner_iterations"] - st_m["inner_iterations"]) <= 2
+    _check_stats(name, ladder, s, x, 1e-8)
+
+
+def test_cgs2_limits(fvc):
+    s = P.config_advdiff(32)
+    with pytest.raises(fvc.FvcError):
+        fvc.solve(P.to_ell(s), fvc.GMRES, gmres_k=33, gmres_orth=fvc.ORTH_CGS2, raise_on_failure=True)
+    with pytest.raises(fvc.FvcError):
+        fvc.solve(P.to_ell(s), fvc.GMRES, gmres_k=10, gmres_orth=7, raise_on_failure=True)
+    # k = 32 on the steady system (many full cycles): every reduction width
+    # (8, 16, 32) in each cycle
+    s = P.config_advdiff(32, dt=None, under_relax=1.0)
+    x, st = fvc.solve(P.to_ell(s), fvc.GMRES, tol=1e-10, gmres_k=32, stop_rule=fvc.STOP_RELRES,
+                      gmres_orth=fvc.ORTH_CGS2)
+    assert st["status"] == 0 and _relres(s, x) <= 1e-10 and st["iterations"] > 1, st
