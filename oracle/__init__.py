@@ -13,6 +13,7 @@ from __future__ import annotations
 
 import ctypes as C
 import os
+import re
 import sys
 import subprocess
 
@@ -95,6 +96,7 @@ def ref(pool_words: int = 1 << 27, omp: int = 0, nthreads: int = 1):
         D, I, P = C.c_double, C.c_int32, C.c_void_p
         L.ref_init.argtypes = [C.c_int64, I, I]
         L.ref_open_log.argtypes = [C.c_char_p, I]
+        L.ref_close_log.argtypes = []
         L.ref_solver.argtypes = [P, P, P, P, P, P, P, I, I, I, D, I, I, I, I]
         L.ref_matvec.argtypes = [I, I, P, P, P, P, P, P, P]
         L.ref_dot.restype = D
@@ -131,6 +133,26 @@ def ref_set_omp(omp: int, nthreads: int) -> None:
 def ref_open_log(path: str) -> None:
     b = path.encode()
     ref().ref_open_log(b, len(b))
+
+
+# the solver's unit-13 line (src/Iterativos.f:91-93,216-218,429-431)
+LOG_RE = re.compile(r"^(.*?)it\s*(\d+) energy norm\s+(\S+) tol\s+(\S+)\s*$")
+
+
+def ref_logged(fn):
+    """Run fn() (a call into the reference library) with unit 13 pointed at a
+    temporary file; return (fn's result, label, iterations, energy) parsed
+    from the line the reference's solver wrote there."""
+    import tempfile
+    with tempfile.TemporaryDirectory() as td:
+        log = os.path.join(td, "solver.log")
+        ref_open_log(log)
+        out = fn()
+        ref().ref_close_log()
+        ref_open_log("/dev/null")
+        line = open(log).read().strip().splitlines()[-1]
+    m = LOG_RE.match(line)
+    return out, m.group(1).strip(), int(m.group(2)), float(m.group(3).replace("D", "E"))
 
 
 def _p(a):

@@ -15,9 +15,7 @@ entries).  Run in the development container (needs the reference build):
     python tests/golden/make_golden.py steady      # ref_steady.npz (tens of minutes)
 """
 import os
-import re
 import sys
-import tempfile
 
 import numpy as np
 
@@ -28,22 +26,11 @@ sys.path.insert(0, ROOT)
 import oracle  # noqa: E402
 from fortranfvcell_amd import problems as P  # noqa: E402
 
-LOG_RE = re.compile(r"^(.*?)it\s*(\d+) energy norm\s+(\S+) tol\s+(\S+)\s*$")
-
 
 def _ref_call(fn):
-    """Run fn() with unit 13 pointed at a temporary file; return the parsed
-    (label, iterations, energy) of the line it wrote."""
-    with tempfile.TemporaryDirectory() as td:
-        log = os.path.join(td, "solver.log")
-        oracle.ref_open_log(log)
-        out = fn()
-        oracle.ref().ref_close_log()
-        oracle.ref_open_log("/dev/null")
-        line = open(log).read().strip().splitlines()[-1]
-    m = LOG_RE.match(line)
-    label, it, en = m.group(1), int(m.group(2)), float(m.group(3).replace("D", "E"))
-    return out, label.strip(), it, en
+    """Run fn() with unit 13 pointed at a temporary file; return (fn's
+    result, label, iterations, energy) of the line the reference wrote."""
+    return oracle.ref_logged(fn)
 
 
 def small_cases():

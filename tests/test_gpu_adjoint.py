@@ -184,6 +184,49 @@ def test_adjoint_full_config_512(fvc):
     assert err_all <= 1e-8
 
 
+def test_adjoint_peclet100(fvc):
+    """The differentiation config with config 2's velocity field scaled to
+    its largest cell Peclet number, 100, at 128x128 cells: the largest mesh
+    on which Jacobi-BiCGSTAB solves this steady problem.  Measured on MI355X
+    (tol 1e-13, true residual): 128^2 forward 57,813 iterations, adjoint
+    1,976; 256^2 the forward goes non-finite after 1,523 iterations (the
+    adjoint converges in 6,067) and GMRES(30) reaches 9e-8 in 4,000 cycles;
+    512^2 the forward breaks down after 984 iterations, the adjoint after
+    10,823, GMRES(30) 1.4e-2 in 4,000 cycles.  So the 512^2 configuration runs
+    at Peclet 10 (test_adjoint_full_config_512) and the Peclet-100 field is
+    checked here: the adjoint against a sparse-LU FP64 adjoint on every cell
+    and the gradient against LU central differences on 12 cells."""
+    from fortranfvcell_amd.adjoint import HeatSolver
+    from heat_fd import fd_gradient_lu
+    op, k, subset = P.heat_problem(128, peclet_max=100.0)
+    solver = HeatSolver(op, tol=1e-13)
+    L0, gk, gq, T = _loss_grad(solver, k, subset)
+    print(f"Pe 100, 128^2: forward {solver.last_forward['iterations']} it, adjoint "
+          f"{solver.last_adjoint['iterations']} it", flush=True)
+    s = op.assemble(k)
+    A = P.scipy_csr(s).tocsc()
+    lu = sla.splu(A)
+    Tl = lu.solve(s.b)
+    for _ in range(2):
+        Tl = Tl + lu.solve(s.b - A @ Tl)
+    assert np.linalg.norm(T - Tl) <= 1e-9 * np.linalg.norm(Tl)
+    g = np.zeros(s.neq)
+    g[subset] = 2.0 * Tl[subset] / len(subset)
+    lam = lu.solve(g, trans="T")
+    for _ in range(2):
+        lam = lam + lu.solve(g - A.T @ lam, trans="T")
+    assert np.linalg.norm(gq - lam) <= 1e-8 * np.linalg.norm(lam)
+    gk_lu = op.grad_k(k, Tl, lam)
+    err_all = np.linalg.norm(gk - gk_lu) / np.linalg.norm(gk_lu)
+    print(f"  all cells vs sparse-LU adjoint: {err_all:.2e}", flush=True)
+    assert err_all <= 1e-8
+    cells = np.sort(np.random.default_rng(100).choice(s.neq, 12, replace=False))
+    fd = fd_gradient_lu(op, k, Tl, subset, cells)
+    rel = np.abs(gk[cells] - fd) / np.abs(fd)
+    print(f"  12 cells vs LU central differences: worst {rel.max():.1e}", flush=True)
+    assert np.all(rel <= 1e-5)
+
+
 def test_gpu_assembly_is_bit_identical_to_host(fvc):
     """fvc_assemble_heat_dev against problems.HeatOperator.assemble: the same
     coefficient forms in the same order, so equal to the last bit."""
