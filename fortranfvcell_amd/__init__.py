@@ -235,12 +235,15 @@ class DeviceSystem:
         self.x = DeviceArray.empty(self.neq)
 
     def solve(self, code: int, tol: float = 1e-10, maxit: int = 50000, x0=None, options=None,
-              stream=None, **opt) -> dict:
+              stream=None, noalloc: bool = False, **opt) -> dict:
+        """fvc_solve_dev on the resident arrays (fvc_solve_dev_noalloc with
+        noalloc: the workspace is the library's fixed per-device array)."""
         if x0 is not None:
             self.x.upload(np.asarray(x0, dtype=np.float64))
         o = options if options is not None else default_options(**opt)
         st = Stats()
-        rc = load().fvc_solve_dev(code, self.matriz, int(self.unsym), self.neq, self.nad,
+        fn = load().fvc_solve_dev_noalloc if noalloc else load().fvc_solve_dev
+        rc = fn(code, self.matriz, int(self.unsym), self.neq, self.nad,
                                   _ptr(self.ad), _ptr(self.au or self.ad), _ptr(self.al or self.ad),
                                   _ptr(self.x), _ptr(self.b), _ptr(self._iax_arg), _ptr(self.ja),
                                   float(tol), int(maxit), C.byref(o), C.byref(st), stream)

@@ -13,6 +13,7 @@ LIB_PATH = os.path.join(HERE, "libfvcell.so")
 FORTRAN_LIB_PATH = os.path.join(HERE, "libfvcell_fortran.a")
 
 FVC_OK, FVC_MAXIT, FVC_BREAKDOWN, FVC_NONFINITE, FVC_UNSUPPORTED, FVC_DEVICE_ERROR, FVC_BAD_ARGUMENT = range(7)
+NOALLOC_WORKSPACE_BYTES = 32 << 20   # FVC_NOALLOC_WORKSPACE_BYTES
 STATUS_NAMES = {0: "converged", 1: "maxit", 2: "breakdown", 3: "nonfinite", 4: "unsupported",
                 5: "device_error", 6: "bad_argument"}
 PCG, BICGSTAB, GMRES = 1, 2, 3
@@ -28,7 +29,7 @@ ORTH_MGS, ORTH_CGS2 = 0, 1
 # Every symbol include/fvcell.h declares.
 EXPORTS = [
     "fvc_default_options", "fvc_init", "fvc_last_error", "fvc_solver", "fvc_solve", "fvc_keep_matrix",
-    "fvc_solve_dev",
+    "fvc_solve_dev", "fvc_solve_dev_noalloc",
     "fvc_solve_batch", "fvc_solve_batch_dev", "fvc_ell_transpose_dev", "fvc_grad_k_dev",
     "fvc_assemble_heat_dev", "fvc_gform_dev", "fvc_simple_update_dev",
     "fvc_matvec", "fvc_matvec_dev", "fvc_dot", "fvc_dot_dev", "fvc_getres", "fvc_getres_dev",
@@ -86,6 +87,8 @@ def load(path: str = LIB_PATH):
     L.fvc_keep_matrix.argtypes = [I]
     L.fvc_solve_dev.argtypes = [I, I, I, I, I, P, P, P, P, P, P, P, D, I, C.POINTER(Options),
                                 C.POINTER(Stats), P]
+    L.fvc_solve_dev_noalloc.argtypes = [I, I, I, I, I, P, P, P, P, P, P, P, D, I, C.POINTER(Options),
+                                        C.POINTER(Stats), P]
     L.fvc_solve_batch.argtypes = [I, I, P, I, P, P, P, P, P, D, I, C.POINTER(Options), P]
     L.fvc_solve_batch_dev.argtypes = [I, I, P, I, P, P, P, P, P, D, I, C.POINTER(Options), P, P]
     L.fvc_ell_transpose_dev.argtypes = [I, I, P, P, P, P, P]

@@ -21,6 +21,14 @@
 
 using namespace fvc;
 
+// The workspace of fvc_solve_dev_noalloc: one fixed array per device, which
+// that device holds until the process ends.  Host code never passes this
+// name to a kernel or a copy (in host code it is only a stand-in for the
+// symbol); it uses the device address that hipGetSymbolAddress returned at
+// set-up (DevState::scratch_dev).
+constexpr size_t kScratchBytes = (size_t)FVC_NOALLOC_WORKSPACE_BYTES;
+static __device__ __align__(256) char scratch[kScratchBytes];
+
 namespace {
 
 thread_local std::string g_err;
@@ -104,6 +112,10 @@ struct DevState {
   const double *mat_m = nullptr;
   bool mat_valid = false;
   bool keep_next = false;
+  // fvc_solve_dev_noalloc: the device address of `scratch` on this device, and
+  // whether the call in progress must take its workspace from there
+  char *scratch_dev = nullptr;
+  bool noalloc = false;
 };
 
 DevState g_dev[64];
@@ -129,6 +141,8 @@ int current_device(DevState **out) {
       HIPCHK(hipHostMalloc((void **)&d.host_scalar, 64, hipHostMallocDefault));
       HIPCHK(hipEventCreate(&d.ev0));
       HIPCHK(hipEventCreate(&d.ev1));
+      // this device's copy of the fixed workspace (its own address per device)
+      HIPCHK(hipGetSymbolAddress((void **)&d.scratch_dev, HIP_SYMBOL(scratch)));
       d.ready = true;
     }
   }
@@ -162,6 +176,22 @@ int ensure_arena(DevState &d, size_t bytes) {
   HIPCHK(hipMalloc((void **)&d.arena, want));
   d.arena_bytes = want;
   return FVC_OK;
+}
+
+// The workspace of a solve: the grow-only arena, or for fvc_solve_dev_noalloc
+// the fixed per-device array, which must hold it.
+int workspace(DevState &d, size_t bytes, char **base) {
+  if (d.noalloc) {
+    if (bytes > kScratchBytes)
+      return fail(FVC_UNSUPPORTED,
+                  "fvc_solve_dev_noalloc: this solve needs %zu bytes of workspace, the fixed workspace "
+                  "holds %zu (FVC_NOALLOC_WORKSPACE_BYTES); use fvc_solve_dev", bytes, kScratchBytes);
+    *base = d.scratch_dev;
+    return FVC_OK;
+  }
+  const int rc = ensure_arena(d, bytes);
+  *base = d.arena;
+  return rc;
 }
 
 // ------------------------------------------------------------ matrix view --
@@ -456,15 +486,15 @@ int run_solve(DevState &d, const SolveIn &in, fvc_stats *stats, hipStream_t s) {
                  align_up(npart * sizeof(double)) + (size_t)nvec * align_up(n * sizeof(double)) +
                  align_up((size_t)(k + 2) * (k + 2) * sizeof(double)) * 2 + align_up(n * sizeof(double)) +
                  align_up(2 * sizeof(KState)) + align_up(2 * kMaxK * (size_t)G * sizeof(double)) + 4096;
-  rc = ensure_arena(d, bytes);
-  if (rc) return rc;
-  Carve cv{d.arena};
+  char *ws = nullptr;
+  if ((rc = workspace(d, bytes, &ws))) return rc;
+  Carve cv{ws};
   unsigned *ctr = cv.take<unsigned>(kCtrWords + (size_t)G);
   KState *st = cv.take<KState>(1);
   double *partials = cv.take<double>(npart);
   // deferred reductions (kernel-per-phase engine of the fused methods)
   Defer dfr{cv.take<KState>(2), cv.take<double>(2 * kMaxK * (size_t)G), 0};
-  std::vector<double *> vec(nvec);
+  double *vec[kMaxGmresK + 1];
   for (int i = 0; i < nvec; ++i) vec[i] = cv.take<double>(n);
   double *gsmall = cv.take<double>((size_t)(k + 2) * (k + 2) * 2);
   double *xbackup = cv.take<double>(n);
@@ -739,8 +769,9 @@ int run_mixed(DevState &d, const SolveIn &in, fvc_stats *stats, hipStream_t s) {
                    align_up(n * sizeof(double)) + align_up(n * sizeof(float)) + 2 * align_up(n * sizeof(float4)) +
                    align_up(2 * sizeof(KState)) + align_up(2 * kMaxK * (size_t)G * sizeof(double)) +
                    align_up(std::max<size_t>(n16, 1) * sizeof(short)) + 4096;
-    if ((rc = ensure_arena(d, bytes))) return rc;
-    Carve cv{d.arena};
+    char *ws = nullptr;
+    if ((rc = workspace(d, bytes, &ws))) return rc;
+    Carve cv{ws};
     // barrier counters, G arrival flags, then the offsets' "does not fit" flag
     unsigned *ctr = cv.take<unsigned>(kCtrWords + (size_t)G + 1);
     int *wide = reinterpret_cast<int *>(ctr + kCtrWords + G);
@@ -1052,18 +1083,40 @@ int build_device_matrix(int matriz, int unsym, int neq, int nad, const double *a
   return FVC_OK;
 }
 
+// Marks a device's solve as taking its workspace from the fixed array for the
+// duration of one call (under the device's mutex).
+struct NoallocScope {
+  DevState *d;
+  NoallocScope(DevState *dd, bool on) : d(dd) { d->noalloc = on; }
+  ~NoallocScope() { d->noalloc = false; }
+};
+
 int solve_impl(int code, int matriz, int unsym, int neq, int nad, const double *ad,
                const double *au, const double *al, double *x, const double *b,
                const int32_t *iax, const int32_t *ja, double tol, int maxit,
-               const fvc_options *opt, fvc_stats *stats, bool on_device, hipStream_t s) {
+               const fvc_options *opt, fvc_stats *stats, bool on_device, hipStream_t s,
+               bool noalloc = false) {
   int rc = check_common(code, matriz, neq, ad, x, b, iax, ja);
   if (stats) { std::memset(stats, 0, sizeof *stats); stats->status = rc; stats->true_relres = -1; }
   if (rc) return rc;
   fvc_options o = opt ? *opt : defaults();
   if (o.gmres_k <= 0) o.gmres_k = 15;
   DevState *d = nullptr;
-  if ((rc = current_device(&d))) return rc;
+  if (noalloc) {
+    // no set-up here: it allocates (pinned host words, events)
+    int dev = 0;
+    HIPCHK(hipGetDevice(&dev));
+    if (dev < 0 || dev >= 64 || !g_dev[dev].ready)
+      return fail(FVC_BAD_ARGUMENT, "fvc_solve_dev_noalloc: call fvc_init(%d) first on this device", dev);
+    if (matriz != FVC_ELLPACK)
+      return fail(FVC_UNSUPPORTED, "fvc_solve_dev_noalloc takes ELLPACK storage (matriz 4): the other "
+                                   "formats build index arrays per call");
+    d = &g_dev[dev];
+  } else if ((rc = current_device(&d))) {
+    return rc;
+  }
   std::lock_guard<std::mutex> lk(d->mu);
+  NoallocScope scope(d, noalloc);
   Owned own;
   MatDesc md;
   const double *m = nullptr;
@@ -1518,6 +1571,14 @@ int fvc_solve_dev(int32_t code, int32_t matriz, int32_t unsym, int32_t neq, int3
                   int32_t maxit, const fvc_options *opt, fvc_stats *stats, void *stream) {
   return solve_impl(code, matriz, unsym, neq, nad, ad, au, al, x, b, iax, ja, tol, maxit, opt,
                     stats, true, (hipStream_t)stream);
+}
+
+int fvc_solve_dev_noalloc(int32_t code, int32_t matriz, int32_t unsym, int32_t neq, int32_t nad,
+                          const double *ad, const double *au, const double *al, double *x,
+                          const double *b, const int32_t *iax, const int32_t *ja, double tol,
+                          int32_t maxit, const fvc_options *opt, fvc_stats *stats, void *stream) {
+  return solve_impl(code, matriz, unsym, neq, nad, ad, au, al, x, b, iax, ja, tol, maxit, opt,
+                    stats, true, (hipStream_t)stream, true);
 }
 
 int fvc_solve_batch(int32_t code, int32_t nsys, const int32_t *row_offsets, int32_t w,

@@ -205,6 +205,27 @@ int fvc_solve_dev(int32_t code, int32_t matriz, int32_t unsym, int32_t neq, int3
                   const double *b, const int32_t *iax, const int32_t *ja, double tol,
                   int32_t maxit, const fvc_options *opt, fvc_stats *stats, void *stream);
 
+/* fvc_solve_dev with no allocation.  No counterpart in the reference: for
+ * callers that cannot allocate device memory (code under graph capture, tight
+ * serving loops, harnesses that forbid hipMalloc).  Same arguments, stopping
+ * rules, results and status codes as fvc_solve_dev, for ELLPACK storage
+ * (matriz 4; other formats return FVC_UNSUPPORTED).  The solver's scratch
+ * (vectors, partial sums, counters, state) comes from one fixed workspace of
+ * FVC_NOALLOC_WORKSPACE_BYTES per device, which the library holds until the
+ * process ends; a solve that needs more returns FVC_UNSUPPORTED (BiCGSTAB:
+ * about 9 vectors of neq doubles, PCG 8, GMRES(k) k+2; so up to ~400,000 rows
+ * for PCG and BiCGSTAB).  fvc_init(device) must have run on the device first
+ * (it sets the workspace up); a call that succeeds makes no allocation of its
+ * own (no hipMalloc, hipFree, pinned or heap memory).
+ * Every call on a device shares its workspace and leaves its contents there:
+ * calls that overlap on one device race (each call waits for its own solve to
+ * finish before it returns, so calls from one thread never overlap). */
+#define FVC_NOALLOC_WORKSPACE_BYTES (32u << 20)
+int fvc_solve_dev_noalloc(int32_t code, int32_t matriz, int32_t unsym, int32_t neq, int32_t nad,
+                          const double *ad, const double *au, const double *al, double *x,
+                          const double *b, const int32_t *iax, const int32_t *ja, double tol,
+                          int32_t maxit, const fvc_options *opt, fvc_stats *stats, void *stream);
+
 /* ----------------------------------------------------------------------------
  * Batched Jacobi-PCG: nsys independent ELLPACK systems of width w, each in
  * the reference's layout (local 1-based ja), concatenated back to back:
