@@ -6,6 +6,7 @@
 #pragma once
 #include "fvc_krylov.h"
 #include "fvc_mixed.h"
+#include "fvc_mixed_krylov.h"
 #include "fvc_batch.h"
 
 // Device formats (Fmt): every format for the FP64 solvers, ELLPACK and CSRD
@@ -32,6 +33,9 @@
 #define FVCI_MIXED(P, F)                                           \
   P __global__ void fvc::mixed_persistent<F>(fvc::MixedArgs);   \
   P __global__ void fvc::mixed_kernel<F>(fvc::MixedArgs);       \
+  P __global__ void fvc::mbicg_persistent<F>(fvc::MBicgArgs);   \
+  P __global__ void fvc::mbicg_kernel<F>(fvc::MBicgArgs);       \
+  P __global__ void fvc::mgmres_persistent<F>(fvc::MGmresArgs); \
   P __global__ void fvc::scale_to_f32_kernel<F>(fvc::Mat, const double *, float *, double *, float *);
 #define FVCI_BATCH(P, F)                                               \
   P __global__ void fvc::pcg_batch_persistent<F>(fvc::BatchArgs);   \

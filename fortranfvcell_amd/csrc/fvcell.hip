@@ -850,11 +850,157 @@ int run_mixed(DevState &d, const SolveIn &in, fvc_stats *stats, hipStream_t s) {
   }
 }
 
+// Mixed-precision BiCGSTAB and GMRES(k) (fvc_mixed_krylov.h): the FP32
+// scaled matrix and int16 column offsets as for the mixed PCG, FP32 vectors,
+// FP64 solution and residuals.
+template <int F>
+int run_mixed_krylov(DevState &d, const SolveIn &in, fvc_stats *stats, hipStream_t s) {
+  if constexpr (F != F_ELL4 && F != F_ELL3 && F != F_ELLW && F != F_CSRD) {
+    return fail(FVC_UNSUPPORTED, "mixed precision supports ELLPACK and unsymmetric CSRD storage");
+  } else {
+    const int neq = in.A.neq;
+    const bool gm = in.code == FVC_GMRES;
+    const int k = in.gmres_k;
+    if (gm && (k < 1 || k > kCgsWide))
+      return fail(FVC_UNSUPPORTED, "mixed-precision GMRES takes k <= %d (asked %d)", kCgsWide, k);
+    if (gm && in.engine == FVC_ENGINE_KERNELS)
+      return fail(FVC_UNSUPPORTED, "mixed-precision GMRES runs as one persistent launch");
+    // BiCGSTAB: one kernel per phase by default, shaped as the FP64 two-phase
+    // BiCGSTAB; GMRES: persistent, shaped as the FP64 CGS2 GMRES
+    const bool persistent = gm || in.engine == FVC_ENGINE_PERSISTENT;
+    const int bs = in.bs > 0 ? in.bs : gm ? 512 : persistent ? kDefaultBlock : kernels_block(FVC_BICGSTAB, true);
+    if (bs % 64 != 0 || bs > (gm ? 512 : 1024))
+      return fail(FVC_BAD_ARGUMENT, "block size %d must be a multiple of 64 <= %d", bs, gm ? 512 : 1024);
+    const void *pfn = gm ? (const void *)&mgmres_persistent<F> : (const void *)&mbicg_persistent<F>;
+    const void *kfn = (const void *)&mbicg_kernel<F>;
+    int occ = 0;
+    int rc = occupancy(pfn, bs, &occ);
+    if (rc) return rc;
+    if (occ < 1) return fail(FVC_DEVICE_ERROR, "persistent kernel does not fit a CU at block size %d", bs);
+    const int G_req = (gm && in.G_req == 0) ? std::max(8, std::min(3 * d.cus / 2, (neq / 512) & ~7)) : in.G_req;
+    const int per_cu = (long)neq >= 24L * 4 * d.cus * bs ? 2 : 4;
+    const int G = persistent ? choose_blocks(neq, d.cus, occ, G_req) : kernels_blocks(neq, d.cus, G_req, per_cu);
+    if (persistent && G > occ * d.cus)
+      return fail(FVC_BAD_ARGUMENT, "%d blocks exceed the %d co-resident blocks of a persistent launch", G, occ * d.cus);
+    if ((rc = check_rows(neq, G, bs))) return rc;
+    const size_t n = (size_t)neq;
+    size_t nv = (F == F_CSRD) ? 0 : (size_t)in.A.w * n;
+    if (F == F_CSRD) {
+      int last = 0;
+      HIPCHK(hipMemcpy(&last, in.A.ia + neq, sizeof(int), hipMemcpyDeviceToHost));
+      nv = (size_t)std::max(last - 1, 0);
+    }
+    constexpr int kW16 = (F == F_ELL4) ? 4 : (F == F_ELL3) ? 3 : 0;
+    const size_t n16 = kW16 ? (size_t)kW16 * n : 0;
+    const int nvec = gm ? k + 1 : 9;
+    const size_t npart = (size_t)(gm ? kCgsChunk : kMaxK) * G;
+    size_t bytes = align_up((kCtrWords + (size_t)G + 1) * sizeof(unsigned)) + align_up(sizeof(KState)) +
+                   align_up(npart * sizeof(double)) + align_up(std::max<size_t>(nv, 1) * sizeof(float)) +
+                   align_up(n * sizeof(double)) + align_up(n * sizeof(float)) +
+                   (size_t)nvec * align_up(n * sizeof(float)) + align_up(2 * sizeof(KState)) +
+                   align_up(2 * kMaxK * (size_t)G * sizeof(double)) +
+                   align_up(std::max<size_t>(n16, 1) * sizeof(short)) +
+                   align_up((size_t)(k + 2) * (k + 2) * 2 * sizeof(double)) + 4096;
+    char *ws = nullptr;
+    if ((rc = workspace(d, bytes, &ws))) return rc;
+    Carve cv{ws};
+    unsigned *ctr = cv.take<unsigned>(kCtrWords + (size_t)G + 1);
+    int *wide = reinterpret_cast<int *>(ctr + kCtrWords + G);
+    KState *st = cv.take<KState>(1);
+    double *partials = cv.take<double>(npart);
+    Defer dfr{cv.take<KState>(2), cv.take<double>(2 * kMaxK * (size_t)G), 0};
+    float *av32 = cv.take<float>(std::max<size_t>(nv, 1));
+    short *jo = cv.take<short>(std::max<size_t>(n16, 1));
+    double *sc = cv.take<double>(n);
+    float *dsq = cv.take<float>(n);
+    const size_t fstride = align_up(n * sizeof(float)) / sizeof(float);
+    float *fv = cv.take<float>((size_t)nvec * fstride);
+    double *gsmall = cv.take<double>((size_t)(k + 2) * (k + 2) * 2);
+    Mat32 A32{neq, in.A.w, av32, in.A.ia, in.A.ja, nullptr};
+    Grid g = make_grid(neq, G, ctr, partials, persistent, in.barrier,
+                       gm ? 0 : use_tiles(in.row_order, FVC_BICGSTAB, neq, G, bs));
+    HIPCHK(hipEventRecord(d.ev0, s));
+    hipLaunchKernelGGL(reset_kernel, dim3(1), dim3(256), 0, s, ctr, (int)kCtrWords + G + 1, st, 1);
+    hipLaunchKernelGGL(scale_to_f32_kernel<F>, dim3((neq + 255) / 256), dim3(256), 0, s, in.A, in.m, av32, sc, dsq);
+    HIPCHK(hipGetLastError());
+    if constexpr (kW16 > 0) {
+      hipLaunchKernelGGL(col_offsets_kernel<kW16>, dim3(xcd_blocks(neq, 256)), dim3(256), 0, s, neq, in.A.ja,
+                         av32, jo, wide);
+      HIPCHK(hipGetLastError());
+      int h_wide = 1;
+      HIPCHK(hipMemcpyAsync(&h_wide, wide, sizeof(int), hipMemcpyDeviceToHost, s));
+      HIPCHK(hipStreamSynchronize(s));
+      if (!h_wide) A32.jo = jo;
+    }
+    const KState *final_state = st;
+    if (gm) {
+      MGmresArgs m{};
+      GmresArgs &a = m.q;
+      a.A = in.A; a.g = g; a.st = st; a.m = in.m; a.x = in.x; a.b = in.b;
+      a.ld = fstride;
+      double *sm = gsmall;
+      a.h = sm; sm += (size_t)(k + 1) * k;
+      a.cs = sm; sm += k + 2;
+      a.sn = sm; sm += k + 2;
+      a.e = sm; sm += k + 2;
+      a.y = sm; sm += k + 2;
+      a.nrm = sm; sm += k + 2;
+      a.tol = in.tol; a.k = k; a.maxit = in.maxit; a.stop_rule = 1; a.orth = 1;
+      m.A32 = A32;
+      m.s = sc;
+      m.gv = fv;
+      if ((rc = launch(pfn, true, G, bs, m, s))) return rc;
+    } else {
+      MBicgArgs a{};
+      a.A = in.A; a.A32 = A32; a.g = g; a.st = st; a.s = sc; a.dsq = dsq; a.x = in.x; a.b = in.b;
+      float **vecs[9] = {&a.r, &a.rh, &a.p0, &a.p1, &a.v0, &a.v1, &a.sv, &a.t, &a.y};
+      for (int i = 0; i < 9; ++i) *vecs[i] = fv + (size_t)i * fstride;
+      a.tol = in.tol; a.maxit = in.maxit; a.x0_zero = in.x0_zero; a.max_inner = in.refresh_every;
+      if (persistent) {
+        a.phase = -1;
+        if ((rc = launch(pfn, true, G, bs, a, s))) return rc;
+      } else {
+        auto phase = [&](int ph) {
+          a.phase = ph;
+          a.d = dfr;
+          dfr.par ^= 1;
+          return launch(kfn, false, G, bs, a, s);
+        };
+        hipLaunchKernelGGL(reset_kernel, dim3(1), dim3(256), 0, s, ctr, 0, dfr.st, 2);
+        if ((rc = phase(MB_SETUP))) return rc;
+        // an iteration is two launches (P, Q), a reliable update two more
+        const int cap = 4 * std::max(in.maxit, 0) + 16;
+        if ((rc = drive_batches(d, s, cap, [&] { return phase(kPhaseNext); }, [&] { return dfr.st + dfr.par; })))
+          return rc;
+        if ((rc = phase(MB_FINAL))) return rc;
+        if ((rc = phase(-1))) return rc;
+        final_state = dfr.st + dfr.par;
+      }
+    }
+    HIPCHK(hipEventRecord(d.ev1, s));
+    if ((rc = read_state(d, const_cast<KState *>(final_state), s))) return rc;
+    float ms = 0.f;
+    HIPCHK(hipEventElapsedTime(&ms, d.ev0, d.ev1));
+    fill_stats(stats, *d.host_st, in.code, persistent, ms, G, bs,
+               (g.tiles ? FVC_FLAG_TILES : 0) | (A32.jo ? FVC_FLAG_COL16 : 0));
+    if (stats && !gm) stats->inner_iterations = d.host_st->cycle;   // FP64 residual recomputations
+    return finish(*d.host_st, ctr);
+  }
+}
+
 int dispatch_fp64(DevState &d, const SolveIn &in, fvc_stats *stats, hipStream_t s);
 
 int dispatch_solve(DevState &d, const SolveIn &in, fvc_stats *stats, hipStream_t s) {
   if (in.precision == FVC_PREC_MIXED) {
-    if (in.code != FVC_PCG) return fail(FVC_UNSUPPORTED, "mixed precision is available for PCG");
+    if (in.code != FVC_PCG) {
+      switch (in.F) {
+        case F_ELL4: return run_mixed_krylov<F_ELL4>(d, in, stats, s);
+        case F_ELL3: return run_mixed_krylov<F_ELL3>(d, in, stats, s);
+        case F_ELLW: return run_mixed_krylov<F_ELLW>(d, in, stats, s);
+        case F_CSRD: return run_mixed_krylov<F_CSRD>(d, in, stats, s);
+      }
+      return fail(FVC_UNSUPPORTED, "mixed precision supports ELLPACK and unsymmetric CSRD storage");
+    }
     switch (in.F) {
       case F_ELL4: return run_mixed<F_ELL4>(d, in, stats, s);
       case F_ELL3: return run_mixed<F_ELL3>(d, in, stats, s);

@@ -101,9 +101,6 @@ def test_mixed_maxit_and_warm_start(fvc):
 
 
 def test_mixed_unsupported_combinations(fvc):
-    s = P.config_advdiff(16)
-    with pytest.raises(fvc.FvcError):
-        fvc.solve(P.to_ell(s), fvc.BICGSTAB, precision=fvc.PREC_MIXED, raise_on_failure=True)
     sp = P.config_poisson(16)
     with pytest.raises(fvc.FvcError):
         fvc.solve(P.to_csrd(sp, sym=True), fvc.PCG, precision=fvc.PREC_MIXED, raise_on_failure=True)

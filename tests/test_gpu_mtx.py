@@ -18,11 +18,12 @@ pytestmark = [pytest.mark.gpu,
                                  reason="oracle/_ref/libmvf_dropin.so is built only where the reference sources are")]
 
 
-@pytest.mark.parametrize("name", ["advdiff48_bicgstab", "lognormal40_pcg", "poisson32_gmres"])
+@pytest.mark.parametrize("name", ["advdiff48_bicgstab", "lognormal40_pcg", "advdiff40_gmres"])
 def test_gpu_on_reference_exported_systems(fvc, tmp_path, name):
     s, code, kw = {"advdiff48_bicgstab": (P.config_advdiff(48), fvc.BICGSTAB, {}),
                    "lognormal40_pcg": (P.config_pressure_lognormal(40), fvc.PCG, {}),
-                   "poisson32_gmres": (P.config_poisson(32, pin="identity"), fvc.GMRES, dict(gmres_k=30))}[name]
+                   "advdiff40_gmres": (P.config_advdiff(40, seed=P.SEED_BASE + 12), fvc.GMRES,
+                                       dict(gmres_k=30))}[name]
     f = P.to_csrd(s)
     path, _ = export.ref_export_coo(f, 1, True, str(tmp_path))
     g = mtx.read_system(path[:-len(".mtx")])
