@@ -16,6 +16,7 @@
 #include "../../include/fvcell.h"
 #include "fvc_instances.h"   // the solver kernels, built in k_*.hip
 #include "fvc_plain.h"
+#include "fvc_host.h"   // host-only algorithms (also built with sanitizers on the CPU)
 #include "fvc_adjoint.h"
 #include "fvc_post.h"
 
@@ -41,6 +42,12 @@ int fail(int code, const char *fmt, ...) {
   va_end(ap);
   g_err = buf;
   return code;
+}
+
+// A workspace whose pieces outgrow the size computed for it is a defect of
+// this library, reported before any kernel would write past its end.
+int carve_overflow(const Carve &cv) {
+  return fail(FVC_DEVICE_ERROR, "internal error: workspace layout needs %zu bytes of %zu", cv.off, cv.cap);
 }
 
 #define HIPCHK(call)                                                                        \
@@ -150,20 +157,6 @@ int current_device(DevState **out) {
   return FVC_OK;
 }
 
-size_t align_up(size_t v, size_t a = 256) { return (v + a - 1) / a * a; }
-
-// Carves the arena into the pieces a solve needs.
-struct Carve {
-  char *base;
-  size_t off = 0;
-  template <class T>
-  T *take(size_t n) {
-    T *p = reinterpret_cast<T *>(base + off);
-    off = align_up(off + n * sizeof(T));
-    return p;
-  }
-};
-
 int ensure_arena(DevState &d, size_t bytes) {
   if (d.arena_bytes >= bytes) return FVC_OK;
   if (d.arena) {
@@ -204,30 +197,6 @@ int fmt_for(int matriz, int unsym, int w, int *F) {
     case FVC_CSRC: *F = unsym ? F_CSRC : F_LSYM; return FVC_OK;
   }
   return fail(FVC_UNSUPPORTED, "matriz %d is not a storage format (1 CSRD, 2 CSR, 3 CSRC, 4 ELLPACK)", matriz);
-}
-
-// Transposed-gather structure of a lower-triangle pattern: for each row c,
-// the entries (row i > c, position k) with ja(k) = c, in increasing i.
-void build_transpose(int neq, const int *ia, const int *ja, std::vector<int> &tptr,
-                     std::vector<int> &tsrc, std::vector<int> &tpos) {
-  const int nnz = ia[neq] - ia[0];
-  tptr.assign(neq + 1, 0);
-  for (int k = 0; k < nnz; ++k) {
-    const int c = ja[k] - 1;
-    if (c >= 0 && c < neq) tptr[c + 1]++;
-  }
-  for (int i = 0; i < neq; ++i) tptr[i + 1] += tptr[i];
-  tsrc.assign(std::max(nnz, 1), 0);
-  tpos.assign(std::max(nnz, 1), 0);
-  std::vector<int> fill(tptr.begin(), tptr.end() - 1);
-  for (int i = 0; i < neq; ++i)
-    for (int k = ia[i] - 1; k < ia[i + 1] - 1; ++k) {
-      const int c = ja[k] - 1;
-      if (c < 0 || c >= neq) continue;
-      const int e = fill[c]++;
-      tsrc[e] = i;
-      tpos[e] = k;
-    }
 }
 
 // --------------------------------------------------------- launch config --
@@ -488,7 +457,7 @@ int run_solve(DevState &d, const SolveIn &in, fvc_stats *stats, hipStream_t s) {
                  align_up(2 * sizeof(KState)) + align_up(2 * kMaxK * (size_t)G * sizeof(double)) + 4096;
   char *ws = nullptr;
   if ((rc = workspace(d, bytes, &ws))) return rc;
-  Carve cv{ws};
+  Carve cv{ws, bytes};
   unsigned *ctr = cv.take<unsigned>(kCtrWords + (size_t)G);
   KState *st = cv.take<KState>(1);
   double *partials = cv.take<double>(npart);
@@ -498,6 +467,7 @@ int run_solve(DevState &d, const SolveIn &in, fvc_stats *stats, hipStream_t s) {
   for (int i = 0; i < nvec; ++i) vec[i] = cv.take<double>(n);
   double *gsmall = cv.take<double>((size_t)(k + 2) * (k + 2) * 2);
   double *xbackup = cv.take<double>(n);
+  if (!cv.fits()) return carve_overflow(cv);
   const bool may_fall_back = persistent && in.engine == FVC_ENGINE_AUTO;
   if (may_fall_back && !in.x0_zero)
     HIPCHK(hipMemcpyAsync(xbackup, in.x, n * sizeof(double), hipMemcpyDeviceToDevice, s));
@@ -771,7 +741,7 @@ int run_mixed(DevState &d, const SolveIn &in, fvc_stats *stats, hipStream_t s) {
                    align_up(std::max<size_t>(n16, 1) * sizeof(short)) + 4096;
     char *ws = nullptr;
     if ((rc = workspace(d, bytes, &ws))) return rc;
-    Carve cv{ws};
+    Carve cv{ws, bytes};
     // barrier counters, G arrival flags, then the offsets' "does not fit" flag
     unsigned *ctr = cv.take<unsigned>(kCtrWords + (size_t)G + 1);
     int *wide = reinterpret_cast<int *>(ctr + kCtrWords + G);
@@ -785,6 +755,7 @@ int run_mixed(DevState &d, const SolveIn &in, fvc_stats *stats, hipStream_t s) {
     MixedArgs a{};
     a.ps = align_up(n * sizeof(float4)) / sizeof(float4);
     a.rec = cv.take<float4>(2 * a.ps);
+    if (!cv.fits()) return carve_overflow(cv);
     a.A = in.A;
     a.A32.neq = neq;
     a.A32.w = in.A.w;
@@ -903,7 +874,7 @@ int run_mixed_krylov(DevState &d, const SolveIn &in, fvc_stats *stats, hipStream
                    align_up((size_t)(k + 2) * (k + 2) * 2 * sizeof(double)) + 4096;
     char *ws = nullptr;
     if ((rc = workspace(d, bytes, &ws))) return rc;
-    Carve cv{ws};
+    Carve cv{ws, bytes};
     unsigned *ctr = cv.take<unsigned>(kCtrWords + (size_t)G + 1);
     int *wide = reinterpret_cast<int *>(ctr + kCtrWords + G);
     KState *st = cv.take<KState>(1);
@@ -916,6 +887,7 @@ int run_mixed_krylov(DevState &d, const SolveIn &in, fvc_stats *stats, hipStream
     const size_t fstride = align_up(n * sizeof(float)) / sizeof(float);
     float *fv = cv.take<float>((size_t)nvec * fstride);
     double *gsmall = cv.take<double>((size_t)(k + 2) * (k + 2) * 2);
+    if (!cv.fits()) return carve_overflow(cv);
     Mat32 A32{neq, in.A.w, av32, in.A.ia, in.A.ja, nullptr};
     Grid g = make_grid(neq, G, ctr, partials, persistent, in.barrier,
                        gm ? 0 : use_tiles(in.row_order, FVC_BICGSTAB, neq, G, bs));
@@ -1420,11 +1392,12 @@ int dot_impl(const double *a, const double *b, int n, double *result, bool on_de
   size_t bytes = align_up((kCtrWords + (size_t)G) * sizeof(unsigned)) + align_up(sizeof(KState)) +
                  align_up(kMaxK * (size_t)G * sizeof(double)) + 4096;
   if ((rc = ensure_arena(*d, bytes))) return rc;
-  Carve cv{d->arena};
+  Carve cv{d->arena, bytes};
   unsigned *ctr = cv.take<unsigned>(kCtrWords + (size_t)G);
   KState *st = cv.take<KState>(1);
   double *partials = cv.take<double>(kMaxK * (size_t)G);
   double *out = cv.take<double>(1);
+  if (!cv.fits()) return carve_overflow(cv);
   hipLaunchKernelGGL(reset_kernel, dim3(1), dim3(256), 0, s, ctr, (int)kCtrWords, st);
   Grid g = make_grid(n, G, ctr, partials, false);
   hipLaunchKernelGGL(dot_kernel, dim3(G), dim3(bs), 0, s, g, da, db, out);
@@ -1436,58 +1409,6 @@ int dot_impl(const double *a, const double *b, int n, double *result, bool on_de
 }
 
 // ------------------------------------------------------------- batches --
-
-// Block groups per system: g_s proportional to neq_s^exp (PCG work ~ rows x
-// iterations, iterations ~ sqrt(rows) on these meshes), so that the systems
-// finish together; at least one block each, at most one per 64 rows, the
-// total within the co-resident capacity.  Shares are rounded by largest
-// remainder, so the large systems that set the batch's time are not starved
-// by the one-block floor of the small ones.
-std::vector<int> allocate_blocks(const std::vector<int> &neq, int cap, double expo) {
-  const int n = (int)neq.size();
-  std::vector<double> w(n);
-  double tot = 0;
-  for (int i = 0; i < n; ++i) tot += (w[i] = std::pow((double)std::max(neq[i], 1), expo));
-  std::vector<int> g(n, 1), lim(n);
-  for (int i = 0; i < n; ++i) lim[i] = std::max(1, (neq[i] + 63) / 64);
-  int used = n;
-  if (cap > n) {
-    // systems whose share is below one block keep one; the rest share what is left
-    double rest = tot;
-    int avail = cap;
-    std::vector<char> floor1(n, 0);
-    for (bool changed = true; changed;) {
-      changed = false;
-      for (int i = 0; i < n; ++i)
-        if (!floor1[i] && (w[i] / rest) * avail < 1.0) {
-          floor1[i] = 1;
-          rest -= w[i];
-          --avail;
-          changed = true;
-        }
-      if (rest <= 0.0) break;
-    }
-    std::vector<std::pair<double, int>> frac;
-    used = 0;
-    for (int i = 0; i < n; ++i) {
-      if (floor1[i]) {
-        g[i] = 1;
-      } else {
-        const double t = (w[i] / rest) * avail;
-        g[i] = std::min(lim[i], std::max(1, (int)std::floor(t)));
-        if (g[i] < lim[i]) frac.push_back({t - std::floor(t), i});
-      }
-      used += g[i];
-    }
-    std::sort(frac.begin(), frac.end(), [](const auto &a, const auto &b) { return a.first > b.first; });
-    for (const auto &f : frac) {
-      if (used >= cap) break;
-      ++g[f.second];
-      ++used;
-    }
-  }
-  return g;
-}
 
 template <int F>
 int run_batch(DevState &d, int nsys, const std::vector<int> &row_off, int w, const double *ad,
@@ -1518,7 +1439,7 @@ int run_batch(DevState &d, int nsys, const std::vector<int> &row_off, int w, con
                       (kOff16 ? align_up(n * w * sizeof(short)) + align_up(sizeof(int)) : 0);
   else bytes += 5 * align_up(n * sizeof(double));
   if ((rc = ensure_arena(d, bytes))) return rc;
-  Carve cv{d.arena};
+  Carve cv{d.arena, bytes};
   unsigned *ctr = cv.take<unsigned>((size_t)wave * kCtrWords + (size_t)cap);
   KState *st = cv.take<KState>(wave);
   double *partials = cv.take<double>((size_t)kMaxK * cap);
@@ -1552,6 +1473,7 @@ int run_batch(DevState &d, int nsys, const std::vector<int> &row_off, int w, con
   } else {
     for (double *&v : B.v) v = cv.take<double>(n);
   }
+  if (!cv.fits()) return carve_overflow(cv);
   B.ctr = ctr;
   B.flags = ctr + (size_t)wave * kCtrWords;
   B.flagbar = o.barrier == 0 ? 1 : 0;
