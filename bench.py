@@ -203,9 +203,17 @@ def gform_bytes(numel: int, nnode: int, nen: int = 4) -> int:
     return numel * (4 * (3 * nen + 2) + 8 + 32 + 8) + nnode * 16
 
 
+def assemble_pc_bytes(numel: int, nnode: int, nen: int = 4) -> int:
+    """Algorithmic bytes of the pressure-correction assembly: per cell ix,
+    nelcon, pedge (as gform), num, w, d, gradp and p in; ad, al (nen), b and
+    the mass residual out; each node's coordinates once (the wall values in
+    sedge are read for wall edges only and not counted)."""
+    return numel * (4 * (3 * nen + 2) + 4 + 56 + 8 * (nen + 3)) + nnode * 16
+
+
 def bench_post(F, n: int, reps: int = 50) -> dict:
-    """fvc_gform_dev and fvc_simple_update_dev on the n x n jittered quad mesh
-    (HIP events on the launch stream)."""
+    """fvc_assemble_pc_dev, fvc_gform_dev and fvc_simple_update_dev on the
+    n x n jittered quad mesh (HIP events on the launch stream)."""
     import ctypes as C
     from fortranfvcell_amd import problems as P
     from fortranfvcell_amd.post import DeviceMesh, simple_update
@@ -240,11 +248,35 @@ def bench_post(F, n: int, reps: int = 50) -> dict:
     L.fvc_event_record(e1, None)
     L.fvc_event_elapsed_ms(e0, e1, C.byref(ms))
     t_s = ms.value * 1e-3 / reps
+    # the pressure-correction assembly (intVel 5, walls: inflow, outflow,
+    # parabolic, zero gradient; the level pinned on cell 1)
+    a2 = P.ref_mesh_arrays(m, (2, 3, 4, 6))
+    pedge = a2["pedge"].copy()
+    pedge[0, m.w] = -1
+    dm2 = DeviceMesh(a2["x"], a2["ix"], a2["nelcon"], pedge)
+    f = {"num": np.arange(1, m.neq + 1, dtype=np.int32), "e": np.array([[0.0, 1.0] + [0.0] * 8]),
+         "sedge": rng.normal(size=(m.neq, m.w + 1, 3)), "w": rng.normal(size=(m.neq, 2)),
+         "d": rng.uniform(0.5, 1.5, size=(m.neq, 2)), "gradp": rng.normal(size=(m.neq, 2)),
+         "p": rng.normal(size=m.neq)}
+    df = {k: F.DeviceArray.from_host(v) for k, v in f.items()}
+    outs = (F.DeviceArray.empty(m.neq), F.DeviceArray.empty((m.neq, m.w)), F.DeviceArray.empty(m.neq),
+            F.DeviceArray.empty(m.neq))
+    pc_args = (df["num"], df["e"], df["sedge"], df["w"], df["d"], df["gradp"], df["p"], *outs)
+    dm2.assemble_pc(*pc_args)
+    L.fvc_synchronize()
+    L.fvc_event_record(e0, None)
+    for _ in range(reps):
+        dm2.assemble_pc(*pc_args)
+    L.fvc_event_record(e1, None)
+    L.fvc_event_elapsed_ms(e0, e1, C.byref(ms))
+    t_a = ms.value * 1e-3 / reps
     L.fvc_event_destroy(e0)
     L.fvc_event_destroy(e1)
+    ab = assemble_pc_bytes(m.neq, len(a["x"]))
     gb = gform_bytes(m.neq, len(a["x"]))
     sb = m.neq * (16 + 8 + 8 + 8 + 8 + 16 + 16 + 8)     # w out, pre r/w, u1, u2, preC, gradPc, d
-    return {"cells": m.neq, "gform_us": t_g * 1e6, "gform_GBs": gb / t_g / 1e9, "gform_bytes": gb,
+    return {"cells": m.neq, "assemble_pc_us": t_a * 1e6, "assemble_pc_GBs": ab / t_a / 1e9, "assemble_pc_bytes": ab,
+            "gform_us": t_g * 1e6, "gform_GBs": gb / t_g / 1e9, "gform_bytes": gb,
             "simple_update_us": t_s * 1e6, "simple_update_GBs": sb / t_s / 1e9}
 
 
@@ -523,6 +555,39 @@ def main():
         mixed["speedup_mixed_over_fp64"] = mixed["fp64"]["ms_per_solve"] / mixed["mixed"]["ms_per_solve"]
         extra["pcg_lognormal_fp64_vs_mixed"] = mixed
         log("mixed", mixed)
+        # mixed-precision BiCGSTAB and GMRES(30) against FP64 on the headline
+        # system and on the same operator at 2n x 2n cells (4M: beyond the
+        # 256 MB Infinity Cache), all to true relative residual 1e-10
+        mk = {}
+        for nn in (n, 2 * n):
+            if nn == n:
+                dm, sm = ds, s
+            else:
+                sm = P.config_advdiff(nn)
+                dm = F.DeviceSystem(P.to_ell(sm))
+            Am = P.scipy_csr(sm)
+            row = {"neq": sm.neq}
+            for mname, code, fp64kw in (("bicgstab", F.BICGSTAB, {}),
+                                        ("gmres30", F.GMRES, dict(gmres_orth=F.ORTH_CGS2))):
+                res = {}
+                for pname, prec, kw in (("fp64", F.PREC_FP64, dict(stop_rule=F.STOP_RELRES, **fp64kw)),
+                                        ("mixed", F.PREC_MIXED, {})):
+                    skw = dict(tol=tol, x0_rule=F.X0_ZERO, gmres_k=30, precision=prec, **kw)
+                    dm.solve(code, **skw)
+                    stm = dm.solve(code, **skw)
+                    xm_ = dm.x.get()
+                    res[pname] = {"iterations": stm["iterations"], "inner_iterations": stm["inner_iterations"],
+                                  "ms_per_solve": stm["seconds"] * 1e3, "solves_per_s": 1.0 / stm["seconds"],
+                                  "true_relres": float(np.linalg.norm(sm.b - Am @ xm_) / np.linalg.norm(sm.b)),
+                                  "status": stm["status"], "_x": xm_}
+                x64_ = res["fp64"].pop("_x")
+                xmx_ = res["mixed"].pop("_x")
+                res["mixed"]["rel_err_vs_fp64"] = float(np.linalg.norm(xmx_ - x64_) / np.linalg.norm(x64_))
+                res["speedup_mixed_over_fp64"] = res["fp64"]["ms_per_solve"] / res["mixed"]["ms_per_solve"]
+                row[mname] = res
+            mk[f"{int(round(sm.neq ** 0.5))}x{int(round(sm.neq ** 0.5))}"] = row
+        extra["advdiff_fp64_vs_mixed"] = mk
+        log("mixed krylov", mk)
         # batching: 256 uneven pressure systems (sides log-uniform in [32, 512])
         # in one persistent launch, FP64 and mixed, against the same systems
         # solved one at a time in FP64 (all resident in HBM)

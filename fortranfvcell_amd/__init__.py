@@ -177,6 +177,10 @@ class DeviceArray:
         check(load().fvc_malloc(C.byref(p), max(self.nbytes, 1)))
         self.ptr = p.value
 
+    @property
+    def ndim(self) -> int:
+        return len(self.shape)
+
     @classmethod
     def from_host(cls, a: np.ndarray) -> "DeviceArray":
         a = np.ascontiguousarray(a)
@@ -233,6 +237,27 @@ class DeviceSystem:
         self.ja = DeviceArray.from_host(ja)
         self.b = DeviceArray.from_host(np.ascontiguousarray(fmt["b"], dtype=np.float64))
         self.x = DeviceArray.empty(self.neq)
+
+    @classmethod
+    def resident_ell(cls, ad: "DeviceArray", al: "DeviceArray", b: "DeviceArray", ja,
+                     unsym: bool = True) -> "DeviceSystem":
+        """An ELLPACK system whose values are already in HBM (assembled on the
+        device): ad (neq), al (neq, w), b (neq) DeviceArrays, taken as they
+        are; ja (neq, w) the 1-based int32 columns, uploaded once."""
+        ja = np.ascontiguousarray(ja, dtype=np.int32)
+        neq, w = al.shape
+        if ad.shape != (neq,) or b.shape != (neq,) or ja.shape != (neq, w):
+            raise ValueError("ad (neq), al (neq, w), b (neq) and ja (neq, w) expected")
+        self = cls.__new__(cls)
+        self.matriz, self.neq, self.unsym = ELLPACK, int(neq), bool(unsym)
+        self.nad = int(neq * w)
+        self.ad, self.al, self.au, self.b = ad, al, al, b
+        iax = np.array([w, neq + 1], dtype=np.int32)
+        self.iax = DeviceArray.from_host(iax)
+        self._iax_arg = iax
+        self.ja = DeviceArray.from_host(ja)
+        self.x = DeviceArray.empty(self.neq)
+        return self
 
     def solve(self, code: int, tol: float = 1e-10, maxit: int = 50000, x0=None, options=None,
               stream=None, noalloc: bool = False, **opt) -> dict:

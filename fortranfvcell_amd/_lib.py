@@ -31,7 +31,8 @@ EXPORTS = [
     "fvc_default_options", "fvc_init", "fvc_last_error", "fvc_solver", "fvc_solve", "fvc_keep_matrix",
     "fvc_solve_dev", "fvc_solve_dev_noalloc",
     "fvc_solve_batch", "fvc_solve_batch_dev", "fvc_ell_transpose_dev", "fvc_grad_k_dev",
-    "fvc_assemble_heat_dev", "fvc_gform_dev", "fvc_simple_update_dev",
+    "fvc_assemble_heat_dev", "fvc_gform_dev", "fvc_simple_update_dev", "fvc_assemble_pc_dev",
+    "fvc_assemble_mom_dev",
     "fvc_matvec", "fvc_matvec_dev", "fvc_dot", "fvc_dot_dev", "fvc_getres", "fvc_getres_dev",
     "fvc_malloc", "fvc_free", "fvc_memcpy_htod", "fvc_memcpy_dtoh", "fvc_memcpy_dtod", "fvc_memset_dev",
     "fvc_synchronize", "fvc_device_info", "fvc_event_create", "fvc_event_destroy",
@@ -95,6 +96,8 @@ def load(path: str = LIB_PATH):
     L.fvc_grad_k_dev.argtypes = [I, I, P, P, P, P, P, P, P, P, P, P]
     L.fvc_assemble_heat_dev.argtypes = [I, I, P, P, P, P, P, P, P, P, P, P, P, P, P]
     L.fvc_gform_dev.argtypes = [I, I, I, P, P, P, P, P, P, P, P]
+    L.fvc_assemble_pc_dev.argtypes = [I, I, I, I, I, I] + [P] * 16
+    L.fvc_assemble_mom_dev.argtypes = [I, I, I, I, I, I, D, D, I, I, I] + [P] * 20
     L.fvc_simple_update_dev.argtypes = [I, I, P, P, P, P, P, P, P, D, D, I, P]
     L.fvc_matvec.argtypes = [I, I, I, I, P, P, P, P, P, P, P]
     L.fvc_matvec_dev.argtypes = [I, I, I, I, P, P, P, P, P, P, P, P]

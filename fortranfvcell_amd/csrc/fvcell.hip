@@ -1719,6 +1719,59 @@ int fvc_gform_dev(int32_t numel, int32_t nnode, int32_t nen, const double *x, co
   return FVC_OK;
 }
 
+int fvc_assemble_pc_dev(int32_t numel, int32_t nnode, int32_t nen, int32_t neq, int32_t nmat,
+                        int32_t int_vel, const double *x, const int32_t *ix, const int32_t *nelcon,
+                        const int32_t *pedge, const int32_t *num, const double *e, const double *sedge,
+                        const double *w, const double *d, const double *gradp, const double *p,
+                        double *ad, double *al, double *b, double *rescell, void *stream) {
+  if (numel <= 0 || nnode <= 0 || neq <= 0 || nmat <= 0 || !x || !ix || !nelcon || !pedge || !num || !e ||
+      !sedge || !w || !d || !gradp || !p || !ad || !al || !b || !rescell)
+    return fail(FVC_BAD_ARGUMENT, "bad pressure-correction assembly arguments");
+  if (int_vel != 1 && int_vel != 2 && int_vel != 5)
+    return fail(FVC_UNSUPPORTED, "intVel %d (1 linear, 2 Rhie-Chow, 5 Ferziger-Peric)", int_vel);
+  if (numel > INT_MAX - 2048) return fail(FVC_UNSUPPORTED, "%d cells overflow 32-bit indices", numel);
+  const dim3 grid(xcd_blocks(numel, 256)), block(256);
+  hipStream_t s = (hipStream_t)stream;
+  if (nen == 4)
+    hipLaunchKernelGGL(assemble_pc_kernel<4>, grid, block, 0, s, numel, nnode, neq, nmat, int_vel, x, ix, nelcon,
+                       pedge, num, e, sedge, w, d, gradp, p, ad, al, b, rescell);
+  else if (nen == 3)
+    hipLaunchKernelGGL(assemble_pc_kernel<3>, grid, block, 0, s, numel, nnode, neq, nmat, int_vel, x, ix, nelcon,
+                       pedge, num, e, sedge, w, d, gradp, p, ad, al, b, rescell);
+  else
+    return fail(FVC_UNSUPPORTED, "pressure-correction assembly: cells of %d edges (triangles and quads only)", nen);
+  HIPCHK(hipGetLastError());
+  return FVC_OK;
+}
+
+int fvc_assemble_mom_dev(int32_t numel, int32_t nnode, int32_t nen, int32_t neq, int32_t nmat, int32_t comp,
+                         double dt, double under_u, int32_t s_pressure, int32_t simple_c, int32_t bs,
+                         const double *x, const int32_t *ix, const int32_t *nelcon, const int32_t *pedge,
+                         const int32_t *num, const double *e, const double *sedge, const double *u,
+                         const double *pres, const double *w, const double *grad1, const double *grad2,
+                         const double *gradp, double *d, double *ad, double *al, double *b, double *rescell,
+                         double *im, void *stream) {
+  if (numel <= 0 || nnode <= 0 || neq <= 0 || nmat <= 0 || !x || !ix || !nelcon || !pedge || !num || !e ||
+      !sedge || !u || !pres || !w || !grad1 || !grad2 || !gradp || !d || !ad || !al || !b || !rescell || !im)
+    return fail(FVC_BAD_ARGUMENT, "bad momentum assembly arguments");
+  if (comp != 1 && comp != 2) return fail(FVC_BAD_ARGUMENT, "momentum component %d (1 or 2)", comp);
+  if (!(dt != 0.0) || !(under_u > 0.0))
+    return fail(FVC_BAD_ARGUMENT, "momentum assembly: dt %g, underU %g", dt, under_u);
+  if (numel > INT_MAX - 2048) return fail(FVC_UNSUPPORTED, "%d cells overflow 32-bit indices", numel);
+  MomArgs A{numel, nnode, neq, nmat, comp - 1, s_pressure ? 1 : 0, simple_c ? 1 : 0, bs ? 1 : 0, under_u, dt,
+            x, ix, nelcon, pedge, num, e, sedge, u, pres, w, grad1, grad2, gradp, d, ad, al, b, rescell, im};
+  const dim3 grid(xcd_blocks(numel, 256)), block(256);
+  hipStream_t s = (hipStream_t)stream;
+  if (nen == 4)
+    hipLaunchKernelGGL(assemble_mom_kernel<4>, grid, block, 0, s, A);
+  else if (nen == 3)
+    hipLaunchKernelGGL(assemble_mom_kernel<3>, grid, block, 0, s, A);
+  else
+    return fail(FVC_UNSUPPORTED, "momentum assembly: cells of %d edges (triangles and quads only)", nen);
+  HIPCHK(hipGetLastError());
+  return FVC_OK;
+}
+
 int fvc_simple_update_dev(int32_t numel, int32_t ndm, double *w, double *pre, const double *u1,
                           const double *u2, const double *prec, const double *gradpc, const double *d,
                           double under_u, double under_pc, int32_t simple_r, void *stream) {

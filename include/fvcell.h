@@ -298,6 +298,58 @@ int fvc_gform_dev(int32_t numel, int32_t nnode, int32_t nen, const double *x, co
                   const int32_t *nelcon, const int32_t *pedge, const double *u, double *grad,
                   double *fluxl, void *stream);
 
+/* fvc_assemble_pc_dev replaces the pressure-correction assembly
+ *   call pform(adPc,auPc,alPc,iaPc,jaPc,num,un,pC,p,p,gradPc,gradP,gradPc,
+ *              div,fluxlPc,iM,rO,wP,bPc,rCPc,x,sedge,e,ddU,eddyVisc,ie,
+ *              nelcon,pedge,ix,numel,ndm,nen,nshared,1,ndf,dt,0.0d0,
+ *              matrizPc,unsymPc,1,2,2,.true.,.true.,.false.,.false.,.true.,bs)
+ * of src/Simple.f:583-588 (src/Pform.f:65-436) for ELLPACK (matrizPc = 4,
+ * the reference's default), cells of types 63/65/73/75 (cell_si_p_i_ggl,
+ * src/CellibPressure.f:390-571, iws = 1).  int_vel is the /simple_/ intVel
+ * (5 in src/Simple.f:186; 1 and 2 also taken).  Inputs as fvc_gform_dev plus
+ * num(numel) (1-based equation numbers, <= 0: no equation), e(10,nmat)
+ * (density in row 2, the material in ix(nen+1,:)), sedge(3,nen+1,numel)
+ * (prescribed edge velocities), w(2,numel) (velocity), d(2,numel) (the
+ * SIMPLE d field, ddU), gradp(2,numel) and p(numel) (the pressure and its
+ * gradient).  Writes, for each cell c with an equation eq = num(c):
+ * ad(eq), al(1:nen, eq) = -a_nb (0 across walls and to cells without an
+ * equation; ellPack, src/EllPack.f:68-93) and b(eq); rescell(c) = the cell's
+ * mass residual.  Rows of equations that no cell owns are not written.
+ * Bit-identical to the reference.  Indices are checked on the device: a
+ * cell with one out of range gets NaN. */
+int fvc_assemble_pc_dev(int32_t numel, int32_t nnode, int32_t nen, int32_t neq, int32_t nmat,
+                        int32_t int_vel, const double *x, const int32_t *ix, const int32_t *nelcon,
+                        const int32_t *pedge, const int32_t *num, const double *e, const double *sedge,
+                        const double *w, const double *d, const double *gradp, const double *p,
+                        double *ad, double *al, double *b, double *rescell, void *stream);
+
+/* fvc_assemble_mom_dev replaces the momentum assembly
+ *   call pform(adU1,auU1,alU1,iaU1,jaU1,num,un,u1,p,p,gradU1,gradU2,gradP,
+ *              div,fluxlU1,iM,rO,w,bU1,rCu1,x,sedge,e,ddU,eddyVisc,ie,
+ *              nelcon,pedge,ix,numel,ndm,nen,nshared,1,ndf,dt,0.0d0,
+ *              matrizU1,.true.,1,1,1,.true.,.true.,.true.,.true.,.true.,bs)
+ * of src/Simple.f:425-431 (comp = 1; :456-462 for u2, comp = 2) for
+ * ELLPACK and cells of types 63/65 (cell_si_eb_i_ggl,
+ * src/CellibSimple.f:591-901, iws = 1).  under_u, s_pressure and simple_c
+ * are the /simple_/ underU, sPressure and simpleC; bs the second-order
+ * backward time term.  Inputs as fvc_assemble_pc_dev with e(10,nmat) =
+ * (viscosity, density, ..., e(6): transient coefficient, e(7): TVD limiter
+ * code 1..7), u(numel) the velocity component, pres(numel) the pressure,
+ * w(2,numel) the velocity, grad1/grad2/gradp(2,numel) the gradients of u,
+ * of the other component and of the pressure.  Writes ad, al, b by
+ * equation as fvc_assemble_pc_dev, rescell(numel), im(comp,:) and
+ * im(comp+2,:) of im(4,numel) (the momentum interpolation terms) and
+ * d(comp,:) of d(2,numel) (the SIMPLE d field).  Bit-identical to the
+ * reference.  A cell with an index out of range or a limiter code outside
+ * 1..7 gets NaN. */
+int fvc_assemble_mom_dev(int32_t numel, int32_t nnode, int32_t nen, int32_t neq, int32_t nmat, int32_t comp,
+                         double dt, double under_u, int32_t s_pressure, int32_t simple_c, int32_t bs,
+                         const double *x, const int32_t *ix, const int32_t *nelcon, const int32_t *pedge,
+                         const int32_t *num, const double *e, const double *sedge, const double *u,
+                         const double *pres, const double *w, const double *grad1, const double *grad2,
+                         const double *gradp, double *d, double *ad, double *al, double *b, double *rescell,
+                         double *im, void *stream);
+
 /* fvc_simple_update_dev replaces
  *   subroutine simpleUpdate(w,pre,u1,u2,preC,gradPc,d,numel,ndm,underU,underPc,simpleR)
  * of src/Simple.f:871-898 (called at :642): w(:,i) = (u1(i),u2(i)) - d(:,i)*gradPc(:,i); and

@@ -1,8 +1,10 @@
-"""The post-solve step on the GPU (fvc_gform_dev, fvc_simple_update_dev)
-against the oracle (oracle/post.py, itself bit-identical to the reference's
-gform and simpleUpdate): bit for bit, on quads and triangles, every
-wall-condition branch, up to the 1M-cell mesh; in place; and an index out of
-range gives NaN for its cell instead of a read out of bounds."""
+"""The SIMPLE steps around the pressure-correction solve on the GPU
+(fvc_assemble_pc_dev, fvc_gform_dev, fvc_simple_update_dev) against the
+oracle (oracle/post.py, itself bit-identical to the reference's pform, gform
+and simpleUpdate): bit for bit, on quads and triangles, every wall-condition
+branch and face-velocity interpolation, up to the 1M-cell mesh; the
+assembled system solved where it was assembled; and an index out of range
+gives NaN for its cell instead of a read out of bounds."""
 import numpy as np
 import pytest
 
@@ -75,3 +77,140 @@ def test_gform_index_out_of_range_gives_nan(fvc):
     bad |= np.any(nelcon == 10, axis=1) | (np.arange(m.neq) == 9)   # cells that gather cell 10's nodes
     assert np.all(np.isnan(gg[5])) and np.all(np.isnan(gg[9])) and np.isnan(ff[5])
     assert np.all(np.isfinite(gg[~bad])) and np.all(ff[~bad] == 1.0)
+
+
+def _pc_case(kind, n, seed, wall_bc=(2, 3, 4, 6), permute=True):
+    m = _mesh(kind, n)
+    a = P.ref_mesh_arrays(m, wall_bc)
+    rng = np.random.default_rng(seed)
+    numel, nsh = a["nelcon"].shape
+    pedge = a["pedge"].copy()
+    pedge[0, nsh] = -1
+    ix = a["ix"].copy()
+    ix[:, nsh] = rng.integers(1, 3, size=numel)
+    num = ((rng.permutation(numel) if permute else np.arange(numel)) + 1).astype(np.int32)
+    e = np.zeros((2, 10))
+    e[:, 1] = (1.3, 0.7)
+    return dict(x=a["x"], ix=ix, nelcon=a["nelcon"], pedge=pedge, num=num, e=e,
+                sedge=rng.normal(size=(numel, nsh + 1, 3)), w=rng.normal(size=(numel, 2)),
+                d=rng.uniform(0.5, 1.5, size=(numel, 2)), gradp=rng.normal(size=(numel, 2)),
+                p=rng.normal(size=numel))
+
+
+def _assemble_on_device(fvc, c, int_vel):
+    from fortranfvcell_amd.post import DeviceMesh
+    numel, nsh = c["nelcon"].shape
+    neq = int(c["num"].max())
+    dm = DeviceMesh(c["x"], c["ix"], c["nelcon"], c["pedge"])
+    dev = {k: fvc.DeviceArray.from_host(c[k]) for k in ("num", "e", "sedge", "w", "d", "gradp", "p")}
+    out = dict(ad=fvc.DeviceArray.empty(neq), al=fvc.DeviceArray.empty((neq, nsh)), b=fvc.DeviceArray.empty(neq),
+               res=fvc.DeviceArray.empty(numel))
+    dm.assemble_pc(dev["num"], dev["e"], dev["sedge"], dev["w"], dev["d"], dev["gradp"], dev["p"],
+                   out["ad"], out["al"], out["b"], out["res"], int_vel=int_vel)
+    fvc.load().fvc_synchronize()
+    return out
+
+
+def _pc_args(c):
+    return (c["x"], c["ix"], c["nelcon"], c["pedge"], c["num"], c["e"], c["sedge"], c["w"], c["d"],
+            c["gradp"], c["p"])
+
+
+@pytest.mark.parametrize("kind,n,int_vel", [("quad", 17, 1), ("quad", 17, 2), ("quad", 17, 5), ("tri", 13, 1),
+                                            ("tri", 13, 2), ("tri", 13, 5), ("quad", 1024, 5), ("tri", 512, 5)])
+def test_assemble_pc_bit_identical_to_oracle(fvc, kind, n, int_vel):
+    c = _pc_case(kind, n, seed=n + int_vel)
+    print(f"assemble_pc {kind} n={n} cells={len(c['num'])} intVel={int_vel}", flush=True)
+    want = O.pform_pc(*_pc_args(c), int_vel=int_vel)
+    got = _assemble_on_device(fvc, c, int_vel)
+    for key, w in zip(("ad", "al", "b", "res"), want):
+        assert np.array_equal(got[key].get(), w), key
+
+
+def test_assemble_pc_then_solve_on_the_device(fvc):
+    # the pressure-correction system of a closed cavity (zero-gradient walls,
+    # the level pinned on cell 1), assembled and solved without leaving HBM,
+    # against a direct solve of the oracle's assembly: the error may not
+    # exceed what the residual allows, ||x - x_ref|| <= ||A^-1|| ||b - A x||
+    # (||A^-1||_2 by power iteration on (A^T A)^-1, which estimates it from
+    # below: hence the factor 2)
+    import scipy.sparse as sp
+    import scipy.sparse.linalg as spla
+    from fortranfvcell_amd.post import ell_pattern
+    c = _pc_case("quad", 128, seed=5, wall_bc=(6, 6, 6, 6), permute=False)
+    ad, al, b, _ = O.pform_pc(*_pc_args(c), int_vel=5)
+    ja = ell_pattern(c["num"], c["nelcon"])
+    neq, w = al.shape
+    rows = np.repeat(np.arange(neq), w)
+    A = sp.csr_matrix((al.ravel(), (rows, ja.ravel() - 1)), shape=(neq, neq)) + sp.diags(ad)
+    x_ref = spla.spsolve(A.tocsc(), b)
+    got = _assemble_on_device(fvc, c, 5)
+    ds = fvc.DeviceSystem.resident_ell(got["ad"], got["al"], got["b"], ja)
+    st = ds.solve(fvc.BICGSTAB, tol=1e-10, maxit=20000, x0_rule=fvc.X0_ZERO, stop_rule=fvc.STOP_RELRES)
+    x = ds.x.get()
+    r = np.linalg.norm(b - A @ x) / np.linalg.norm(b)
+    print(f"p' 128^2: {st['iterations']} BiCGSTAB iterations, relres {r:.2e}", flush=True)
+    assert st["status"] == 0 and r <= 1e-10
+    lu = spla.splu(A.tocsc())
+    v = np.random.default_rng(0).normal(size=neq)
+    for _ in range(40):
+        v = v / np.linalg.norm(v)
+        v = lu.solve(lu.solve(v, trans="T"))
+    inv_norm = np.sqrt(np.linalg.norm(v))
+    err = np.linalg.norm(x - x_ref)
+    bound = 2.0 * inv_norm * np.linalg.norm(b - A @ x)
+    print(f"||x - x_ref|| {err:.2e}, bound {bound:.2e} (||A^-1|| ~ {inv_norm:.2e})", flush=True)
+    assert err <= bound + 1e-12 * np.linalg.norm(x_ref)
+
+
+def _mom_case(kind, n, seed, limiters=(3, 5)):
+    m = _mesh(kind, n)
+    a = P.ref_mesh_arrays(m, (0, 1, 2, 3))
+    rng = np.random.default_rng(seed)
+    numel, nsh = a["nelcon"].shape
+    pedge = a["pedge"].copy()
+    pedge[:, :nsh] = np.where(a["nelcon"] < 0, rng.integers(0, 5, size=(numel, nsh)), 0)
+    pedge[3, nsh] = -2
+    ix = a["ix"].copy()
+    ix[:, nsh] = rng.integers(1, 3, size=numel)
+    e = np.zeros((2, 10))
+    e[:, 0] = (0.01, 0.02)
+    e[:, 1] = (1.3, 0.7)
+    e[:, 5] = (1.0, 2.0)
+    e[:, 6] = limiters
+    return dict(x=a["x"], ix=ix, nelcon=a["nelcon"], pedge=pedge, num=(rng.permutation(numel) + 1).astype(np.int32),
+                e=e, sedge=rng.normal(size=(numel, nsh + 1, 3)), u=rng.normal(size=numel),
+                pres=rng.normal(size=numel), w=rng.normal(size=(numel, 2)), grad1=rng.normal(size=(numel, 2)),
+                grad2=rng.normal(size=(numel, 2)), gradp=rng.normal(size=(numel, 2)),
+                d=rng.uniform(0.5, 1.5, size=(numel, 2)))
+
+
+def _mom_args(c):
+    return (c["x"], c["ix"], c["nelcon"], c["pedge"], c["num"], c["e"], c["sedge"], c["u"], c["pres"], c["w"],
+            c["grad1"], c["grad2"], c["gradp"], c["d"])
+
+
+@pytest.mark.parametrize("kind,n,comp,flags,limiters", [
+    ("quad", 17, 1, 0, (1, 3)), ("quad", 17, 2, 1, (2, 4)), ("tri", 13, 1, 1, (5, 6)), ("tri", 13, 2, 0, (7, 7)),
+    ("quad", 1024, 1, 0, (3, 5)), ("tri", 512, 2, 1, (4, 6))])
+def test_assemble_mom_bit_identical_to_oracle(fvc, kind, n, comp, flags, limiters):
+    from fortranfvcell_amd.post import DeviceMesh
+    kw = [dict(under_u=0.8, s_pressure=True, simple_c=False, bs=False),
+          dict(under_u=1.0, s_pressure=False, simple_c=True, bs=True)][flags]
+    c = _mom_case(kind, n, seed=n + comp, limiters=limiters)
+    numel, nsh = c["nelcon"].shape
+    print(f"assemble_mom {kind} n={n} cells={numel} comp={comp} {kw} limiters={limiters}", flush=True)
+    want = O.pform_mom(*_mom_args(c), comp, 0.01, **kw)
+    neq = int(c["num"].max())
+    dm = DeviceMesh(c["x"], c["ix"], c["nelcon"], c["pedge"])
+    dev = {k: fvc.DeviceArray.from_host(c[k]) for k in ("num", "e", "sedge", "u", "pres", "w", "grad1", "grad2",
+                                                         "gradp", "d")}
+    out = dict(ad=fvc.DeviceArray.empty(neq), al=fvc.DeviceArray.empty((neq, nsh)), b=fvc.DeviceArray.empty(neq),
+               res=fvc.DeviceArray.empty(numel), im=fvc.DeviceArray.from_host(np.zeros((numel, 4))))
+    dm.assemble_mom(comp, dev["num"], dev["e"], dev["sedge"], dev["u"], dev["pres"], dev["w"], dev["grad1"],
+                    dev["grad2"], dev["gradp"], dev["d"], out["ad"], out["al"], out["b"], out["res"], out["im"],
+                    dt=0.01, **kw)
+    fvc.load().fvc_synchronize()
+    got = (out["ad"].get(), out["al"].get(), out["b"].get(), out["res"].get(), out["im"].get(), dev["d"].get())
+    for name, g, w in zip(("ad", "al", "b", "rescell", "im", "d"), got, want):
+        assert np.array_equal(g, w), name
