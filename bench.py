@@ -593,15 +593,18 @@ def main():
         # solved one at a time in FP64 (all resident in HBM)
         systems = P.config_batch()
         fmts = [P.to_ell(q) for q in systems]
-        batch = F.concat_ell(fmts)
-        db = F.DeviceBatch(batch)
+        layouts = {"ell": F.concat_ell(fmts), "csrd": F.concat_csrd([P.to_csrd(q) for q in systems])}
+        batch = layouts["ell"]
         bres = {"systems": len(fmts), "rows": int(batch["row_offsets"][-1])}
         from fortranfvcell_amd import autotune
         sizes = [(int(f["neq"]), int(f["nad"])) for f in fmts]
         for name, prec in (("fp64", F.PREC_FP64), ("mixed", F.PREC_MIXED)):
-            # launch shape (threads per block, block allocation exponent) by the
-            # project autotuner, timing whole batched solves of this batch
-            tuned = autotune.tune_batch(batch, precision=prec)["options"]
+            # layout (ELLPACK or CSRD), threads per block, block allocation
+            # exponent and rows per wavefront by the project autotuner, timing
+            # whole batched solves of this batch
+            tb = autotune.tune_batch(layouts, precision=prec)
+            tuned = tb["options"]
+            db = F.DeviceBatch(layouts[tb["layout"]])
             db.solve(tol=tol, precision=prec, stop_rule=F.STOP_RELRES, **tuned)
             sts = db.solve(tol=tol, precision=prec, stop_rule=F.STOP_RELRES, **tuned)
             t = sts[0]["seconds"]
@@ -615,6 +618,9 @@ def main():
             bres[name] = {"ms": t * 1e3, "solves_per_s": len(sts) / t,
                           "max_iterations": max(q["iterations"] for q in sts),
                           "all_converged": all(q["status"] == 0 for q in sts), "tuned": tuned,
+                          "layout": tb["layout"],
+                          "trials_ms": {f"{q['layout']} {q['options']}": round(q["seconds"] * 1e3, 2)
+                                        for q in tb["trials"]},
                           "algorithmic_bytes": nb, "GBs_algorithmic": nb / t / 1e9,
                           "frac": nb / t / 1e9 / HBM_PEAK_GBS}
         single = 0.0

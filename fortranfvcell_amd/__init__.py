@@ -313,30 +313,57 @@ def concat_ell(fmts: list) -> dict:
                 b=np.ascontiguousarray(np.concatenate([f["b"] for f in fmts]), dtype=np.float64))
 
 
+def concat_csrd(fmts: list) -> dict:
+    """Concatenate CSRD systems (the reference's matriz 1 layout with unsym:
+    ad, then each row's off-diagonals, 1-based ia and ja local to the system)
+    into the batch layout of fvc_solve_batch_csrd."""
+    if any(int(f["matriz"]) != CSRD or f.get("sym", False) for f in fmts):
+        raise ValueError("concat_csrd takes nonsymmetric CSRD systems (problems.to_csrd)")
+    sizes = np.array([int(f["neq"]) for f in fmts], dtype=np.int64)
+    nvals = np.array([len(f["ja"]) for f in fmts], dtype=np.int64)
+    row_off = np.concatenate([[0], np.cumsum(sizes)]).astype(np.int32)
+    val_off = np.concatenate([[0], np.cumsum(nvals)]).astype(np.int32)
+    return dict(matriz=CSRD, nsys=len(fmts), row_offsets=row_off, val_offsets=val_off,
+                ad=np.ascontiguousarray(np.concatenate([f["ad"] for f in fmts]), dtype=np.float64),
+                al=np.ascontiguousarray(np.concatenate([f["al"] for f in fmts]), dtype=np.float64),
+                ia=np.ascontiguousarray(np.concatenate([f["iax"] for f in fmts]), dtype=np.int32),
+                ja=np.ascontiguousarray(np.concatenate([f["ja"] for f in fmts]), dtype=np.int32),
+                b=np.ascontiguousarray(np.concatenate([f["b"] for f in fmts]), dtype=np.float64))
+
+
 def _batch_stats(sts, n):
     return [sts[i].as_dict() for i in range(n)]
 
 
 def solve_batch(batch: dict, tol: float = 1e-10, maxit: int = 50000, x0=None, options=None, **opt):
-    """Jacobi-PCG on every system of a batch (concat_ell) in one persistent
-    launch.  Returns (x concatenated, list of per-system stats)."""
+    """Jacobi-PCG on every system of a batch (concat_ell or concat_csrd) in
+    one persistent launch.  Returns (x concatenated, list of per-system
+    stats)."""
     n = int(batch["row_offsets"][-1])
     x = np.zeros(n) if x0 is None else np.array(x0, dtype=np.float64, copy=True)
     o = options if options is not None else default_options(**opt)
     sts = (Stats * batch["nsys"])()
-    rc = load().fvc_solve_batch(PCG, int(batch["nsys"]), _ptr(batch["row_offsets"]), int(batch["w"]),
-                                _ptr(batch["ad"]), _ptr(batch["al"]), _ptr(x), _ptr(batch["b"]),
-                                _ptr(batch["ja"]), float(tol), int(maxit), C.byref(o), sts)
+    if int(batch.get("matriz", ELLPACK)) == CSRD:
+        rc = load().fvc_solve_batch_csrd(PCG, int(batch["nsys"]), _ptr(batch["row_offsets"]),
+                                         _ptr(batch["val_offsets"]), _ptr(batch["ad"]), _ptr(batch["al"]), _ptr(x),
+                                         _ptr(batch["b"]), _ptr(batch["ia"]), _ptr(batch["ja"]), float(tol),
+                                         int(maxit), C.byref(o), sts)
+    else:
+        rc = load().fvc_solve_batch(PCG, int(batch["nsys"]), _ptr(batch["row_offsets"]), int(batch["w"]),
+                                    _ptr(batch["ad"]), _ptr(batch["al"]), _ptr(x), _ptr(batch["b"]),
+                                    _ptr(batch["ja"]), float(tol), int(maxit), C.byref(o), sts)
     check(rc, _NONFATAL)
     return x, _batch_stats(sts, batch["nsys"])
 
 
 class DeviceBatch:
-    """A batch of ELLPACK systems resident in HBM."""
+    """A batch of ELLPACK (concat_ell) or CSRD (concat_csrd) systems resident
+    in HBM."""
 
     def __init__(self, batch: dict):
         self.nsys = int(batch["nsys"])
-        self.w = int(batch["w"])
+        self.matriz = int(batch.get("matriz", ELLPACK))
+        self.w = int(batch.get("w", 0))
         self.n = int(batch["row_offsets"][-1])
         self.row_offsets = DeviceArray.from_host(batch["row_offsets"])
         self.ad = DeviceArray.from_host(batch["ad"])
@@ -344,12 +371,21 @@ class DeviceBatch:
         self.ja = DeviceArray.from_host(batch["ja"])
         self.b = DeviceArray.from_host(batch["b"])
         self.x = DeviceArray.empty(self.n)
+        if self.matriz == CSRD:
+            self.val_offsets = DeviceArray.from_host(batch["val_offsets"])
+            self.ia = DeviceArray.from_host(batch["ia"])
 
     def solve(self, tol: float = 1e-10, maxit: int = 50000, options=None, stream=None, **opt):
         o = options if options is not None else default_options(**opt)
         sts = (Stats * self.nsys)()
-        rc = load().fvc_solve_batch_dev(PCG, self.nsys, _ptr(self.row_offsets), self.w, _ptr(self.ad),
-                                        _ptr(self.al), _ptr(self.x), _ptr(self.b), _ptr(self.ja),
-                                        float(tol), int(maxit), C.byref(o), sts, stream)
+        if self.matriz == CSRD:
+            rc = load().fvc_solve_batch_csrd_dev(PCG, self.nsys, _ptr(self.row_offsets), _ptr(self.val_offsets),
+                                                 _ptr(self.ad), _ptr(self.al), _ptr(self.x), _ptr(self.b),
+                                                 _ptr(self.ia), _ptr(self.ja), float(tol), int(maxit), C.byref(o),
+                                                 sts, stream)
+        else:
+            rc = load().fvc_solve_batch_dev(PCG, self.nsys, _ptr(self.row_offsets), self.w, _ptr(self.ad),
+                                            _ptr(self.al), _ptr(self.x), _ptr(self.b), _ptr(self.ja),
+                                            float(tol), int(maxit), C.byref(o), sts, stream)
         check(rc, _NONFATAL)
         return _batch_stats(sts, self.nsys)

@@ -30,7 +30,8 @@ ORTH_MGS, ORTH_CGS2 = 0, 1
 EXPORTS = [
     "fvc_default_options", "fvc_init", "fvc_last_error", "fvc_solver", "fvc_solve", "fvc_keep_matrix",
     "fvc_solve_dev", "fvc_solve_dev_noalloc",
-    "fvc_solve_batch", "fvc_solve_batch_dev", "fvc_ell_transpose_dev", "fvc_grad_k_dev",
+    "fvc_solve_batch", "fvc_solve_batch_dev", "fvc_solve_batch_csrd", "fvc_solve_batch_csrd_dev",
+    "fvc_ell_transpose_dev", "fvc_grad_k_dev",
     "fvc_assemble_heat_dev", "fvc_gform_dev", "fvc_simple_update_dev", "fvc_assemble_pc_dev",
     "fvc_assemble_mom_dev",
     "fvc_matvec", "fvc_matvec_dev", "fvc_dot", "fvc_dot_dev", "fvc_getres", "fvc_getres_dev",
@@ -46,7 +47,7 @@ class Options(C.Structure):
                 ("grid_blocks", C.c_int32), ("precision", C.c_int32), ("batch_work_exp10", C.c_int32),
                 ("barrier", C.c_int32), ("recurrence", C.c_int32), ("refresh_every", C.c_int32),
                 ("refresh_drop_e3", C.c_int32), ("row_order", C.c_int32), ("lane_exchange", C.c_int32),
-                ("gmres_orth", C.c_int32)]
+                ("gmres_orth", C.c_int32), ("batch_rows_per_wave", C.c_int32)]
 
 
 class Stats(C.Structure):
@@ -92,6 +93,8 @@ def load(path: str = LIB_PATH):
                                         C.POINTER(Stats), P]
     L.fvc_solve_batch.argtypes = [I, I, P, I, P, P, P, P, P, D, I, C.POINTER(Options), P]
     L.fvc_solve_batch_dev.argtypes = [I, I, P, I, P, P, P, P, P, D, I, C.POINTER(Options), P, P]
+    L.fvc_solve_batch_csrd.argtypes = [I, I, P, P, P, P, P, P, P, P, D, I, C.POINTER(Options), P]
+    L.fvc_solve_batch_csrd_dev.argtypes = [I, I, P, P, P, P, P, P, P, P, D, I, C.POINTER(Options), P, P]
     L.fvc_ell_transpose_dev.argtypes = [I, I, P, P, P, P, P]
     L.fvc_grad_k_dev.argtypes = [I, I, P, P, P, P, P, P, P, P, P, P]
     L.fvc_assemble_heat_dev.argtypes = [I, I, P, P, P, P, P, P, P, P, P, P, P, P, P]

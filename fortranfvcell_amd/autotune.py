@@ -1,11 +1,15 @@
 """Project autotuner: picks the launch shape (threads per block, row chunks =
 blocks, grid-barrier kind) and the storage format (ELLPACK or CSRD) for a
 system by timing short, iteration-capped solves of that system on the GPU,
-and remembers the winner per (device, method, format family, size bucket).
+and remembers the winner per (device, method, format family, size bucket);
+for a batch it picks the layout (ELLPACK or CSRD), threads per block, the
+block-allocation exponent and the rows per wavefront of each system's block
+group by timing whole batched solves.
 
-The kernels process one row per lane (64 rows per wavefront); "rows per
-block" is rows-per-chunk = neq / blocks, which is what the launch shape
-changes.  A cached choice is reused for every system in the same bucket.
+The kernels process one row per lane; a wavefront's rows per sweep are 64,
+and its rows in all are neq / (blocks x waves per block), which is what the
+launch shape (single systems) or the allocation (batches) changes.  A cached
+choice is reused for every system in the same bucket.
 
 Cache file: $FVC_TUNE_CACHE, default ~/.cache/fvcell_tune.json.
 """
@@ -102,19 +106,52 @@ def tune(fmts: dict, method: int, iters: int = 60, use_cache: bool = True, verbo
     return best
 
 
-def tune_batch(batch: dict, precision: int = 0, verbose: bool = False) -> dict:
-    """Best (threads per block, block-allocation exponent) for a batch, by
-    timing whole batched solves (each system runs to its tolerance) of the
-    batch resident in HBM."""
+ROWS_PER_WAVE = (16, 64, 256)
+
+
+def tune_batch(batch, precision: int = 0, verbose: bool = False) -> dict:
+    """Best layout and launch shape for a batch, by timing whole batched
+    solves (each system runs to its tolerance) of the batch resident in HBM.
+
+    batch: one concatenated batch (concat_ell or concat_csrd), or a dict of
+    layouts of the same batch, e.g. {"ell": concat_ell(fmts), "csrd":
+    concat_csrd(fmts)}.  The search is staged, since each probe is a whole
+    batched solve: threads per block x block-allocation exponent on the first
+    layout; then, at the winner, the rows per wavefront of each system's
+    block group (batch_rows_per_wave: a system gets at most neq / (rows x
+    waves per block) blocks, so small systems keep their wavefronts full);
+    then the other layouts at the winning options.  Returns {"layout": name,
+    "options": {...}, "seconds": t, "trials": [...]}."""
     from . import DeviceBatch
-    db = DeviceBatch(batch)
+    layouts = batch if "nsys" not in batch else {"ell" if int(batch.get("matriz", 4)) == 4 else "csrd": batch}
+    names = list(layouts)
+    trials = []
+
+    def timed(name, db, opts):
+        sts = db.solve(tol=1e-10, precision=precision, stop_rule=1, **opts)
+        t = sts[0]["seconds"] if all(q["status"] == 0 for q in sts) else float("inf")
+        trials.append({"layout": name, "options": dict(opts), "seconds": t})
+        if verbose:
+            print(f"tune batch {name} {opts} {t * 1e3:.2f} ms", flush=True)
+        return t
+
     best = None
+    db = DeviceBatch(layouts[names[0]])
     for bs in BLOCK_SIZES:
         for exp10 in (10, 15, 20):
-            sts = db.solve(tol=1e-10, precision=precision, stop_rule=1, block_size=bs, batch_work_exp10=exp10)
-            t = sts[0]["seconds"]
-            if verbose:
-                print(f"tune batch bs={bs} exp={exp10 / 10} {t * 1e3:.2f} ms", flush=True)
+            opts = dict(block_size=bs, batch_work_exp10=exp10)
+            t = timed(names[0], db, opts)
             if best is None or t < best["seconds"]:
-                best = {"options": dict(block_size=bs, batch_work_exp10=exp10), "seconds": t}
+                best = {"layout": names[0], "options": opts, "seconds": t}
+    for rpw in ROWS_PER_WAVE:
+        opts = dict(best["options"], batch_rows_per_wave=rpw)
+        t = timed(names[0], db, opts)
+        if t < best["seconds"]:
+            best = {"layout": names[0], "options": opts, "seconds": t}
+    del db
+    for name in names[1:]:
+        t = timed(name, DeviceBatch(layouts[name]), best["options"])
+        if t < best["seconds"]:
+            best = {"layout": name, "options": dict(best["options"]), "seconds": t}
+    best["trials"] = trials
     return best

@@ -20,13 +20,16 @@ struct SysDesc {
   int neq;
   int g_first;   // first block of its group
   int g_count;   // blocks in its group
+  int val_off;   // CSRD: first stored value of the system (ELLPACK: w row_off)
+  int ia_off;    // CSRD: first entry of its row pointer (row_off + system index)
 };
 
 struct BatchArgs {
-  int w;                    // ELLPACK width
+  int w;                    // ELLPACK width (CSRD: 0)
   const double *ad;         // concatenated ad (Jacobi diagonal as well)
-  const double *av;         // concatenated a(w, .)
-  const int *ja;            // concatenated ja(w, .), local 1-based indices
+  const double *av;         // concatenated a(w, .), or the CSRD values
+  const int *ia;            // CSRD: concatenated row pointers, each local and 1-based
+  const int *ja;            // concatenated ja, local 1-based indices
   const SysDesc *desc;
   const int *block_sys;     // system of each block
   KState *st;               // per system
@@ -64,15 +67,16 @@ __device__ __forceinline__ void batch_grid(const BatchArgs &B, int sys, const Sy
   g.flags = B.flags + D.g_first;
 }
 
+template <int F>
 __device__ __forceinline__ void batch_mat(const BatchArgs &B, const SysDesc &D, Mat &A) {
   const size_t off = (size_t)D.row_off;
   A.neq = D.neq;
   A.w = B.w;
   A.ad = B.ad + off;
-  A.av = B.av + off * B.w;
+  A.av = B.av + D.val_off;
   A.au = nullptr;
-  A.ia = nullptr;
-  A.ja = B.ja + off * B.w;
+  A.ia = F == F_CSRD ? B.ia + D.ia_off : nullptr;
+  A.ja = B.ja + D.val_off;
   A.tptr = A.tsrc = A.tpos = nullptr;
 }
 
@@ -89,7 +93,7 @@ __global__ __launch_bounds__(1024) void pcg_batch_persistent(BatchArgs B) {
   const SysDesc D = B.desc[sys];
   const size_t off = (size_t)D.row_off;
   PcgArgs a{};
-  batch_mat(B, D, a.A);
+  batch_mat<F>(B, D, a.A);
   batch_grid(B, sys, D, a.g);
   a.g.ep = &s_ep;
   a.st = B.st + sys;
@@ -125,11 +129,11 @@ __global__ __launch_bounds__(1024) void mixed_batch_persistent(BatchArgs B) {
   const SysDesc D = B.desc[sys];
   const size_t off = (size_t)D.row_off;
   MixedArgs a{};
-  batch_mat(B, D, a.A);
+  batch_mat<F>(B, D, a.A);
   a.A32.neq = D.neq;
   a.A32.w = B.w;
-  a.A32.av = B.av32 + off * B.w;
-  a.A32.ia = nullptr;
+  a.A32.av = B.av32 + D.val_off;
+  a.A32.ia = a.A.ia;
   a.A32.ja = a.A.ja;
   a.A32.jo = B.jo ? B.jo + off * B.w : nullptr;
   batch_grid(B, sys, D, a.g);
@@ -153,8 +157,9 @@ __global__ __launch_bounds__(1024) void mixed_batch_persistent(BatchArgs B) {
 // S A S in FP32 for every system of the batch (rows are independent; the
 // column indices are local to each system), and with B.jo the int16 column
 // offsets of col_offsets_kernel (local row and column, the same rules).
+// val_off: CSRD, the first value of each system (nsys entries).
 template <int F>
-__global__ void batch_scale_kernel(BatchArgs B, int nsys, const int *row_off) {
+__global__ void batch_scale_kernel(BatchArgs B, int nsys, const int *row_off, const int *val_off) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   // system of row i: binary search in row_off[0..nsys]
   int lo = 0, hi = nsys;
@@ -168,6 +173,18 @@ __global__ void batch_scale_kernel(BatchArgs B, int nsys, const int *row_off) {
   const double si = 1.0 / sqrt(B.ad[i]);
   B.s[i] = si;
   B.dsq[i] = (float)sqrt(B.ad[i]);
+  if constexpr (F == F_CSRD) {
+    // the system's row pointer starts at row_off + lo; positions are local
+    const int *ia = B.ia + off + lo;
+    const int il = i - (int)off;
+    const size_t v0 = (size_t)val_off[lo];
+    for (int k = ia[il] - 1; k < ia[il + 1] - 1; ++k) {
+      const size_t e = v0 + k;
+      const int j = B.ja[e] - 1;
+      B.av32[e] = (float)(B.av[e] * si * (1.0 / sqrt(B.ad[off + j])));
+    }
+    return;
+  }
   const int w = (F == F_ELL4) ? 4 : (F == F_ELL3) ? 3 : B.w;
   bool fits = true;
   for (int k = 0; k < w; ++k) {

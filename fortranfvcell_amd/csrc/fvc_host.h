@@ -55,17 +55,19 @@ inline void build_transpose(int neq, const int *ia, const int *ja, std::vector<i
 
 // Block groups per system: g_s proportional to neq_s^exp (PCG work ~ rows x
 // iterations, iterations ~ sqrt(rows) on these meshes), so that the systems
-// finish together; at least one block each, at most one per 64 rows, the
-// total within the co-resident capacity.  Shares are rounded by largest
+// finish together; at least one block each, at most one per
+// min_rows_per_block rows, the total within the co-resident capacity.  Shares are rounded by largest
 // remainder, so the large systems that set the batch's time are not starved
 // by the one-block floor of the small ones.
-inline std::vector<int> allocate_blocks(const std::vector<int> &neq, int cap, double expo) {
+inline std::vector<int> allocate_blocks(const std::vector<int> &neq, int cap, double expo,
+                                        int min_rows_per_block = 64) {
   const int n = (int)neq.size();
+  const long long mr = std::max(min_rows_per_block, 1);
   std::vector<double> w(n);
   double tot = 0;
   for (int i = 0; i < n; ++i) tot += (w[i] = std::pow((double)std::max(neq[i], 1), expo));
   std::vector<int> g(n, 1), lim(n);
-  for (int i = 0; i < n; ++i) lim[i] = std::max(1, (neq[i] + 63) / 64);
+  for (int i = 0; i < n; ++i) lim[i] = (int)std::max(1LL, ((long long)neq[i] + mr - 1) / mr);
   int used = n;
   if (cap > n) {
     // systems whose share is below one block keep one; the rest share what is left

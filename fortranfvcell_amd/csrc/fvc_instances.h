@@ -10,10 +10,10 @@
 #include "fvc_batch.h"
 
 // Device formats (Fmt): every format for the FP64 solvers, ELLPACK and CSRD
-// for mixed precision, ELLPACK for batches.
+// for mixed precision and for batches.
 #define FVCI_FORMATS_ALL(X, P) X(P, 0) X(P, 1) X(P, 2) X(P, 3) X(P, 4) X(P, 5) X(P, 6)
 #define FVCI_FORMATS_MIXED(X, P) X(P, 0) X(P, 1) X(P, 2) X(P, 3)
-#define FVCI_FORMATS_BATCH(X, P) X(P, 0) X(P, 1) X(P, 2)
+#define FVCI_FORMATS_BATCH(X, P) X(P, 0) X(P, 1) X(P, 2) X(P, 3)
 
 #define FVCI_CG(P, F)                                      \
   P __global__ void fvc::cg1_persistent<F>(fvc::Cg1Args); \
@@ -40,7 +40,7 @@
 #define FVCI_BATCH(P, F)                                               \
   P __global__ void fvc::pcg_batch_persistent<F>(fvc::BatchArgs);   \
   P __global__ void fvc::mixed_batch_persistent<F>(fvc::BatchArgs); \
-  P __global__ void fvc::batch_scale_kernel<F>(fvc::BatchArgs, int, const int *);
+  P __global__ void fvc::batch_scale_kernel<F>(fvc::BatchArgs, int, const int *, const int *);
 #define FVCI_COLOFF(P, W) P __global__ void fvc::col_offsets_kernel<W>(int, const int *, const float *, short *, int *);
 
 // In the host translation unit: declarations only.

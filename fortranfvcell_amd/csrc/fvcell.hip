@@ -1411,9 +1411,10 @@ int dot_impl(const double *a, const double *b, int n, double *result, bool on_de
 // ------------------------------------------------------------- batches --
 
 template <int F>
-int run_batch(DevState &d, int nsys, const std::vector<int> &row_off, int w, const double *ad,
-              const double *av, const int *ja, double *x, const double *b, const int *d_row_off,
-              double tol, int maxit, const fvc_options &o, fvc_stats *stats, hipStream_t s) {
+int run_batch(DevState &d, int nsys, const std::vector<int> &row_off, const std::vector<int> &val_off, int w,
+              const double *ad, const double *av, const int *ia, const int *ja, double *x, const double *b,
+              const int *d_row_off, const int *d_val_off, double tol, int maxit, const fvc_options &o,
+              fvc_stats *stats, hipStream_t s) {
   const bool mixed = o.precision == FVC_PREC_MIXED;
   const int bs = o.block_size > 0 ? o.block_size : kDefaultBlock;
   if (bs % 64 != 0 || bs > 1024) return fail(FVC_BAD_ARGUMENT, "block size %d", bs);
@@ -1429,12 +1430,15 @@ int run_batch(DevState &d, int nsys, const std::vector<int> &row_off, int w, con
   // systems go in waves of at most cap/2 systems (each needs a block)
   const int wave = std::max(1, std::min(nsys, cap / 2 > 0 ? cap / 2 : 1));
   const double expo = o.batch_work_exp10 > 0 ? o.batch_work_exp10 / 10.0 : 1.5;
+  if (o.batch_rows_per_wave < 0) return fail(FVC_BAD_ARGUMENT, "batch_rows_per_wave %d", o.batch_rows_per_wave);
+  const int min_rows = o.batch_rows_per_wave > 0 ? o.batch_rows_per_wave * (bs / 64) : 64;
   const size_t n = (size_t)total;
+  const size_t nv = (size_t)val_off[nsys];   // stored off-diagonal values
   size_t bytes = align_up(((size_t)wave * kCtrWords + cap) * sizeof(unsigned)) + align_up((size_t)wave * sizeof(KState)) +
                  align_up((size_t)kMaxK * cap * sizeof(double)) + align_up((size_t)wave * sizeof(SysDesc)) +
                  align_up((size_t)cap * sizeof(int)) + 4096;
   constexpr bool kOff16 = (F == F_ELL4 || F == F_ELL3);
-  if (mixed) bytes += 2 * align_up(n * sizeof(float4)) + align_up(n * w * sizeof(float)) +
+  if (mixed) bytes += 2 * align_up(n * sizeof(float4)) + align_up(std::max<size_t>(nv, 1) * sizeof(float)) +
                       align_up(n * sizeof(double)) + align_up(n * sizeof(float)) +
                       (kOff16 ? align_up(n * w * sizeof(short)) + align_up(sizeof(int)) : 0);
   else bytes += 5 * align_up(n * sizeof(double));
@@ -1449,6 +1453,7 @@ int run_batch(DevState &d, int nsys, const std::vector<int> &row_off, int w, con
   B.w = w;
   B.ad = ad;
   B.av = av;
+  B.ia = ia;
   B.ja = ja;
   B.x = x;
   B.b = b;
@@ -1463,7 +1468,7 @@ int run_batch(DevState &d, int nsys, const std::vector<int> &row_off, int w, con
   if (mixed) {
     B.fs = align_up(n * sizeof(float4)) / sizeof(float4);
     B.rec = cv.take<float4>(2 * B.fs);
-    B.av32 = cv.take<float>(n * w);
+    B.av32 = cv.take<float>(std::max<size_t>(nv, 1));
     B.s = cv.take<double>(n);
     B.dsq = cv.take<float>(n);
     if (kOff16) {
@@ -1484,7 +1489,8 @@ int run_batch(DevState &d, int nsys, const std::vector<int> &row_off, int w, con
   HIPCHK(hipEventRecord(d.ev0, s));
   if (mixed) {
     if (B.jo) hipLaunchKernelGGL(reset_kernel, dim3(1), dim3(64), 0, s, reinterpret_cast<unsigned *>(B.wide), 1, st, 0);
-    hipLaunchKernelGGL(batch_scale_kernel<F>, dim3((total + 255) / 256), dim3(256), 0, s, B, nsys, d_row_off);
+    hipLaunchKernelGGL(batch_scale_kernel<F>, dim3((total + 255) / 256), dim3(256), 0, s, B, nsys, d_row_off,
+                       d_val_off);
     HIPCHK(hipGetLastError());
     if (B.jo) {
       int h_wide = 1;
@@ -1498,12 +1504,12 @@ int run_batch(DevState &d, int nsys, const std::vector<int> &row_off, int w, con
     const int m = std::min(wave, nsys - s0);
     std::vector<int> sizes(m);
     for (int i = 0; i < m; ++i) sizes[i] = row_off[s0 + i + 1] - row_off[s0 + i];
-    std::vector<int> g = allocate_blocks(sizes, cap, expo);
+    std::vector<int> g = allocate_blocks(sizes, cap, expo, min_rows);
     std::vector<SysDesc> desc(m);
     std::vector<int> bsys;
     int first = 0;
     for (int i = 0; i < m; ++i) {
-      desc[i] = SysDesc{row_off[s0 + i], sizes[i], first, g[i]};
+      desc[i] = SysDesc{row_off[s0 + i], sizes[i], first, g[i], val_off[s0 + i], row_off[s0 + i] + s0 + i};
       for (int k = 0; k < g[i]; ++k) bsys.push_back(i);
       first += g[i];
     }
@@ -1537,46 +1543,93 @@ int run_batch(DevState &d, int nsys, const std::vector<int> &row_off, int w, con
   return FVC_OK;
 }
 
-int batch_impl(int code, int nsys, const int32_t *row_offsets, int w, const double *ad,
-               const double *al, double *x, const double *b, const int32_t *ja, double tol,
-               int maxit, const fvc_options *opt, fvc_stats *stats, bool on_device, hipStream_t s) {
+// Batched Jacobi-PCG over ELLPACK (matriz 4, width w) or CSRD (matriz 1,
+// row pointers ia and value offsets val_offsets) systems.
+int batch_impl(int code, int matriz, int nsys, const int32_t *row_offsets, const int32_t *val_offsets, int w,
+               const double *ad, const double *al, double *x, const double *b, const int32_t *ia,
+               const int32_t *ja, double tol, int maxit, const fvc_options *opt, fvc_stats *stats,
+               bool on_device, hipStream_t s) {
   if (code != FVC_PCG) return fail(FVC_UNSUPPORTED, "batched solves run Jacobi-PCG (code 1)");
-  if (nsys <= 0 || !row_offsets || !ad || !al || !x || !b || !ja) return fail(FVC_BAD_ARGUMENT, "bad batch arguments");
-  if (w < 1 || w > 64) return fail(FVC_BAD_ARGUMENT, "ELLPACK width %d", w);
+  const bool csrd = matriz == FVC_CSRD;
+  if (nsys <= 0 || !row_offsets || !ad || !al || !x || !b || !ja || (csrd && (!ia || !val_offsets)))
+    return fail(FVC_BAD_ARGUMENT, "bad batch arguments");
+  if (!csrd && (w < 1 || w > 64)) return fail(FVC_BAD_ARGUMENT, "ELLPACK width %d", w);
   fvc_options o = opt ? *opt : defaults();
   DevState *d = nullptr;
   int rc = current_device(&d);
   if (rc) return rc;
   std::lock_guard<std::mutex> lk(d->mu);
-  std::vector<int> row_off(nsys + 1);
-  if (on_device) HIPCHK(hipMemcpy(row_off.data(), row_offsets, (nsys + 1) * sizeof(int), hipMemcpyDeviceToHost));
-  else std::memcpy(row_off.data(), row_offsets, (nsys + 1) * sizeof(int));
+  std::vector<int> row_off(nsys + 1), val_off(nsys + 1);
+  const hipMemcpyKind kind = on_device ? hipMemcpyDeviceToHost : hipMemcpyHostToHost;
+  HIPCHK(hipMemcpy(row_off.data(), row_offsets, (nsys + 1) * sizeof(int), kind));
   if (row_off[0] != 0) return fail(FVC_BAD_ARGUMENT, "row_offsets[0] must be 0");
   for (int i = 0; i < nsys; ++i)
     if (row_off[i + 1] <= row_off[i]) return fail(FVC_BAD_ARGUMENT, "system %d is empty", i);
-  if (row_off[nsys] > INT_MAX - 1024) return fail(FVC_UNSUPPORTED, "%d rows overflow 32-bit row indices", row_off[nsys]);
-  const size_t n = (size_t)row_off[nsys];
+  if (row_off[nsys] > INT_MAX - 1024 - nsys)
+    return fail(FVC_UNSUPPORTED, "%d rows overflow 32-bit row indices", row_off[nsys]);
+  if (csrd) {
+    HIPCHK(hipMemcpy(val_off.data(), val_offsets, (nsys + 1) * sizeof(int), kind));
+    if (val_off[0] != 0) return fail(FVC_BAD_ARGUMENT, "val_offsets[0] must be 0");
+    for (int i = 0; i < nsys; ++i)
+      if (val_off[i + 1] < val_off[i]) return fail(FVC_BAD_ARGUMENT, "val_offsets decrease at system %d", i);
+  } else {
+    if ((long long)row_off[nsys] * w > INT_MAX - 1024)
+      return fail(FVC_UNSUPPORTED, "%d rows of width %d overflow 32-bit value indices", row_off[nsys], w);
+    for (int i = 0; i <= nsys; ++i) val_off[i] = row_off[i] * w;
+  }
+  const size_t n = (size_t)row_off[nsys], nv = (size_t)val_off[nsys], nia = n + nsys;
+  if (csrd && !on_device) {
+    // each system's row pointer: 1 at its first row, nondecreasing, ending at
+    // its value count + 1 (so no row reads outside its own values)
+    for (int sys = 0; sys < nsys; ++sys) {
+      const int32_t *p = ia + row_off[sys] + sys;
+      const int ns = row_off[sys + 1] - row_off[sys];
+      if (p[0] != 1 || p[ns] != val_off[sys + 1] - val_off[sys] + 1)
+        return fail(FVC_BAD_ARGUMENT, "system %d: row pointer does not span its %d values", sys,
+                    val_off[sys + 1] - val_off[sys]);
+      for (int r = 0; r < ns; ++r)
+        if (p[r + 1] < p[r]) return fail(FVC_BAD_ARGUMENT, "system %d: row pointer decreases at row %d", sys, r);
+    }
+  }
   Owned own;
   const double *dad = ad, *dav = al, *db = b;
-  const int *dja = ja, *droff = row_offsets;
+  const int *dja = ja, *dia = ia, *droff = row_offsets, *dvoff = val_offsets;
   double *dx = x;
   if (!on_device) {
     double *t1, *t2, *t3, *t4;
     int *i1, *i2;
-    if ((rc = own.up(&t1, ad, n)) || (rc = own.up(&t2, al, n * w)) || (rc = own.up(&t3, b, n)) ||
-        (rc = own.up(&t4, x, n)) || (rc = own.up(&i1, ja, n * w)) ||
+    if ((rc = own.up(&t1, ad, n)) || (rc = own.up(&t2, al, std::max<size_t>(nv, 1))) || (rc = own.up(&t3, b, n)) ||
+        (rc = own.up(&t4, x, n)) || (rc = own.up(&i1, ja, std::max<size_t>(nv, 1))) ||
         (rc = own.up(&i2, (const int *)row_off.data(), (size_t)nsys + 1)))
       return rc;
     dad = t1; dav = t2; db = t3; dx = t4; dja = i1; droff = i2;
+    if (csrd) {
+      int *i3;
+      if ((rc = own.up(&i3, ia, nia))) return rc;
+      dia = i3;
+    }
   } else if (o.precision == FVC_PREC_MIXED) {
     int *i2;
     if ((rc = own.up(&i2, (const int *)row_off.data(), (size_t)nsys + 1))) return rc;
     droff = i2;
   }
-  switch (w) {
-    case 4: rc = run_batch<F_ELL4>(*d, nsys, row_off, w, dad, dav, dja, dx, db, droff, tol, maxit, o, stats, s); break;
-    case 3: rc = run_batch<F_ELL3>(*d, nsys, row_off, w, dad, dav, dja, dx, db, droff, tol, maxit, o, stats, s); break;
-    default: rc = run_batch<F_ELLW>(*d, nsys, row_off, w, dad, dav, dja, dx, db, droff, tol, maxit, o, stats, s); break;
+  if (csrd && o.precision == FVC_PREC_MIXED) {
+    int *i4;
+    if ((rc = own.up(&i4, (const int *)val_off.data(), (size_t)nsys + 1))) return rc;
+    dvoff = i4;
+  }
+  if (csrd) {
+    rc = run_batch<F_CSRD>(*d, nsys, row_off, val_off, 0, dad, dav, dia, dja, dx, db, droff, dvoff, tol, maxit, o,
+                           stats, s);
+  } else {
+    switch (w) {
+      case 4: rc = run_batch<F_ELL4>(*d, nsys, row_off, val_off, w, dad, dav, nullptr, dja, dx, db, droff, nullptr,
+                                     tol, maxit, o, stats, s); break;
+      case 3: rc = run_batch<F_ELL3>(*d, nsys, row_off, val_off, w, dad, dav, nullptr, dja, dx, db, droff, nullptr,
+                                     tol, maxit, o, stats, s); break;
+      default: rc = run_batch<F_ELLW>(*d, nsys, row_off, val_off, w, dad, dav, nullptr, dja, dx, db, droff,
+                                      nullptr, tol, maxit, o, stats, s); break;
+    }
   }
   if (!on_device && (rc == FVC_OK || rc == FVC_MAXIT || rc == FVC_BREAKDOWN || rc == FVC_NONFINITE))
     HIPCHK(hipMemcpy(x, dx, n * sizeof(double), hipMemcpyDeviceToHost));
@@ -1653,15 +1706,32 @@ int fvc_solve_batch(int32_t code, int32_t nsys, const int32_t *row_offsets, int3
                     const double *ad, const double *al, double *x, const double *b,
                     const int32_t *ja, double tol, int32_t maxit, const fvc_options *opt,
                     fvc_stats *stats) {
-  return batch_impl(code, nsys, row_offsets, w, ad, al, x, b, ja, tol, maxit, opt, stats, false, nullptr);
+  return batch_impl(code, FVC_ELLPACK, nsys, row_offsets, nullptr, w, ad, al, x, b, nullptr, ja, tol, maxit, opt,
+                    stats, false, nullptr);
 }
 
 int fvc_solve_batch_dev(int32_t code, int32_t nsys, const int32_t *row_offsets, int32_t w,
                         const double *ad, const double *al, double *x, const double *b,
                         const int32_t *ja, double tol, int32_t maxit, const fvc_options *opt,
                         fvc_stats *stats, void *stream) {
-  return batch_impl(code, nsys, row_offsets, w, ad, al, x, b, ja, tol, maxit, opt, stats, true,
-                    (hipStream_t)stream);
+  return batch_impl(code, FVC_ELLPACK, nsys, row_offsets, nullptr, w, ad, al, x, b, nullptr, ja, tol, maxit, opt,
+                    stats, true, (hipStream_t)stream);
+}
+
+int fvc_solve_batch_csrd(int32_t code, int32_t nsys, const int32_t *row_offsets,
+                         const int32_t *val_offsets, const double *ad, const double *a, double *x,
+                         const double *b, const int32_t *ia, const int32_t *ja, double tol,
+                         int32_t maxit, const fvc_options *opt, fvc_stats *stats) {
+  return batch_impl(code, FVC_CSRD, nsys, row_offsets, val_offsets, 0, ad, a, x, b, ia, ja, tol, maxit, opt, stats,
+                    false, nullptr);
+}
+
+int fvc_solve_batch_csrd_dev(int32_t code, int32_t nsys, const int32_t *row_offsets,
+                             const int32_t *val_offsets, const double *ad, const double *a, double *x,
+                             const double *b, const int32_t *ia, const int32_t *ja, double tol,
+                             int32_t maxit, const fvc_options *opt, fvc_stats *stats, void *stream) {
+  return batch_impl(code, FVC_CSRD, nsys, row_offsets, val_offsets, 0, ad, a, x, b, ia, ja, tol, maxit, opt, stats,
+                    true, (hipStream_t)stream);
 }
 
 int fvc_ell_transpose_dev(int32_t neq, int32_t w, const int32_t *ja, const int32_t *back,

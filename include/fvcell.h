@@ -122,6 +122,10 @@ typedef struct fvc_options {
                              (off: measured slower on MI355X), 1 on, 2 off           */
   int32_t gmres_orth;     /* GMRES orthogonalisation: FVC_ORTH_MGS (default) or
                              FVC_ORTH_CGS2                                           */
+  int32_t batch_rows_per_wave; /* batches: at least this many rows per wavefront of a
+                             system's block group (a system gets at most
+                             ceil(neq / (rows x waves per block)) blocks); 0 = the
+                             default of 64 rows per block                            */
 } fvc_options;
 
 /* fvc_options.gmres_orth */
@@ -243,6 +247,23 @@ int fvc_solve_batch_dev(int32_t code, int32_t nsys, const int32_t *row_offsets, 
                         const double *ad, const double *al, double *x, const double *b,
                         const int32_t *ja, double tol, int32_t maxit, const fvc_options *opt,
                         fvc_stats *stats, void *stream);
+
+/* The same for CSRD systems (the reference's matriz 1 with unsym: ad, then
+ * the off-diagonals of each row, src/Datastruct.f:40-77): system s owns rows
+ * row_offsets[s] .. row_offsets[s+1]-1 of ad, b, x, the stored values and
+ * columns val_offsets[s] .. val_offsets[s+1]-1 of a and ja (columns local and
+ * 1-based), and the row pointer ia[row_offsets[s] + s .. row_offsets[s+1] + s]
+ * (neq_s + 1 entries, local and 1-based: the system's own reference ia).  No
+ * counterpart in the reference; the autotuner chooses between this layout
+ * and ELLPACK per batch. */
+int fvc_solve_batch_csrd(int32_t code, int32_t nsys, const int32_t *row_offsets,
+                         const int32_t *val_offsets, const double *ad, const double *a, double *x,
+                         const double *b, const int32_t *ia, const int32_t *ja, double tol,
+                         int32_t maxit, const fvc_options *opt, fvc_stats *stats);
+int fvc_solve_batch_csrd_dev(int32_t code, int32_t nsys, const int32_t *row_offsets,
+                             const int32_t *val_offsets, const double *ad, const double *a, double *x,
+                             const double *b, const int32_t *ia, const int32_t *ja, double tol,
+                             int32_t maxit, const fvc_options *opt, fvc_stats *stats, void *stream);
 
 /* ----------------------------------------------------------------------------
  * Differentiable solve (no counterpart in the reference).

@@ -44,10 +44,32 @@ def test_fortran_dropin_exports_solver_and_getres():
     assert "solver_" in syms and "getres_" in syms
 
 
-def test_options_and_stats_layout_match_header():
+def _header_layout(tmp_path):
+    """sizeof and field offsets of fvc_options and fvc_stats as a C compiler
+    lays out include/fvcell.h."""
     from fortranfvcell_amd import _lib
-    assert C.sizeof(_lib.Options) == 16 * 4
-    assert C.sizeof(_lib.Stats) == 4 * 4 + 5 * 8 + 2 * 4 + 4 * 8
+    src = tmp_path / "layout.c"
+    lines = ['#include <stdio.h>', '#include <stddef.h>', f'#include "{HEADER}"', 'int main(void) {',
+             'printf("options %zu\\n", sizeof(fvc_options));', 'printf("stats %zu\\n", sizeof(fvc_stats));']
+    for struct, cls in (("fvc_options", _lib.Options), ("fvc_stats", _lib.Stats)):
+        for name, _ in cls._fields_:
+            lines.append(f'printf("{struct}.{name} %zu\\n", offsetof({struct}, {name}));')
+    lines += ['return 0;', '}']
+    src.write_text("\n".join(lines) + "\n")
+    exe = tmp_path / "layout"
+    subprocess.run(["gcc", "-o", str(exe), str(src)], check=True)
+    out = subprocess.run([str(exe)], capture_output=True, text=True, check=True).stdout
+    return dict(line.rsplit(" ", 1) for line in out.splitlines())
+
+
+def test_options_and_stats_layout_match_header(tmp_path):
+    from fortranfvcell_amd import _lib
+    lay = _header_layout(tmp_path)
+    assert C.sizeof(_lib.Options) == int(lay["options"])
+    assert C.sizeof(_lib.Stats) == int(lay["stats"])
+    for struct, cls in (("fvc_options", _lib.Options), ("fvc_stats", _lib.Stats)):
+        for name, _ in cls._fields_:
+            assert getattr(cls, name).offset == int(lay[f"{struct}.{name}"]), (struct, name)
     o = _lib.default_options()
     assert o.x0_rule == _lib.X0_AS_OPENMP and o.gmres_k == 15 and o.engine == _lib.ENGINE_AUTO
     assert o.row_order == 0

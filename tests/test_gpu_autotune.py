@@ -32,11 +32,19 @@ def test_tune_single_system(fvc, tmp_path, monkeypatch):
 
 
 def test_tune_batch(fvc):
+    # layout (ELLPACK or CSRD), threads per block, allocation exponent and
+    # rows per wavefront, each probe a whole batched solve; the winner
+    # solves every system
     from fortranfvcell_amd import autotune
     sides = P.config_batch_sides(24, 8, 64, 3)
-    fmts = [P.to_ell(P.pressure_system(P.quad_mesh(int(n)), seed=100 + i)) for i, n in enumerate(sides)]
-    batch = fvc.concat_ell(fmts)
-    best = autotune.tune_batch(batch, verbose=True)
-    assert best["options"]["block_size"] in autotune.BLOCK_SIZES
-    x, sts = fvc.solve_batch(batch, tol=1e-10, stop_rule=fvc.STOP_RELRES, **best["options"])
+    systems = [P.pressure_system(P.quad_mesh(int(n)), seed=100 + i) for i, n in enumerate(sides)]
+    layouts = {"ell": fvc.concat_ell([P.to_ell(s) for s in systems]),
+               "csrd": fvc.concat_csrd([P.to_csrd(s) for s in systems])}
+    best = autotune.tune_batch(layouts, verbose=True)
+    assert best["layout"] in layouts and best["options"]["block_size"] in autotune.BLOCK_SIZES
+    tried = {(q["layout"], q["options"].get("batch_rows_per_wave", 0)) for q in best["trials"]}
+    assert ("csrd", best["options"].get("batch_rows_per_wave", 0)) in tried
+    assert {r for l, r in tried if l == "ell"} >= set(autotune.ROWS_PER_WAVE)
+    assert best["seconds"] == min(q["seconds"] for q in best["trials"])
+    x, sts = fvc.solve_batch(layouts[best["layout"]], tol=1e-10, stop_rule=fvc.STOP_RELRES, **best["options"])
     assert all(st["status"] == 0 for st in sts)

@@ -141,3 +141,59 @@ def test_full_batch_config_every_system_converges(fvc, full_batch, precision, ba
         worst = max(worst, np.linalg.norm(s.b - P.scipy_csr(s) @ xi) / np.linalg.norm(s.b))
     print(f"  worst true relative residual {worst:.2e}", flush=True)
     assert worst <= 1.01e-10
+
+
+@pytest.mark.parametrize("precision", [0, 1])
+def test_batch_csrd_matches_ellpack(fvc, precision):
+    # the same uneven batch in CSRD (per-system row pointers, local columns)
+    # and ELLPACK: every system to true 1e-10, the two solutions within the
+    # bar (the two layouts sum a row in different orders: the reference's
+    # CSRD from the diagonal, its ELLPACK to it)
+    systems = _systems(20, 8, 64, 11)
+    ell = fvc.concat_ell([P.to_ell(s) for s in systems])
+    csr = fvc.concat_csrd([P.to_csrd(s) for s in systems])
+    print(f"CSRD batch of {len(systems)} systems, {csr['row_offsets'][-1]} rows, {csr['val_offsets'][-1]} values, "
+          f"precision={precision}", flush=True)
+    xe, se = fvc.solve_batch(ell, tol=1e-10, precision=precision, stop_rule=fvc.STOP_RELRES)
+    xc, sc = fvc.solve_batch(csr, tol=1e-10, precision=precision, stop_rule=fvc.STOP_RELRES)
+    ro = csr["row_offsets"]
+    for i, s in enumerate(systems):
+        assert sc[i]["status"] == 0 and se[i]["status"] == 0, (i, sc[i], se[i])
+        xi = xc[ro[i]:ro[i + 1]]
+        r = np.linalg.norm(s.b - P.scipy_csr(s) @ xi) / np.linalg.norm(s.b)
+        assert r <= 1e-10 * 1.01, (i, r)
+        ye = xe[ro[i]:ro[i + 1]]
+        assert np.linalg.norm(xi - ye) <= 1e-8 * np.linalg.norm(ye), i
+    # device-resident CSRD batch
+    db = fvc.DeviceBatch(csr)
+    sts = db.solve(tol=1e-10, precision=precision, stop_rule=fvc.STOP_RELRES)
+    assert all(q["status"] == 0 for q in sts)
+    assert np.linalg.norm(db.x.get() - xc) <= 1e-8 * np.linalg.norm(xc)
+
+
+@pytest.mark.parametrize("rpw", [16, 256])
+def test_batch_rows_per_wave(fvc, rpw):
+    # fewer blocks for the small systems (at most neq / (rows x waves per
+    # block) each): every system still converges to true 1e-10
+    systems = _systems(30, 8, 96, 13)
+    batch = fvc.concat_ell([P.to_ell(s) for s in systems])
+    x, sts = fvc.solve_batch(batch, tol=1e-10, stop_rule=fvc.STOP_RELRES, batch_rows_per_wave=rpw, block_size=256)
+    ro = batch["row_offsets"]
+    for i, s in enumerate(systems):
+        assert sts[i]["status"] == 0
+        xi = x[ro[i]:ro[i + 1]]
+        assert np.linalg.norm(s.b - P.scipy_csr(s) @ xi) <= 1.01e-10 * np.linalg.norm(s.b)
+    with pytest.raises(fvc.FvcError):
+        fvc.solve_batch(batch, tol=1e-10, batch_rows_per_wave=-1)
+
+
+def test_batch_csrd_refusals(fvc):
+    systems = _systems(3, 8, 16, 17)
+    csr = fvc.concat_csrd([P.to_csrd(s) for s in systems])
+    bad = dict(csr, ia=csr["ia"].copy())
+    bad["ia"][int(csr["row_offsets"][1]) + 1 + 2] = 10_000     # a row pointer past the system's values
+    with pytest.raises(fvc.FvcError):
+        fvc.solve_batch(bad, tol=1e-10)
+    bad = dict(csr, val_offsets=csr["val_offsets"][::-1].copy())
+    with pytest.raises(fvc.FvcError):
+        fvc.solve_batch(bad, tol=1e-10)
