@@ -46,12 +46,13 @@ _ref = None
 
 def build(ref: bool | None = None) -> None:
     """Compile the C restatement; when the reference's sources exist, also the
-    reference's solver path (oracle/_ref/libfvcref.so) and, once the drop-in
-    is built, the reference program's objects linked against it
+    reference's solver path (oracle/_ref/libfvcref.so), the reference program
+    linked with its own solver (oracle/_ref/libmvf_ref.so) and, once the
+    drop-in is built, the same program linked against the drop-in
     (oracle/_ref/libmvf_dropin.so)."""
     targets = ["oracle"]
     if ref or (ref is None and os.path.isdir("/root/reference/src")):
-        targets.append("ref")
+        targets += ["ref", "refprog"]
         pkg = os.path.join(os.path.dirname(HERE), "fortranfvcell_amd")
         if all(os.path.exists(os.path.join(pkg, f)) for f in ("libfvcell.so", "libfvcell_fortran.a")):
             targets.append("dropin")
