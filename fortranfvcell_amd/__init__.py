@@ -18,7 +18,7 @@ import numpy as np
 
 from . import _lib
 from ._lib import (BICGSTAB, CSR, CSRC, CSRD, ELLPACK, ENGINE_AUTO, ENGINE_KERNELS,
-                   ENGINE_PERSISTENT, FLAG_COL16, FLAG_TILES, GMRES, ORTH_CGS2, ORTH_MGS, PCG,
+                   ENGINE_PERSISTENT, FLAG_COL16, FLAG_TILES, GMRES, ORTH_CGS2, ORTH_CGS2_LAG, ORTH_MGS, PCG,
                    PREC_FP64, PREC_MIXED,
                    REC_FUSED, REC_REFERENCE,
                    STOP_REFERENCE,

@@ -24,7 +24,7 @@ ENGINE_AUTO, ENGINE_KERNELS, ENGINE_PERSISTENT = 0, 1, 2
 PREC_FP64, PREC_MIXED = 0, 1
 REC_FUSED, REC_REFERENCE = 0, 1
 FLAG_COL16, FLAG_TILES = 1, 2
-ORTH_MGS, ORTH_CGS2 = 0, 1
+ORTH_MGS, ORTH_CGS2, ORTH_CGS2_LAG = 0, 1, 2
 
 # Every symbol include/fvcell.h declares.
 EXPORTS = [

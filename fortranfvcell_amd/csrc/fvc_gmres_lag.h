@@ -1,0 +1,301 @@
+// fvc_gmres_lag.h -- GMRES(k) with one grid reduction per Arnoldi step:
+// classical Gram-Schmidt with one re-orthogonalisation whose second pass and
+// normalisation are lagged by one step (the "DCGS2" Arnoldi of Swirydowicz,
+// Langou, Ananthan, Yang & Thomas, Numer. Linear Algebra Appl. 2021, and
+// Bielich et al., Parallel Computing 2022).  The restart loop, the Givens
+// rotations, the convergence test and the back substitution are those of the
+// reference's gmres (src/Iterativos.f:271-446, Givens :363-375, back
+// substitution :382-389), shared with the MGS and CGS2 variants
+// (gmres_close_step); only the orthogonalisation differs.
+//
+// Step j holds V_{j-1} = [v_1 .. v_{j-1}] (orthonormal, final) and the
+// candidate u_j, already projected once against V_{j-1}.  Three launches:
+//   D_j  w = M^-1 A u_j and, in the same sweep, a = V_{j-1}' u_j, beta = u_j'u_j,
+//        b = V_{j-1}' w, gamma = u_j'w: 2j dot products, written per block;
+//   R_j  one block sums the partials in a fixed order and runs the scalar
+//        part: nu = sqrt(beta - a'a) (u_j - V a is orthogonal to V), the
+//        finished column j-1 of H (first pass + a, and nu below it), its
+//        Givens rotation and the convergence test; then, from the Arnoldi
+//        relation M^-1 A V_{j-1} = V_j H(1:j, 1:j-1), the first-pass
+//        coefficients of column j, h = (V_j'w - H a) / nu with
+//        V_j'w = [b; (gamma - a'b) / nu], and c = H a / nu + h;
+//   B_j  v_j = (u_j - V_{j-1} a) / nu into column j, and the next candidate
+//        u_{j+1} = w / nu - V_j c, in place.
+// Per step the basis is read twice (D and B) instead of 2j+ times (MGS), and
+// the only grid-wide dependences are the kernel boundaries.  Step k+1 only
+// re-orthogonalises u_{k+1} (column k), without its SpMV.
+#pragma once
+#include "fvc_krylov.h"
+
+namespace fvc {
+
+constexpr int kLagChunk = 8;      // basis columns per pass of D's dot products
+constexpr int kMaxGmresLag = 64;  // largest restart length of this variant
+
+// Value q of step j's reduction: a_1..a_{j-1} at 0..j-2, beta at j-1,
+// b_1..b_{j-1} at j..2j-2, gamma at 2j-1.  Partials: part[block * kp + q].
+__host__ __device__ inline int lag_kp(int k) { return 2 * (k + 1); }
+
+template <int F>
+__global__ __launch_bounds__(256) void gmres_lag_d_kernel(GmresArgs a) {
+  __shared__ double lds[2 * kLagChunk * kMaxWaves];
+  const KState *st = a.st;
+  if (ld_int(&st->done) || ld_int(&st->inner_done)) return;
+  const int j = ld_int(&st->step_i);
+  const int nv = j - 1;
+  const bool last = j == a.k + 1;   // column k's re-orthogonalisation only
+  const int i0 = a.g.first(), i1 = a.g.row_end(), is = a.g.stride();
+  double *pout = a.lag_part + (size_t)a.g.chunk() * a.lag_kp;
+  {
+    double v[2] = {0.0, 0.0};
+    for (int r = i0; r < i1; r += is) {
+      const double ur = a.gu[r];
+      v[0] += ur * ur;
+      if (!last) {
+        const double wr = row_apply<F>(a.A, r, XVec{a.gu}) / a.m[r];
+        a.gw[r] = wr;
+        v[1] += ur * wr;
+      }
+    }
+    block_sum<2>(v, lds);
+    if (threadIdx.x == 0) {
+      pout[nv] = v[0];
+      pout[2 * j - 1] = v[1];
+    }
+  }
+  for (int c0 = 0; c0 < nv; c0 += kLagChunk) {
+    double v[2 * kLagChunk];
+#pragma unroll
+    for (int q = 0; q < 2 * kLagChunk; ++q) v[q] = 0.0;
+    for (int r = i0; r < i1; r += is) {
+      const double ur = a.gu[r];
+      const double wr = last ? 0.0 : a.gw[r];
+      // the chunk's columns loaded together; a column past nv reads column
+      // nv again (an L1 hit), so no load waits behind a branch
+      double g[kLagChunk];
+#pragma unroll
+      for (int q = 0; q < kLagChunk; ++q) g[q] = a.gv[(size_t)min(c0 + q, nv - 1) * a.ld + r];
+#pragma unroll
+      for (int q = 0; q < kLagChunk; ++q) {
+        if (c0 + q < nv) {
+          v[q] += g[q] * ur;
+          v[kLagChunk + q] += g[q] * wr;
+        }
+      }
+    }
+    block_sum<2 * kLagChunk>(v, lds);
+    if (threadIdx.x == 0) {
+#pragma unroll
+      for (int q = 0; q < kLagChunk; ++q) {
+        if (c0 + q < nv) {
+          pout[c0 + q] = v[q];
+          pout[j + c0 + q] = v[kLagChunk + q];
+        }
+      }
+    }
+  }
+}
+
+// One block: the reduction of D_j's partials (each value over the G blocks in
+// block order, then the block's fixed tree) and the scalar part of step j.
+// Everything the scalar part reads is first loaded into LDS by all threads
+// together, so the one thread that runs the sequential part (the Givens
+// rotations, the back substitution) waits on no global memory.  (Templated
+// on the format like D and B only so that every family of solver kernels is
+// instantiated in one translation unit.)
+template <int F>
+__global__ __launch_bounds__(1024) void gmres_lag_r_kernel(GmresArgs a) {
+  constexpr int L = kMaxGmresLag + 1;
+  __shared__ double lds[16 * kMaxWaves];
+  __shared__ double tot[2 * L];
+  __shared__ double hr[L * L];          // H before the rotations, rows and columns 1..j
+  __shared__ double hc[L + 1];          // H(1:j, 1:j-1) a
+  __shared__ double cs[L + 1], sn[L + 1], ee[L + 2];
+  __shared__ double col[L + 1];         // column j-1 under rotation; y at the end
+  __shared__ int s_ni;
+  KState *st = a.st;
+  if (ld_int(&st->done) || ld_int(&st->inner_done)) return;
+  const int j = ld_int(&st->step_i);
+  const int nv = j - 1, K = 2 * j, k = a.k;
+  const int G = a.g.G;
+  auto HR = [&](int r, int c) -> double & { return hr[(c - 1) * L + (r - 1)]; };
+  auto Hg = [&](double *h, int r, int c) -> double & { return h[(size_t)(c - 1) * (k + 1) + (r - 1)]; };
+  // the small state, loaded by all threads at once
+  for (int t = threadIdx.x; t < j * nv; t += blockDim.x) {
+    const int r = t % j + 1, c = t / j + 1;
+    HR(r, c) = r <= c + 1 ? Hg(a.hraw, r, c) : 0.0;
+  }
+  for (int t = threadIdx.x; t <= j; t += blockDim.x) {
+    cs[t] = a.cs[t];
+    sn[t] = a.sn[t];
+    ee[t] = a.e[t];
+  }
+  // D_j's partials, 16 values at a time; the buffer holds 16 values past the
+  // last block's, so every load is unconditional
+  for (int q0 = 0; q0 < K; q0 += 16) {
+    double v[16];
+#pragma unroll
+    for (int q = 0; q < 16; ++q) v[q] = 0.0;
+    for (int b = threadIdx.x; b < G; b += blockDim.x) {
+      const double *p = a.lag_part + (size_t)b * a.lag_kp + q0;
+      double w[16];
+#pragma unroll
+      for (int q = 0; q < 16; ++q) w[q] = p[q];
+#pragma unroll
+      for (int q = 0; q < 16; ++q)
+        if (q0 + q < K) v[q] += w[q];
+    }
+    block_sum<16>(v, lds);
+    if (threadIdx.x == 0) {
+#pragma unroll
+      for (int q = 0; q < 16; ++q)
+        if (q0 + q < K) tot[q0 + q] = v[q];
+    }
+  }
+  __syncthreads();
+  // H(1:j, 1:j-1) a with column j-1 as it stands (its correction follows)
+  for (int r = threadIdx.x + 1; r <= j; r += blockDim.x) {
+    double s = 0.0;
+    for (int c = r > 1 ? r - 1 : 1; c <= nv; ++c) s += HR(r, c) * tot[c - 1];   // Hessenberg: c >= r - 1
+    hc[r - 1] = s;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    s_ni = 0;
+    double aa = 0.0, ab = 0.0;
+    for (int q = 0; q < nv; ++q) {
+      aa += tot[q] * tot[q];
+      ab += tot[q] * tot[j + q];
+    }
+    const double beta = tot[nv], gamma = tot[2 * j - 1];
+    const double nu2 = beta - aa;
+    const double nu = nu2 > 0.0 ? sqrt(nu2) : 0.0;
+    bool ended = false;
+    if (j == 1) {
+      a.e[1] = nu;   // ||r0 / m||: the least-squares right-hand side
+      ee[1] = nu;
+      a.nrm[1] = nu;
+    } else {
+      // column i = j-1 is finished: first pass + re-orthogonalisation, nu below
+      const int i = nv;
+      for (int r = 1; r <= i; ++r) HR(r, i) = HR(r, i) + tot[r - 1];
+      HR(j, i) = nu;
+      for (int r = 1; r <= j; ++r) Hg(a.hraw, r, i) = HR(r, i);
+      // (H a)_r with the finished column: the a_{j-1} a_r term it gained
+      for (int r = 1; r <= i; ++r) hc[r - 1] += tot[r - 1] * tot[i - 1];
+      hc[i] = nu * tot[i - 1];
+      // the Givens rotations of src/Iterativos.f:363-375, as gmres_close_step
+      for (int r = 1; r <= j; ++r) col[r] = HR(r, i);
+      for (int jj = 1; jj <= i - 1; ++jj) {
+        const double aux1 = cs[jj] * col[jj] + sn[jj] * col[jj + 1];
+        const double aux2 = -sn[jj] * col[jj] + cs[jj] * col[jj + 1];
+        col[jj] = aux1;
+        col[jj + 1] = aux2;
+      }
+      const double rr = sqrt(col[i] * col[i] + col[i + 1] * col[i + 1]);
+      cs[i] = col[i] / rr;
+      sn[i] = col[i + 1] / rr;
+      col[i] = rr;
+      col[i + 1] = 0.0;
+      ee[i + 1] = -sn[i] * ee[i];
+      ee[i] = cs[i] * ee[i];
+      for (int r = 1; r <= j; ++r) Hg(a.h, r, i) = col[r];
+      a.cs[i] = cs[i];
+      a.sn[i] = sn[i];
+      a.e[i] = ee[i];
+      a.e[i + 1] = ee[i + 1];
+      st->inner = st->inner + 1;
+      const bool conv = fabs(ee[i + 1]) <= st->econv;   // src/Iterativos.f:376
+      if (conv || i == k || !finite(ee[i + 1])) {
+        st->ni = i;
+        st->inner_done = 1;
+        s_ni = i;
+        ended = true;
+      }
+    }
+    if (!ended) {
+      if (!(nu > 0.0) || !finite(nu)) {
+        // u_j lies in span(V_{j-1}) (or is not finite): no column j
+        st->ni = nv;
+        st->inner_done = 1;
+        s_ni = nv;
+        if (!finite(nu)) {
+          st->status = S_NONFINITE;
+          st->done = 1;
+          s_ni = 0;
+        }
+      } else {
+        // column j, first pass: h = (V_j' w - H a) / nu; next candidate's c
+        double *la = a.lag;   // la[0] = nu, la[1..j-1] = a, la[k+1..k+j] = c
+        la[0] = nu;
+        for (int r = 1; r <= nv; ++r) la[r] = tot[r - 1];
+        for (int r = 1; r <= j; ++r) {
+          const double vtw = r < j ? tot[j + r - 1] : (gamma - ab) / nu;
+          const double h1 = (vtw - hc[r - 1]) / nu;
+          Hg(a.hraw, r, j) = h1;
+          la[k + r] = hc[r - 1] / nu + h1;
+        }
+        a.nrm[j] = nu;
+        st->step_i = j + 1;
+      }
+    }
+  }
+  __syncthreads();
+  const int ni = s_ni;
+  if (ni == 0) return;
+  // the cycle ended at ni steps: back substitution h y = e
+  // (src/Iterativos.f:382-389) on the rotated H, loaded into LDS first
+  for (int t = threadIdx.x; t < ni * ni; t += blockDim.x) {
+    const int r = t % ni + 1, c = t / ni + 1;
+    HR(r, c) = r <= c ? Hg(a.h, r, c) : 0.0;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double *y = col;
+    y[ni] = ee[ni] / HR(ni, ni);
+    for (int ii = ni - 1; ii >= 1; --ii) {
+      double yi = 0.0;
+      for (int jj = ii + 1; jj <= ni; ++jj) yi = yi - HR(ii, jj) * y[jj];
+      y[ii] = (yi + ee[ii]) / HR(ii, ii);
+    }
+    for (int ii = 1; ii <= ni; ++ii) a.y[ii] = y[ii];
+  }
+}
+
+// v_j = (u_j - V_{j-1} a) / nu into column j; u <- w / nu - V_j c.
+template <int F>
+__global__ __launch_bounds__(256) void gmres_lag_b_kernel(GmresArgs a) {
+  __shared__ double ca[kMaxGmresLag + 1], cc[kMaxGmresLag + 2];
+  const KState *st = a.st;
+  if (ld_int(&st->done) || ld_int(&st->inner_done)) return;
+  const int j = ld_int(&st->step_i) - 1;
+  const int nv = j - 1, k = a.k;
+  for (int q = threadIdx.x; q <= k + 1; q += blockDim.x) {
+    if (q <= nv) ca[q] = ld_dbl(a.lag + q);        // ca[0] = nu
+    if (q >= 1 && q <= j) cc[q] = ld_dbl(a.lag + k + q);
+  }
+  __syncthreads();
+  const double nu = ca[0];
+  const int i0 = a.g.first(), i1 = a.g.row_end(), is = a.g.stride();
+  double *vj = a.gv + (size_t)nv * a.ld;
+  for (int r = i0; r < i1; r += is) {
+    double s1 = a.gu[r], s2 = 0.0;
+    for (int c0 = 0; c0 < nv; c0 += kLagChunk) {
+      double g[kLagChunk];
+#pragma unroll
+      for (int q = 0; q < kLagChunk; ++q) g[q] = a.gv[(size_t)min(c0 + q, nv - 1) * a.ld + r];
+#pragma unroll
+      for (int q = 0; q < kLagChunk; ++q) {
+        if (c0 + q < nv) {
+          s1 = s1 - ca[c0 + q + 1] * g[q];
+          s2 = s2 + cc[c0 + q + 1] * g[q];
+        }
+      }
+    }
+    const double v = s1 / nu;
+    vj[r] = v;
+    a.gu[r] = a.gw[r] / nu - (s2 + cc[j] * v);
+  }
+}
+
+}  // namespace fvc

@@ -8,6 +8,7 @@
 #include "fvc_mixed.h"
 #include "fvc_mixed_krylov.h"
 #include "fvc_batch.h"
+#include "fvc_gmres_lag.h"
 
 // Device formats (Fmt): every format for the FP64 solvers, ELLPACK and CSRD
 // for mixed precision and for batches.
@@ -29,7 +30,10 @@
   P __global__ void fvc::gmres_persistent<F, 0>(fvc::GmresArgs); \
   P __global__ void fvc::gmres_kernel<F, 0>(fvc::GmresArgs);     \
   P __global__ void fvc::gmres_persistent<F, 1>(fvc::GmresArgs); \
-  P __global__ void fvc::gmres_kernel<F, 1>(fvc::GmresArgs);
+  P __global__ void fvc::gmres_kernel<F, 1>(fvc::GmresArgs);     \
+  P __global__ void fvc::gmres_lag_d_kernel<F>(fvc::GmresArgs); \
+  P __global__ void fvc::gmres_lag_r_kernel<F>(fvc::GmresArgs); \
+  P __global__ void fvc::gmres_lag_b_kernel<F>(fvc::GmresArgs);
 #define FVCI_MIXED(P, F)                                           \
   P __global__ void fvc::mixed_persistent<F>(fvc::MixedArgs);   \
   P __global__ void fvc::mixed_kernel<F>(fvc::MixedArgs);       \

@@ -1071,9 +1071,15 @@ struct GmresArgs {
   double tol;
   int k, maxit, phase;
   int stop_rule;   // 0 the reference's test on the preconditioned residual, 1 the true residual
-  int orth;        // 0 modified Gram-Schmidt (the reference's order), 1 CGS2
+  int orth;        // 0 modified Gram-Schmidt (the reference's order), 1 CGS2,
+                   // 2 CGS2 lagged by one step (fvc_gmres_lag.h)
   int chunk;       // CGS2, one kernel per phase: the chunk of dot products
   size_t ld;       // leading dimension of gv (= neq)
+  // lagged CGS2 only: the candidate u and w = M^-1 A u (neq each), the
+  // Hessenberg before the Givens rotations h(k+1, k), the coefficients of the
+  // update step (nu, a, c) and the per-block partials (lag_kp per block)
+  double *gu, *gw, *hraw, *lag, *lag_part;
+  int lag_kp;
 };
 
 // The end of Arnoldi step i once h(1..i+1, i) is known: the previous Givens
@@ -1249,7 +1255,7 @@ __device__ __forceinline__ bool gmres_phase(const GmresArgs &a, int ph, double *
   if (ph == GM_START) {
     // g1 = (b - A x)/m; e(1) = ||g1||; cycle 1 starts from x = 0 and sets econv
     const int first = ld_int(&st->cycle) == 0;
-    double *g1 = col(1);
+    double *g1 = a.orth == 2 ? a.gu : col(1);   // lagged CGS2: the first candidate
     double v[3] = {0.0, 0.0, 0.0};
     for (int i = i0; i < i1; i += is) {
       double ax;

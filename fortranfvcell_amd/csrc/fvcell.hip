@@ -386,7 +386,12 @@ int run_solve(DevState &d, const SolveIn &in, fvc_stats *stats, hipStream_t s) {
   // for ordering, a phase needs no fence and no atomic; measured faster on
   // MI355X than the grid barrier of one persistent launch); GMRES, whose
   // Arnoldi step is a chain of short phases, runs persistent.
-  bool persistent = in.engine == FVC_ENGINE_PERSISTENT || (in.engine == FVC_ENGINE_AUTO && in.code == FVC_GMRES);
+  // GMRES with lagged CGS2 runs one kernel per phase only (fvc_gmres_lag.h)
+  const bool lag = in.code == FVC_GMRES && in.orth == FVC_ORTH_CGS2_LAG;
+  if (lag && in.engine == FVC_ENGINE_PERSISTENT)
+    return fail(FVC_UNSUPPORTED, "GMRES with lagged CGS2 runs one kernel per phase (engine %d)", in.engine);
+  bool persistent = in.engine == FVC_ENGINE_PERSISTENT ||
+                    (in.engine == FVC_ENGINE_AUTO && in.code == FVC_GMRES && !lag);
   // Persistent GMRES: its Arnoldi step is j+1 short phases, each ended by a
   // grid barrier whose cost grows with the blocks taking part, so fewer and
   // larger blocks win: 1024 threads on three quarters of the CUs (1M cells,
@@ -396,8 +401,9 @@ int run_solve(DevState &d, const SolveIn &in, fvc_stats *stats, hipStream_t s) {
   const bool cgs = in.code == FVC_GMRES && in.orth == FVC_ORTH_CGS2;
   // CGS2 holds 16 partial sums per thread: 512 threads per block (see gmres_persistent)
   const int bs = in.bs > 0 ? in.bs : gmres_p ? (cgs ? 512 : 1024) : persistent ? kDefaultBlock
-                                                                   : kernels_block(in.code, fused);
+                                                                   : lag ? 256 : kernels_block(in.code, fused);
   if (bs % 64 != 0 || bs > 1024) return fail(FVC_BAD_ARGUMENT, "block size %d must be a multiple of 64 <= 1024", bs);
+  if (lag && bs > 256) return fail(FVC_BAD_ARGUMENT, "GMRES with lagged CGS2 takes blocks of at most 256 threads (%d)", bs);
   const void *pfn = nullptr, *kfn = nullptr;
   if (in.code == FVC_PCG && fused) {
     pfn = (const void *)&cg1_persistent<F>;
@@ -445,16 +451,22 @@ int run_solve(DevState &d, const SolveIn &in, fvc_stats *stats, hipStream_t s) {
     return fail(FVC_BAD_ARGUMENT, "GMRES restart k = %d outside [1, %d]", k, kMaxGmresK);
   if (cgs && k > kCgsWide)
     return fail(FVC_UNSUPPORTED, "GMRES with CGS2 orthogonalisation takes k <= %d (asked %d)", kCgsWide, k);
+  if (lag && k > kMaxGmresLag)
+    return fail(FVC_UNSUPPORTED, "GMRES with lagged CGS2 takes k <= %d (asked %d)", kMaxGmresLag, k);
+  const size_t lag_small = lag ? (size_t)(k + 1) * k + 2 * (size_t)k + 3 : 0;   // hraw, (nu, a, c)
+  const size_t lag_parts = lag ? (size_t)lag_kp(k) * G + 16 : 0;   // + the reduction's overreach
   // partial sums per block: CGS2 reduces kCgsChunk dot products at once
   const size_t npart = (size_t)(cgs ? kCgsChunk : kMaxK) * G;
 
   // workspace
   const size_t n = (size_t)neq;
-  int nvec = in.code == FVC_PCG ? (fused ? 7 : 5) : in.code == FVC_BICGSTAB ? (fused ? 8 : 10) : (k + 1);
+  // GMRES: basis columns 1..k+1; lagged CGS2: columns 1..k, the candidate u and w
+  int nvec = in.code == FVC_PCG ? (fused ? 7 : 5) : in.code == FVC_BICGSTAB ? (fused ? 8 : 10) : (k + 1 + (lag ? 1 : 0));
   size_t bytes = align_up((kCtrWords + (size_t)G) * sizeof(unsigned)) + align_up(sizeof(KState)) +
                  align_up(npart * sizeof(double)) + (size_t)nvec * align_up(n * sizeof(double)) +
                  align_up((size_t)(k + 2) * (k + 2) * sizeof(double)) * 2 + align_up(n * sizeof(double)) +
-                 align_up(2 * sizeof(KState)) + align_up(2 * kMaxK * (size_t)G * sizeof(double)) + 4096;
+                 align_up(2 * sizeof(KState)) + align_up(2 * kMaxK * (size_t)G * sizeof(double)) +
+                 align_up(lag_small * sizeof(double)) + align_up(lag_parts * sizeof(double)) + 4096;
   char *ws = nullptr;
   if ((rc = workspace(d, bytes, &ws))) return rc;
   Carve cv{ws, bytes};
@@ -467,6 +479,8 @@ int run_solve(DevState &d, const SolveIn &in, fvc_stats *stats, hipStream_t s) {
   for (int i = 0; i < nvec; ++i) vec[i] = cv.take<double>(n);
   double *gsmall = cv.take<double>((size_t)(k + 2) * (k + 2) * 2);
   double *xbackup = cv.take<double>(n);
+  double *lagsmall = cv.take<double>(lag_small);
+  double *lagpart = cv.take<double>(lag_parts);
   if (!cv.fits()) return carve_overflow(cv);
   const bool may_fall_back = persistent && in.engine == FVC_ENGINE_AUTO;
   if (may_fall_back && !in.x0_zero)
@@ -613,8 +627,55 @@ int run_solve(DevState &d, const SolveIn &in, fvc_stats *stats, hipStream_t s) {
     a.e = sm; sm += k + 2;
     a.y = sm; sm += k + 2;
     a.nrm = sm; sm += k + 2;
-    a.tol = in.tol; a.k = k; a.maxit = in.maxit; a.stop_rule = in.stop_rule; a.orth = cgs ? 1 : 0;
-    if (persistent) {
+    a.tol = in.tol; a.k = k; a.maxit = in.maxit; a.stop_rule = in.stop_rule; a.orth = lag ? 2 : cgs ? 1 : 0;
+    if (lag) {
+      a.gu = vec[k];
+      a.gw = vec[k + 1];
+      a.hraw = lagsmall;
+      a.lag = lagsmall + (size_t)(k + 1) * k;
+      a.lag_part = lagpart;
+      a.lag_kp = lag_kp(k);
+    }
+    if (lag) {
+      // lagged CGS2: per Arnoldi step D (SpMV and dot products), R (one
+      // block: the reduction and the scalar part) and B (basis column and
+      // next candidate).  A cycle's steps are enqueued in batches; the host
+      // stops enqueueing one batch after the device reports the cycle done.
+      for (int cyc = 0; cyc <= in.maxit; ++cyc) {
+        a.phase = GM_START;
+        if ((rc = launch(kfn, false, G, bs, a, s))) return rc;
+        int j = 1, batch = 4, q = 0;
+        bool waiting[2] = {false, false};
+        while (j <= k + 1) {
+          const int nb = std::min(batch, k + 2 - j);
+          for (int t = 0; t < nb; ++t, ++j) {
+            hipLaunchKernelGGL(gmres_lag_d_kernel<F>, dim3(G), dim3(bs), 0, s, a);
+            hipLaunchKernelGGL(gmres_lag_r_kernel<F>, dim3(1), dim3(1024), 0, s, a);
+            if (j <= k) hipLaunchKernelGGL(gmres_lag_b_kernel<F>, dim3(G), dim3(bs), 0, s, a);
+          }
+          HIPCHK(hipGetLastError());
+          const int slot = q & 1, prev = slot ^ 1;
+          HIPCHK(hipMemcpyAsync(d.host_ring + slot, st, sizeof(KState), hipMemcpyDeviceToHost, s));
+          HIPCHK(hipEventRecord(d.evq[slot], s));
+          waiting[slot] = true;
+          ++q;
+          if (waiting[prev]) {
+            HIPCHK(hipEventSynchronize(d.evq[prev]));
+            waiting[prev] = false;
+            if (d.host_ring[prev].inner_done || d.host_ring[prev].done) break;
+          }
+          batch = std::min(batch * 2, 8);
+        }
+        a.phase = GM_XUPD;
+        if ((rc = launch(kfn, false, G, bs, a, s))) return rc;
+        if ((rc = read_state(d, st, s))) return rc;
+        if (d.host_st->done) break;
+      }
+      a.phase = GM_FINAL;
+      if ((rc = launch(kfn, false, G, bs, a, s))) return rc;
+      HIPCHK(hipEventRecord(d.ev1, s));
+      if ((rc = read_state(d, st, s))) return rc;
+    } else if (persistent) {
       a.phase = -1;
       if ((rc = launch(pfn, true, G, bs, a, s))) return rc;
       HIPCHK(hipEventRecord(d.ev1, s));
@@ -1310,8 +1371,8 @@ int solve_impl(int code, int matriz, int unsym, int neq, int nad, const double *
   // per-column selection splits a row's gathers into branches that no longer
   // issue together.
   in.lanex = o.lane_exchange == 1;
-  if (o.gmres_orth != FVC_ORTH_MGS && o.gmres_orth != FVC_ORTH_CGS2)
-    return fail(FVC_BAD_ARGUMENT, "gmres_orth %d (0 MGS, 1 CGS2)", o.gmres_orth);
+  if (o.gmres_orth != FVC_ORTH_MGS && o.gmres_orth != FVC_ORTH_CGS2 && o.gmres_orth != FVC_ORTH_CGS2_LAG)
+    return fail(FVC_BAD_ARGUMENT, "gmres_orth %d (0 MGS, 1 CGS2, 2 lagged CGS2)", o.gmres_orth);
   in.orth = o.gmres_orth;
   if (o.row_order < 0 || o.row_order > 2)
     return fail(FVC_BAD_ARGUMENT, "row_order %d (0 default, 1 chunks, 2 XCD tiles)", o.row_order);

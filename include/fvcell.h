@@ -135,6 +135,10 @@ typedef struct fvc_options {
 #define FVC_ORTH_CGS2 1         /* classical Gram-Schmidt with one re-orthogonalisation:
                                    three grid reductions per step, the j dot products of
                                    a pass reduced together; gmres_k <= 32                 */
+#define FVC_ORTH_CGS2_LAG 2     /* CGS2 with its second pass and normalisation lagged by
+                                   one step: one reduction per Arnoldi step, the basis
+                                   read twice per step; one kernel per phase only,
+                                   gmres_k <= 64, block_size <= 256                       */
 
 typedef struct fvc_stats {
   int32_t status;           /* FVC_* status                                          */

@@ -189,3 +189,75 @@ def test_cgs2_limits(fvc):
     x, st = fvc.solve(P.to_ell(s), fvc.GMRES, tol=1e-10, gmres_k=32, stop_rule=fvc.STOP_RELRES,
                       gmres_orth=fvc.ORTH_CGS2)
     assert st["status"] == 0 and _relres(s, x) <= 1e-10 and st["iterations"] > 1, st
+
+
+# ---------------------------------------------------------- lagged CGS2 --
+# CGS2 with its second pass and normalisation lagged by one Arnoldi step
+# (FVC_ORTH_CGS2_LAG): one reduction per step, the basis read twice.  Held to
+# the parity bar against the reference's solutions, like CGS2.
+
+@pytest.mark.parametrize("key", [k for k in CONFIG_KEYS if k.endswith("gmres30")])
+@pytest.mark.parametrize("layout", ["ell", "csrd"])
+def test_lagged_cgs2_small(fvc, key, layout):
+    s, f = config_system(key)
+    if layout == "csrd":
+        f = P.to_csrd(s)
+    xref = REF_FIXTURES[key + "_x"]
+    x, st = fvc.solve(f, fvc.GMRES, tol=1e-10, gmres_k=30, stop_rule=fvc.STOP_RELRES,
+                      gmres_orth=fvc.ORTH_CGS2_LAG)
+    r = _relres(s, x)
+    _, sm = fvc.solve(f, fvc.GMRES, tol=1e-10, gmres_k=30, stop_rule=fvc.STOP_RELRES)
+    print(f"lagged cgs2 {key} {layout}: {st['iterations']} cycles {st['inner_iterations']} steps "
+          f"(mgs {sm['inner_iterations']}) relres {r:.2e}", flush=True)
+    assert st["status"] == 0 and r <= 1e-10
+    assert st["engine"] == fvc.ENGINE_KERNELS
+    assert abs(st["true_relres"] - r) <= 1e-3 * r
+    assert np.linalg.norm(x - xref) <= 1e-8 * np.linalg.norm(xref)
+    assert abs(st["inner_iterations"] - sm["inner_iterations"]) <= 2
+    # deterministic: a second solve is bit-identical
+    x2, st2 = fvc.solve(f, fvc.GMRES, tol=1e-10, gmres_k=30, stop_rule=fvc.STOP_RELRES,
+                        gmres_orth=fvc.ORTH_CGS2_LAG)
+    assert np.array_equal(x, x2) and st2["inner_iterations"] == st["inner_iterations"]
+    # the reference rule at the ladder tolerance: the reference's cycle count
+    i = CONFIG_KEYS.index(key)
+    tol = float(REF_FIXTURES["tol"][i])
+    x, st = fvc.solve(f, fvc.GMRES, tol=tol, gmres_k=30, gmres_orth=fvc.ORTH_CGS2_LAG)
+    it_ref = int(REF_FIXTURES["iterations"][i])
+    assert st["status"] == 0 and abs(st["iterations"] - it_ref) <= 1, (st["iterations"], it_ref)
+    assert np.linalg.norm(x - xref) <= 1e-8 * np.linalg.norm(xref)
+
+
+def test_lagged_cgs2_1024_true_residual(fvc, ladder, advdiff1024):
+    s, ds = advdiff1024
+    name = "advdiff1024_ell_gmres30"
+    st_c = ds.solve(fvc.GMRES, tol=1e-10, gmres_k=30, stop_rule=fvc.STOP_RELRES, gmres_orth=fvc.ORTH_CGS2)
+    st = ds.solve(fvc.GMRES, tol=1e-10, gmres_k=30, stop_rule=fvc.STOP_RELRES, gmres_orth=fvc.ORTH_CGS2_LAG)
+    x = ds.x.get()
+    r = _relres(s, x)
+    print(f"lagged cgs2 1024: {st['iterations']} cycles {st['inner_iterations']} steps "
+          f"{st['seconds'] * 1e3:.1f} ms relres {r:.2e}; cgs2 {st_c['inner_iterations']} steps "
+          f"{st_c['seconds'] * 1e3:.1f} ms", flush=True)
+    assert st["status"] == 0 and r <= 1e-10
+    assert abs(st["inner_iterations"] - st_c["inner_iterations"]) <= 2
+    _check_stats(name, ladder, s, x, 1e-8)
+
+
+def test_lagged_cgs2_limits(fvc):
+    s = P.config_advdiff(32)
+    f = P.to_ell(s)
+    with pytest.raises(fvc.FvcError):
+        fvc.solve(f, fvc.GMRES, gmres_k=65, gmres_orth=fvc.ORTH_CGS2_LAG, raise_on_failure=True)
+    with pytest.raises(fvc.FvcError):
+        fvc.solve(f, fvc.GMRES, gmres_k=10, gmres_orth=fvc.ORTH_CGS2_LAG, engine=fvc.ENGINE_PERSISTENT,
+                  raise_on_failure=True)
+    with pytest.raises(fvc.FvcError):
+        fvc.solve(f, fvc.GMRES, gmres_k=10, gmres_orth=fvc.ORTH_CGS2_LAG, block_size=512,
+                  raise_on_failure=True)
+    # the steady system: many full cycles at k = 64 and at k = 1 (every step
+    # of a cycle is the re-orthogonalisation-only step k + 1 or its SpMV)
+    s = P.config_advdiff(32, dt=None, under_relax=1.0)
+    for k in (64, 3):
+        x, st = fvc.solve(P.to_ell(s), fvc.GMRES, tol=1e-10, gmres_k=k, stop_rule=fvc.STOP_RELRES,
+                          gmres_orth=fvc.ORTH_CGS2_LAG, maxit=20000)
+        print(f"steady 32x32 k={k}: {st['iterations']} cycles {st['inner_iterations']} steps", flush=True)
+        assert st["status"] == 0 and _relres(s, x) <= 1e-10 and st["iterations"] > 1, st
