@@ -218,6 +218,9 @@ __device__ __forceinline__ void block_barrier() {
 
 // Sums K values over the block; the totals land in thread 0's v[].
 // LdsOnly: the barriers order LDS only (see lds_barrier).
+// Each value is summed over the waves in wave order by its own thread (k),
+// so the dependent LDS reads form K chains of nw in parallel instead of one
+// chain of K nw on thread 0; thread 0 then reads the K totals at once.
 template <int K, bool LdsOnly = false>
 __device__ __forceinline__ void block_sum(double *v, double *lds) {
   const int lane = threadIdx.x & (kWave - 1), wid = threadIdx.x / kWave;
@@ -229,13 +232,18 @@ __device__ __forceinline__ void block_sum(double *v, double *lds) {
     for (int k = 0; k < K; ++k) lds[k * kMaxWaves + wid] = v[k];
   }
   block_barrier<LdsOnly>();
+  double s = 0.0;
+  const int q = (int)threadIdx.x;
+  if (q < K) {
+    s = lds[q * kMaxWaves];
+    for (int w = 1; w < nw; ++w) s += lds[q * kMaxWaves + w];
+  }
+  block_barrier<LdsOnly>();
+  if (q < K) lds[q * kMaxWaves] = s;
+  block_barrier<LdsOnly>();
   if (threadIdx.x == 0) {
 #pragma unroll
-    for (int k = 0; k < K; ++k) {
-      double s = lds[k * kMaxWaves];
-      for (int w = 1; w < nw; ++w) s += lds[k * kMaxWaves + w];
-      v[k] = s;
-    }
+    for (int k = 0; k < K; ++k) v[k] = lds[k * kMaxWaves];
   }
   block_barrier<LdsOnly>();
 }
