@@ -29,98 +29,143 @@
 
 namespace fvc {
 
-constexpr int kLagChunk = 8;      // basis columns per pass of D's dot products
+constexpr int kLagChunk = 8;      // basis columns loaded together by B
 constexpr int kMaxGmresLag = 64;  // largest restart length of this variant
+constexpr int kLagBlock = 256;    // threads per block of D and B
+constexpr int kLagTile = 1024;    // rows of u and w a D block stages in LDS at a time
+constexpr int kLagPerWave = (kMaxGmresLag + kLagBlock / kWave - 1) / (kLagBlock / kWave);
+// arrival counter of the R blocks: a word of the grid counters that the
+// kernel-per-phase reductions leave unused (zeroed with them per solve)
+constexpr int kCtrLag = kCtrErr + 16;
 
 // Value q of step j's reduction: a_1..a_{j-1} at 0..j-2, beta at j-1,
-// b_1..b_{j-1} at j..2j-2, gamma at 2j-1.  Partials: part[block * kp + q].
+// b_1..b_{j-1} at j..2j-2, gamma at 2j-1.  Partials value-major:
+// part[q * G + block]; totals tot[q].
 __host__ __device__ inline int lag_kp(int k) { return 2 * (k + 1); }
 
+// D_j.  A block owns a contiguous range of rows (Grid chunks) and takes it
+// kLagTile rows at a time: its threads form w = A u / m for the tile (u of
+// the neighbours gathered), store w, and stage u and w in LDS; then wave v
+// streams basis columns v, v + 4, ... over the tile, multiplying with u and
+// w from LDS, so that each basis element is read once and u, w once.  Each
+// column's sum over a tile is reduced over the wave and added to the wave's
+// running total in LDS, in tile order: a fixed order.
 template <int F>
-__global__ __launch_bounds__(256) void gmres_lag_d_kernel(GmresArgs a) {
-  __shared__ double lds[2 * kLagChunk * kMaxWaves];
+__global__ __launch_bounds__(kLagBlock) void gmres_lag_d_kernel(GmresArgs a) {
+  constexpr int nwv = kLagBlock / kWave;
+  __shared__ double us[kLagTile], ws[kLagTile];
+  __shared__ double lds[2 * kMaxWaves];
+  __shared__ double cw[nwv][2 * kLagPerWave];   // each wave's running column totals
   const KState *st = a.st;
   if (ld_int(&st->done) || ld_int(&st->inner_done)) return;
   const int j = ld_int(&st->step_i);
-  const int nv = j - 1;
+  const int nv = j - 1, G = a.g.G;
   const bool last = j == a.k + 1;   // column k's re-orthogonalisation only
-  const int i0 = a.g.first(), i1 = a.g.row_end(), is = a.g.stride();
-  double *pout = a.lag_part + (size_t)a.g.chunk() * a.lag_kp;
-  {
-    double v[2] = {0.0, 0.0};
-    for (int r = i0; r < i1; r += is) {
+  const int blk = a.g.chunk();
+  const int r0 = blk * a.g.R;
+  const int r1 = min(r0 + a.g.R, a.g.neq);
+  const int wid = (int)threadIdx.x / kWave, lane = (int)threadIdx.x % kWave;
+  if (lane == 0)
+    for (int p = 0; p < 2 * kLagPerWave; ++p) cw[wid][p] = 0.0;
+  double uu = 0.0, uw = 0.0;
+  for (int t0 = r0; t0 < r1; t0 += kLagTile) {
+    const int tn = min(kLagTile, r1 - t0);
+    for (int q = threadIdx.x; q < tn; q += kLagBlock) {
+      const int r = t0 + q;
       const double ur = a.gu[r];
-      v[0] += ur * ur;
+      double wr = 0.0;
       if (!last) {
-        const double wr = row_apply<F>(a.A, r, XVec{a.gu}) / a.m[r];
+        wr = row_apply<F>(a.A, r, XVec{a.gu}) / a.m[r];
         a.gw[r] = wr;
-        v[1] += ur * wr;
+      }
+      us[q] = ur;
+      ws[q] = wr;
+      uu += ur * ur;
+      uw += ur * wr;
+    }
+    __syncthreads();
+    // wave wid: columns wid, wid + nwv, ... over the tile (lane-strided
+    // rows), reduced over the wave and added to the wave's running totals
+    for (int c = wid; c < nv; c += nwv) {
+      const double *vc = a.gv + (size_t)c * a.ld + t0;
+      double sa = 0.0, sb = 0.0;
+      for (int q = lane; q < tn; q += kWave) {
+        const double g = vc[q];
+        sa += g * us[q];
+        sb += g * ws[q];
+      }
+      sa = wave_sum(sa);
+      sb = wave_sum(sb);
+      if (lane == 0) {
+        const int p = c / nwv;
+        cw[wid][2 * p] += sa;
+        cw[wid][2 * p + 1] += sb;
       }
     }
-    block_sum<2>(v, lds);
-    if (threadIdx.x == 0) {
-      pout[nv] = v[0];
-      pout[2 * j - 1] = v[1];
+    __syncthreads();
+  }
+  if (lane == 0) {
+    for (int c = wid; c < nv; c += nwv) {
+      const int p = c / nwv;
+      a.lag_part[(size_t)c * G + blk] = cw[wid][2 * p];
+      a.lag_part[(size_t)(j + c) * G + blk] = cw[wid][2 * p + 1];
     }
   }
-  for (int c0 = 0; c0 < nv; c0 += kLagChunk) {
-    double v[2 * kLagChunk];
-#pragma unroll
-    for (int q = 0; q < 2 * kLagChunk; ++q) v[q] = 0.0;
-    for (int r = i0; r < i1; r += is) {
-      const double ur = a.gu[r];
-      const double wr = last ? 0.0 : a.gw[r];
-      // the chunk's columns loaded together; a column past nv reads column
-      // nv again (an L1 hit), so no load waits behind a branch
-      double g[kLagChunk];
-#pragma unroll
-      for (int q = 0; q < kLagChunk; ++q) g[q] = a.gv[(size_t)min(c0 + q, nv - 1) * a.ld + r];
-#pragma unroll
-      for (int q = 0; q < kLagChunk; ++q) {
-        if (c0 + q < nv) {
-          v[q] += g[q] * ur;
-          v[kLagChunk + q] += g[q] * wr;
-        }
-      }
-    }
-    block_sum<2 * kLagChunk>(v, lds);
-    if (threadIdx.x == 0) {
-#pragma unroll
-      for (int q = 0; q < kLagChunk; ++q) {
-        if (c0 + q < nv) {
-          pout[c0 + q] = v[q];
-          pout[j + c0 + q] = v[kLagChunk + q];
-        }
-      }
-    }
+  double v[2] = {uu, uw};
+  block_sum<2>(v, lds);
+  if (threadIdx.x == 0) {
+    a.lag_part[(size_t)nv * G + blk] = v[0];
+    a.lag_part[(size_t)(2 * j - 1) * G + blk] = v[1];
   }
 }
 
-// One block: the reduction of D_j's partials (each value over the G blocks in
-// block order, then the block's fixed tree) and the scalar part of step j.
-// Everything the scalar part reads is first loaded into LDS by all threads
-// together, so the one thread that runs the sequential part (the Givens
-// rotations, the back substitution) waits on no global memory.  (Templated
-// on the format like D and B only so that every family of solver kernels is
-// instantiated in one translation unit.)
+// R_j, launched with 2(k+1) blocks: block q < 2j sums value q over the G
+// blocks of D (thread t takes blocks t, t + 256, ... in order, then the
+// block's fixed tree) into tot[q]; the last block to arrive then runs the
+// scalar part of step j.  Everything the scalar part reads is first loaded
+// into LDS by all the block's threads together, so the one thread that runs
+// the sequential part (the Givens rotations, the back substitution) waits on
+// no global memory.  (Templated on the format like D and B only so that
+// every family of solver kernels is instantiated in one translation unit.)
 template <int F>
-__global__ __launch_bounds__(1024) void gmres_lag_r_kernel(GmresArgs a) {
+__global__ __launch_bounds__(kLagBlock) void gmres_lag_r_kernel(GmresArgs a) {
   constexpr int L = kMaxGmresLag + 1;
-  __shared__ double lds[16 * kMaxWaves];
+  __shared__ double lds[2 * kMaxWaves];
   __shared__ double tot[2 * L];
   __shared__ double hr[L * L];          // H before the rotations, rows and columns 1..j
   __shared__ double hc[L + 1];          // H(1:j, 1:j-1) a
   __shared__ double cs[L + 1], sn[L + 1], ee[L + 2];
   __shared__ double col[L + 1];         // column j-1 under rotation; y at the end
-  __shared__ int s_ni;
+  __shared__ int s_last, s_ni;
   KState *st = a.st;
   if (ld_int(&st->done) || ld_int(&st->inner_done)) return;
   const int j = ld_int(&st->step_i);
   const int nv = j - 1, K = 2 * j, k = a.k;
   const int G = a.g.G;
+  const int qv = (int)blockIdx.x;
+  if (qv >= K) return;
+  {
+    const double *p = a.lag_part + (size_t)qv * G;
+    double v[1] = {0.0};
+    for (int b = threadIdx.x; b < G; b += blockDim.x) v[0] += p[b];
+    block_sum<1>(v, lds);
+    if (threadIdx.x == 0) {
+      a.lag_tot[qv] = v[0];
+      fence_release();
+      const unsigned t = add_relaxed(a.g.ctr + kCtrLag, 1u);
+      s_last = (int)t + 1 == K;
+      if (s_last) {
+        st_relaxed(a.g.ctr + kCtrLag, 0u);
+        fence_acquire();
+      }
+    }
+    __syncthreads();
+    if (!s_last) return;
+  }
   auto HR = [&](int r, int c) -> double & { return hr[(c - 1) * L + (r - 1)]; };
   auto Hg = [&](double *h, int r, int c) -> double & { return h[(size_t)(c - 1) * (k + 1) + (r - 1)]; };
   // the small state, loaded by all threads at once
+  for (int t = threadIdx.x; t < K; t += blockDim.x) tot[t] = a.lag_tot[t];
   for (int t = threadIdx.x; t < j * nv; t += blockDim.x) {
     const int r = t % j + 1, c = t / j + 1;
     HR(r, c) = r <= c + 1 ? Hg(a.hraw, r, c) : 0.0;
@@ -129,28 +174,6 @@ __global__ __launch_bounds__(1024) void gmres_lag_r_kernel(GmresArgs a) {
     cs[t] = a.cs[t];
     sn[t] = a.sn[t];
     ee[t] = a.e[t];
-  }
-  // D_j's partials, 16 values at a time; the buffer holds 16 values past the
-  // last block's, so every load is unconditional
-  for (int q0 = 0; q0 < K; q0 += 16) {
-    double v[16];
-#pragma unroll
-    for (int q = 0; q < 16; ++q) v[q] = 0.0;
-    for (int b = threadIdx.x; b < G; b += blockDim.x) {
-      const double *p = a.lag_part + (size_t)b * a.lag_kp + q0;
-      double w[16];
-#pragma unroll
-      for (int q = 0; q < 16; ++q) w[q] = p[q];
-#pragma unroll
-      for (int q = 0; q < 16; ++q)
-        if (q0 + q < K) v[q] += w[q];
-    }
-    block_sum<16>(v, lds);
-    if (threadIdx.x == 0) {
-#pragma unroll
-      for (int q = 0; q < 16; ++q)
-        if (q0 + q < K) tot[q0 + q] = v[q];
-    }
   }
   __syncthreads();
   // H(1:j, 1:j-1) a with column j-1 as it stands (its correction follows)
@@ -264,7 +287,7 @@ __global__ __launch_bounds__(1024) void gmres_lag_r_kernel(GmresArgs a) {
 
 // v_j = (u_j - V_{j-1} a) / nu into column j; u <- w / nu - V_j c.
 template <int F>
-__global__ __launch_bounds__(256) void gmres_lag_b_kernel(GmresArgs a) {
+__global__ __launch_bounds__(kLagBlock) void gmres_lag_b_kernel(GmresArgs a) {
   __shared__ double ca[kMaxGmresLag + 1], cc[kMaxGmresLag + 2];
   const KState *st = a.st;
   if (ld_int(&st->done) || ld_int(&st->inner_done)) return;

@@ -1077,8 +1077,9 @@ struct GmresArgs {
   size_t ld;       // leading dimension of gv (= neq)
   // lagged CGS2 only: the candidate u and w = M^-1 A u (neq each), the
   // Hessenberg before the Givens rotations h(k+1, k), the coefficients of the
-  // update step (nu, a, c) and the per-block partials (lag_kp per block)
-  double *gu, *gw, *hraw, *lag, *lag_part;
+  // update step (nu, a, c), the per-block partials (lag_kp values per block)
+  // and their totals
+  double *gu, *gw, *hraw, *lag, *lag_part, *lag_tot;
   int lag_kp;
 };
 

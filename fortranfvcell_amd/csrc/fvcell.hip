@@ -403,7 +403,8 @@ int run_solve(DevState &d, const SolveIn &in, fvc_stats *stats, hipStream_t s) {
   const int bs = in.bs > 0 ? in.bs : gmres_p ? (cgs ? 512 : 1024) : persistent ? kDefaultBlock
                                                                    : lag ? 256 : kernels_block(in.code, fused);
   if (bs % 64 != 0 || bs > 1024) return fail(FVC_BAD_ARGUMENT, "block size %d must be a multiple of 64 <= 1024", bs);
-  if (lag && bs > 256) return fail(FVC_BAD_ARGUMENT, "GMRES with lagged CGS2 takes blocks of at most 256 threads (%d)", bs);
+  if (lag && bs != kLagBlock)
+    return fail(FVC_BAD_ARGUMENT, "GMRES with lagged CGS2 runs blocks of %d threads (asked %d)", kLagBlock, bs);
   const void *pfn = nullptr, *kfn = nullptr;
   if (in.code == FVC_PCG && fused) {
     pfn = (const void *)&cg1_persistent<F>;
@@ -453,8 +454,9 @@ int run_solve(DevState &d, const SolveIn &in, fvc_stats *stats, hipStream_t s) {
     return fail(FVC_UNSUPPORTED, "GMRES with CGS2 orthogonalisation takes k <= %d (asked %d)", kCgsWide, k);
   if (lag && k > kMaxGmresLag)
     return fail(FVC_UNSUPPORTED, "GMRES with lagged CGS2 takes k <= %d (asked %d)", kMaxGmresLag, k);
-  const size_t lag_small = lag ? (size_t)(k + 1) * k + 2 * (size_t)k + 3 : 0;   // hraw, (nu, a, c)
-  const size_t lag_parts = lag ? (size_t)lag_kp(k) * G + 16 : 0;   // + the reduction's overreach
+  // hraw, (nu, a, c), totals
+  const size_t lag_small = lag ? (size_t)(k + 1) * k + 2 * (size_t)k + 3 + (size_t)lag_kp(k) : 0;
+  const size_t lag_parts = lag ? (size_t)lag_kp(k) * G : 0;
   // partial sums per block: CGS2 reduces kCgsChunk dot products at once
   const size_t npart = (size_t)(cgs ? kCgsChunk : kMaxK) * G;
 
@@ -486,7 +488,9 @@ int run_solve(DevState &d, const SolveIn &in, fvc_stats *stats, hipStream_t s) {
   if (may_fall_back && !in.x0_zero)
     HIPCHK(hipMemcpyAsync(xbackup, in.x, n * sizeof(double), hipMemcpyDeviceToDevice, s));
 
-  Grid g = make_grid(neq, G, ctr, partials, persistent, in.barrier, use_tiles(in.row_order, in.code, neq, G, bs));
+  // lagged CGS2's D kernel takes contiguous row chunks
+  Grid g = make_grid(neq, G, ctr, partials, persistent, in.barrier,
+                     lag ? 0 : use_tiles(in.row_order, in.code, neq, G, bs));
   HIPCHK(hipEventRecord(d.ev0, s));
   hipLaunchKernelGGL(reset_kernel, dim3(1), dim3(256), 0, s, ctr, (int)kCtrWords + G, st, 1);
   HIPCHK(hipGetLastError());
@@ -633,6 +637,7 @@ int run_solve(DevState &d, const SolveIn &in, fvc_stats *stats, hipStream_t s) {
       a.gw = vec[k + 1];
       a.hraw = lagsmall;
       a.lag = lagsmall + (size_t)(k + 1) * k;
+      a.lag_tot = a.lag + 2 * (size_t)k + 3;
       a.lag_part = lagpart;
       a.lag_kp = lag_kp(k);
     }
@@ -650,7 +655,7 @@ int run_solve(DevState &d, const SolveIn &in, fvc_stats *stats, hipStream_t s) {
           const int nb = std::min(batch, k + 2 - j);
           for (int t = 0; t < nb; ++t, ++j) {
             hipLaunchKernelGGL(gmres_lag_d_kernel<F>, dim3(G), dim3(bs), 0, s, a);
-            hipLaunchKernelGGL(gmres_lag_r_kernel<F>, dim3(1), dim3(1024), 0, s, a);
+            hipLaunchKernelGGL(gmres_lag_r_kernel<F>, dim3(2 * j), dim3(kLagBlock), 0, s, a);
             if (j <= k) hipLaunchKernelGGL(gmres_lag_b_kernel<F>, dim3(G), dim3(bs), 0, s, a);
           }
           HIPCHK(hipGetLastError());
