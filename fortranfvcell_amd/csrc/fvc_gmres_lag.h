@@ -89,7 +89,19 @@ __global__ __launch_bounds__(kLagBlock) void gmres_lag_d_kernel(GmresArgs a) {
     for (int c = wid; c < nv; c += nwv) {
       const double *vc = a.gv + (size_t)c * a.ld + t0;
       double sa = 0.0, sb = 0.0;
-      for (int q = lane; q < tn; q += kWave) {
+      int q = lane;
+      // four rows per lane loaded together, then the rest one at a time
+      for (; q + 3 * kWave < tn; q += 4 * kWave) {
+        double g[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) g[u] = vc[q + u * kWave];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+          sa += g[u] * us[q + u * kWave];
+          sb += g[u] * ws[q + u * kWave];
+        }
+      }
+      for (; q < tn; q += kWave) {
         const double g = vc[q];
         sa += g * us[q];
         sb += g * ws[q];
