@@ -82,12 +82,32 @@ def gmres_cycle_bytes(neq: int, nnz_off: int, steps: int, cgs2: bool = False) ->
     return sum(spmv_bytes(neq, nnz_off) + per(j) * 8 * neq for j in range(1, steps + 1))
 
 
-def gmres_solve_bytes(neq: int, nnz_off: int, cycles: int, steps: int, k: int, cgs2: bool) -> int:
-    """All Arnoldi steps (cycles of k steps, the last one shorter) plus one
-    residual SpMV per cycle start (and one to confirm the last)."""
+def gmres_lag_cycle_bytes(neq: int, nnz_off: int, steps: int, k: int) -> int:
+    """A cycle of `steps` Arnoldi steps with lagged CGS2: D_j for j = 1 ..
+    steps + 1 (the SpMV u -> w, the j - 1 basis columns; step k + 1 reads u and
+    the k columns only) and B_j for j = 1 .. steps (reads the j - 1 columns,
+    u and w; writes column j and u)."""
+    b = 0
+    for j in range(1, steps + 2):
+        nv = j - 1
+        b += (spmv_bytes(neq, nnz_off) if j <= k else 8 * neq) + 8 * nv * neq
+        if j <= steps:
+            b += 8 * neq * (nv + 2) + 16 * neq
+    return b
+
+
+def gmres_solve_bytes(neq: int, nnz_off: int, cycles: int, steps: int, k: int, cgs2: bool,
+                      lag: bool = False) -> int:
+    """All Arnoldi steps (cycles of k steps, the last one shorter), one
+    residual SpMV per cycle start (and one to confirm the last), and each
+    cycle's update of x (its columns and x read, x written)."""
     full, last = divmod(steps, k)
-    b = full * gmres_cycle_bytes(neq, nnz_off, k, cgs2) + gmres_cycle_bytes(neq, nnz_off, last, cgs2)
-    return b + (cycles + 1) * (spmv_bytes(neq, nnz_off) + 16 * neq)
+    if lag:
+        b = full * gmres_lag_cycle_bytes(neq, nnz_off, k, k) + (gmres_lag_cycle_bytes(neq, nnz_off, last, k)
+                                                                 if last else 0)
+    else:
+        b = full * gmres_cycle_bytes(neq, nnz_off, k, cgs2) + gmres_cycle_bytes(neq, nnz_off, last, cgs2)
+    return b + (cycles + 1) * (spmv_bytes(neq, nnz_off) + 16 * neq) + (steps + 2 * cycles) * 8 * neq
 
 
 def pcg_ref_iter_bytes(neq: int, nnz_off: int) -> int:
@@ -485,12 +505,15 @@ def main():
                                             "true_relres": st_l["true_relres"]}
         log("bicgstab reference rule", extra["bicgstab_reference_rule"])
         # GMRES(30) on the same system, to the same true residual: modified
-        # Gram-Schmidt in the reference's order and CGS2
+        # Gram-Schmidt in the reference's order, CGS2, and CGS2 lagged by one
+        # step (one reduction per Arnoldi step); bytes by each one's schedule
         gm = {}
-        for oname, orth in (("mgs", F.ORTH_MGS), ("cgs2", F.ORTH_CGS2)):
+        for oname, orth in (("mgs", F.ORTH_MGS), ("cgs2", F.ORTH_CGS2), ("cgs2_lag", F.ORTH_CGS2_LAG)):
             ds.solve(F.GMRES, tol=tol, gmres_k=30, stop_rule=F.STOP_RELRES, gmres_orth=orth)
-            st_g = ds.solve(F.GMRES, tol=tol, gmres_k=30, stop_rule=F.STOP_RELRES, gmres_orth=orth)
-            gb = gmres_solve_bytes(neq, nnz, st_g["iterations"], st_g["inner_iterations"], 30, orth == F.ORTH_CGS2)
+            st_g = min((ds.solve(F.GMRES, tol=tol, gmres_k=30, stop_rule=F.STOP_RELRES, gmres_orth=orth)
+                        for _ in range(3)), key=lambda q: q["seconds"])
+            gb = gmres_solve_bytes(neq, nnz, st_g["iterations"], st_g["inner_iterations"], 30, orth == F.ORTH_CGS2,
+                                   lag=orth == F.ORTH_CGS2_LAG)
             gm[oname] = {"cycles": st_g["iterations"], "arnoldi_steps": st_g["inner_iterations"],
                          "ms_per_solve": st_g["seconds"] * 1e3, "solves_per_s": 1.0 / st_g["seconds"],
                          "true_relres": st_g["true_relres"], "status": st_g["status"],
@@ -510,14 +533,14 @@ def main():
         st_sb = dss.solve(F.BICGSTAB, tol=tol, maxit=100000, **opts)
         sbb = bicgstab_solve_bytes(ss.neq, ells["nad"], st_sb["iterations"])
         st_sg = dss.solve(F.GMRES, tol=tol, gmres_k=30, maxit=20000, stop_rule=F.STOP_RELRES,
-                          gmres_orth=F.ORTH_CGS2)
+                          gmres_orth=F.ORTH_CGS2_LAG)
         extra["steady_reference_defaults"] = {
             "workload": f"the same advection-diffusion operator with underU=1 and no transient term, "
                         f"{ns}x{ns} cells",
             "bicgstab": {"iterations": st_sb["iterations"], "ms_per_solve": st_sb["seconds"] * 1e3,
                          "true_relres": st_sb["true_relres"], "status": st_sb["status"],
                          "GBs_algorithmic": sbb / st_sb["seconds"] / 1e9},
-            "gmres30_cgs2": {"cycles": st_sg["iterations"], "arnoldi_steps": st_sg["inner_iterations"],
+            "gmres30_cgs2_lag": {"cycles": st_sg["iterations"], "arnoldi_steps": st_sg["inner_iterations"],
                              "ms_per_solve": st_sg["seconds"] * 1e3, "true_relres": st_sg["true_relres"],
                              "status": st_sg["status"]},
             "note": "Jacobi-BiCGSTAB does not converge on this system at 512^2 and 1024^2 cells "
@@ -567,11 +590,16 @@ def main():
                 dm = F.DeviceSystem(P.to_ell(sm))
             Am = P.scipy_csr(sm)
             row = {"neq": sm.neq}
+            # GMRES: mixed precision (FP32 CGS2 basis) against FP64 CGS2, and
+            # FP64 with lagged CGS2 beside them
             for mname, code, fp64kw in (("bicgstab", F.BICGSTAB, {}),
                                         ("gmres30", F.GMRES, dict(gmres_orth=F.ORTH_CGS2))):
                 res = {}
-                for pname, prec, kw in (("fp64", F.PREC_FP64, dict(stop_rule=F.STOP_RELRES, **fp64kw)),
-                                        ("mixed", F.PREC_MIXED, {})):
+                runs = [("fp64", F.PREC_FP64, dict(stop_rule=F.STOP_RELRES, **fp64kw)), ("mixed", F.PREC_MIXED, {})]
+                if code == F.GMRES:
+                    runs.append(("fp64_cgs2_lag", F.PREC_FP64, dict(stop_rule=F.STOP_RELRES,
+                                                                    gmres_orth=F.ORTH_CGS2_LAG)))
+                for pname, prec, kw in runs:
                     skw = dict(tol=tol, x0_rule=F.X0_ZERO, gmres_k=30, precision=prec, **kw)
                     dm.solve(code, **skw)
                     stm = dm.solve(code, **skw)
@@ -582,6 +610,7 @@ def main():
                                   "status": stm["status"], "_x": xm_}
                 x64_ = res["fp64"].pop("_x")
                 xmx_ = res["mixed"].pop("_x")
+                res.get("fp64_cgs2_lag", {}).pop("_x", None)
                 res["mixed"]["rel_err_vs_fp64"] = float(np.linalg.norm(xmx_ - x64_) / np.linalg.norm(x64_))
                 res["speedup_mixed_over_fp64"] = res["fp64"]["ms_per_solve"] / res["mixed"]["ms_per_solve"]
                 row[mname] = res
