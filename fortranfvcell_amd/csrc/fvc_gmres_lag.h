@@ -24,13 +24,17 @@
 // Per step the basis is read twice (D and B) instead of 2j+ times (MGS), and
 // the only grid-wide dependences are the kernel boundaries.  Step k+1 only
 // re-orthogonalises u_{k+1} (column k), without its SpMV.
+//
+// Two storages share these kernels: FP64 (GmresArgs: left-Jacobi operator
+// M^-1 A, basis in FP64) and mixed precision (MGmresArgs: the Jacobi-scaled
+// operator S A S in FP32, basis, u and w in FP32, every dot product and the
+// scalar part in FP64), each an access policy below.
 #pragma once
-#include "fvc_krylov.h"
+#include "fvc_mixed_krylov.h"
 
 namespace fvc {
 
 constexpr int kLagChunk = 8;      // basis columns loaded together by B
-constexpr int kMaxGmresLag = 64;  // largest restart length of this variant
 constexpr int kLagBlock = 256;    // threads per block of D and B
 constexpr int kLagTile = 1024;    // rows of u and w a D block stages in LDS at a time
 constexpr int kLagPerWave = (kMaxGmresLag + kLagBlock / kWave - 1) / (kLagBlock / kWave);
@@ -43,6 +47,41 @@ constexpr int kCtrLag = kCtrErr + 16;
 // part[q * G + block]; totals tot[q].
 __host__ __device__ inline int lag_kp(int k) { return 2 * (k + 1); }
 
+// FP64 storage: w = M^-1 A u (left Jacobi, src/Iterativos.f:333-336).
+template <int F>
+struct LagF64 {
+  const GmresArgs &a;
+  __device__ const GmresArgs &q() const { return a; }
+  __device__ double u(int r) const { return a.gu[r]; }
+  __device__ void set_u(int r, double v) const { a.gu[r] = v; }
+  __device__ double w(int r) const { return a.gw[r]; }
+  __device__ double set_w(int r, double v) const { a.gw[r] = v; return v; }   // returns the stored value
+  __device__ double op(int r) const { return row_apply<F>(a.A, r, XVec{a.gu}) / a.m[r]; }
+  __device__ double col(int c, int r) const { return a.gv[(size_t)c * a.ld + r]; }
+  __device__ void set_col(int c, int r, double v) const { a.gv[(size_t)c * a.ld + r] = v; }
+};
+// Mixed storage: w = S A S u in FP32 (the scaled FP32 matrix), FP32 vectors.
+template <int F>
+struct LagF32 {
+  const MGmresArgs &m;
+  struct XU {
+    const float *g;
+    __device__ __forceinline__ float operator()(int j) const { return g[j]; }
+  };
+  __device__ const GmresArgs &q() const { return m.q; }
+  __device__ double u(int r) const { return (double)m.gu[r]; }
+  __device__ void set_u(int r, double v) const { m.gu[r] = (float)v; }
+  __device__ double w(int r) const { return (double)m.gw[r]; }
+  __device__ double set_w(int r, double v) const {   // returns the stored value
+    const float f = (float)v;
+    m.gw[r] = f;
+    return (double)f;
+  }
+  __device__ double op(int r) const { return (double)row_apply32<F>(m.A32, r, XU{m.gu}); }
+  __device__ double col(int c, int r) const { return (double)m.gv[(size_t)c * m.q.ld + r]; }
+  __device__ void set_col(int c, int r, double v) const { m.gv[(size_t)c * m.q.ld + r] = (float)v; }
+};
+
 // D_j.  A block owns a contiguous range of rows (Grid chunks) and takes it
 // kLagTile rows at a time: its threads form w = A u / m for the tile (u of
 // the neighbours gathered), store w, and stage u and w in LDS; then wave v
@@ -50,12 +89,13 @@ __host__ __device__ inline int lag_kp(int k) { return 2 * (k + 1); }
 // w from LDS, so that each basis element is read once and u, w once.  Each
 // column's sum over a tile is reduced over the wave and added to the wave's
 // running total in LDS, in tile order: a fixed order.
-template <int F>
-__global__ __launch_bounds__(kLagBlock) void gmres_lag_d_kernel(GmresArgs a) {
+template <class P>
+__device__ __forceinline__ void gmres_lag_d(const P &pol) {
   constexpr int nwv = kLagBlock / kWave;
   __shared__ double us[kLagTile], ws[kLagTile];
   __shared__ double lds[2 * kMaxWaves];
   __shared__ double cw[nwv][2 * kLagPerWave];   // each wave's running column totals
+  const GmresArgs &a = pol.q();
   const KState *st = a.st;
   if (ld_int(&st->done) || ld_int(&st->inner_done)) return;
   const int j = ld_int(&st->step_i);
@@ -72,12 +112,9 @@ __global__ __launch_bounds__(kLagBlock) void gmres_lag_d_kernel(GmresArgs a) {
     const int tn = min(kLagTile, r1 - t0);
     for (int q = threadIdx.x; q < tn; q += kLagBlock) {
       const int r = t0 + q;
-      const double ur = a.gu[r];
+      const double ur = pol.u(r);
       double wr = 0.0;
-      if (!last) {
-        wr = row_apply<F>(a.A, r, XVec{a.gu}) / a.m[r];
-        a.gw[r] = wr;
-      }
+      if (!last) wr = pol.set_w(r, pol.op(r));
       us[q] = ur;
       ws[q] = wr;
       uu += ur * ur;
@@ -87,14 +124,13 @@ __global__ __launch_bounds__(kLagBlock) void gmres_lag_d_kernel(GmresArgs a) {
     // wave wid: columns wid, wid + nwv, ... over the tile (lane-strided
     // rows), reduced over the wave and added to the wave's running totals
     for (int c = wid; c < nv; c += nwv) {
-      const double *vc = a.gv + (size_t)c * a.ld + t0;
       double sa = 0.0, sb = 0.0;
       int q = lane;
       // four rows per lane loaded together, then the rest one at a time
       for (; q + 3 * kWave < tn; q += 4 * kWave) {
         double g[4];
 #pragma unroll
-        for (int u = 0; u < 4; ++u) g[u] = vc[q + u * kWave];
+        for (int u = 0; u < 4; ++u) g[u] = pol.col(c, t0 + q + u * kWave);
 #pragma unroll
         for (int u = 0; u < 4; ++u) {
           sa += g[u] * us[q + u * kWave];
@@ -102,7 +138,7 @@ __global__ __launch_bounds__(kLagBlock) void gmres_lag_d_kernel(GmresArgs a) {
         }
       }
       for (; q < tn; q += kWave) {
-        const double g = vc[q];
+        const double g = pol.col(c, t0 + q);
         sa += g * us[q];
         sb += g * ws[q];
       }
@@ -129,6 +165,15 @@ __global__ __launch_bounds__(kLagBlock) void gmres_lag_d_kernel(GmresArgs a) {
     a.lag_part[(size_t)nv * G + blk] = v[0];
     a.lag_part[(size_t)(2 * j - 1) * G + blk] = v[1];
   }
+}
+
+template <int F>
+__global__ __launch_bounds__(kLagBlock) void gmres_lag_d_kernel(GmresArgs a) {
+  gmres_lag_d(LagF64<F>{a});
+}
+template <int F>
+__global__ __launch_bounds__(kLagBlock) void mgmres_lag_d_kernel(MGmresArgs m) {
+  gmres_lag_d(LagF32<F>{m});
 }
 
 // R_j, launched with 2(k+1) blocks: block q < 2j sums value q over the G
@@ -298,9 +343,10 @@ __global__ __launch_bounds__(kLagBlock) void gmres_lag_r_kernel(GmresArgs a) {
 }
 
 // v_j = (u_j - V_{j-1} a) / nu into column j; u <- w / nu - V_j c.
-template <int F>
-__global__ __launch_bounds__(kLagBlock) void gmres_lag_b_kernel(GmresArgs a) {
+template <class P>
+__device__ __forceinline__ void gmres_lag_b(const P &pol) {
   __shared__ double ca[kMaxGmresLag + 1], cc[kMaxGmresLag + 2];
+  const GmresArgs &a = pol.q();
   const KState *st = a.st;
   if (ld_int(&st->done) || ld_int(&st->inner_done)) return;
   const int j = ld_int(&st->step_i) - 1;
@@ -312,13 +358,12 @@ __global__ __launch_bounds__(kLagBlock) void gmres_lag_b_kernel(GmresArgs a) {
   __syncthreads();
   const double nu = ca[0];
   const int i0 = a.g.first(), i1 = a.g.row_end(), is = a.g.stride();
-  double *vj = a.gv + (size_t)nv * a.ld;
   for (int r = i0; r < i1; r += is) {
-    double s1 = a.gu[r], s2 = 0.0;
+    double s1 = pol.u(r), s2 = 0.0;
     for (int c0 = 0; c0 < nv; c0 += kLagChunk) {
       double g[kLagChunk];
 #pragma unroll
-      for (int q = 0; q < kLagChunk; ++q) g[q] = a.gv[(size_t)min(c0 + q, nv - 1) * a.ld + r];
+      for (int q = 0; q < kLagChunk; ++q) g[q] = pol.col(min(c0 + q, nv - 1), r);
 #pragma unroll
       for (int q = 0; q < kLagChunk; ++q) {
         if (c0 + q < nv) {
@@ -328,9 +373,18 @@ __global__ __launch_bounds__(kLagBlock) void gmres_lag_b_kernel(GmresArgs a) {
       }
     }
     const double v = s1 / nu;
-    vj[r] = v;
-    a.gu[r] = a.gw[r] / nu - (s2 + cc[j] * v);
+    pol.set_col(nv, r, v);
+    pol.set_u(r, pol.w(r) / nu - (s2 + cc[j] * v));
   }
+}
+
+template <int F>
+__global__ __launch_bounds__(kLagBlock) void gmres_lag_b_kernel(GmresArgs a) {
+  gmres_lag_b(LagF64<F>{a});
+}
+template <int F>
+__global__ __launch_bounds__(kLagBlock) void mgmres_lag_b_kernel(MGmresArgs m) {
+  gmres_lag_b(LagF32<F>{m});
 }
 
 }  // namespace fvc

@@ -35,6 +35,9 @@
   P __global__ void fvc::gmres_lag_r_kernel<F>(fvc::GmresArgs); \
   P __global__ void fvc::gmres_lag_b_kernel<F>(fvc::GmresArgs);
 #define FVCI_MIXED(P, F)                                           \
+  P __global__ void fvc::mgmres_kernel<F>(fvc::MGmresArgs);      \
+  P __global__ void fvc::mgmres_lag_d_kernel<F>(fvc::MGmresArgs); \
+  P __global__ void fvc::mgmres_lag_b_kernel<F>(fvc::MGmresArgs); \
   P __global__ void fvc::mixed_persistent<F>(fvc::MixedArgs);   \
   P __global__ void fvc::mixed_kernel<F>(fvc::MixedArgs);       \
   P __global__ void fvc::mbicg_persistent<F>(fvc::MBicgArgs);   \

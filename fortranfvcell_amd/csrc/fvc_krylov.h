@@ -1050,6 +1050,7 @@ __global__ __launch_bounds__(1024) void bicg2_kernel(Bicg2Args a) {
 
 constexpr int kMaxGmresK = 256;
 constexpr int kCgsWide = 32;   // CGS2: largest restart length
+constexpr int kMaxGmresLag = 64;  // lagged CGS2 (fvc_gmres_lag.h): largest restart length
 constexpr int kCgsChunk = 16;  // CGS2: dot products reduced together
 
 enum { GM_START = 0, GM_MV = 1, GM_MGS = 2, GM_XUPD = 3, GM_FINAL = 4, GM_C1 = 5, GM_C2 = 6, GM_C3 = 7 };

@@ -325,6 +325,7 @@ struct MGmresArgs {
   Mat32 A32;
   const double *s;    // 1/sqrt(a_ii)
   float *gv;          // basis g(neq, k+1) in FP32, column-major
+  float *gu, *gw;     // lagged CGS2 (q.orth 2): the candidate u and w = B u in FP32
 };
 
 template <int F>
@@ -337,7 +338,7 @@ __device__ __forceinline__ bool mgmres_phase(const MGmresArgs &m, int ph, int c,
   auto H = [&](int r, int cc) -> double & { return a.h[(size_t)(cc - 1) * (k + 1) + (r - 1)]; };
   if (ph == MG_START) {
     const int first = ld_int(&st->cycle) == 0;
-    float *g1 = col(1);
+    float *g1 = a.orth == 2 ? m.gu : col(1);   // lagged CGS2: the first candidate
     double v[3] = {0.0, 0.0, 0.0};
     for (int i = i0; i < i1; i += is) {
       double ax;
@@ -389,7 +390,7 @@ __device__ __forceinline__ bool mgmres_phase(const MGmresArgs &m, int ph, int c,
   if (ph == MG_XUPD) {
     // x += S sum_j y(j) g(j), j in order, in FP64
     const int nI = ld_int(&st->ni);
-    __shared__ double ysh[kCgsWide + 2];
+    __shared__ double ysh[kMaxGmresLag + 2];   // k <= kCgsWide (CGS2) or kMaxGmresLag (lagged)
     for (int j = (int)threadIdx.x + 1; j <= nI; j += blockDim.x) ysh[j] = ld_dbl(&a.y[j]);
     __syncthreads();
     double v[1] = {0.0};
@@ -523,6 +524,17 @@ __device__ __forceinline__ bool mgmres_phase(const MGmresArgs &m, int ph, int c,
     a.nrm[i + 1] = norm;
     gmres_close_step(a, st, i);
   });
+}
+
+// The cycle start, x update and final phases of the mixed GMRES with lagged
+// CGS2 (fvc_gmres_lag.h), one launch each (their reductions run in the last
+// block to arrive, grid_reduce).
+template <int F>
+__global__ __launch_bounds__(1024) void mgmres_kernel(MGmresArgs m) {
+  __shared__ double lds[kCgsChunk * kMaxWaves];
+  const int ph = m.q.phase;
+  if (ph != MG_FINAL && ld_int(&m.q.st->done)) return;
+  mgmres_phase<F>(m, ph, 0, lds);
 }
 
 // One persistent launch per solve, shaped as the FP64 CGS2 GMRES.

@@ -897,24 +897,33 @@ int run_mixed_krylov(DevState &d, const SolveIn &in, fvc_stats *stats, hipStream
   } else {
     const int neq = in.A.neq;
     const bool gm = in.code == FVC_GMRES;
+    // GMRES with lagged CGS2: one kernel per phase (fvc_gmres_lag.h, FP32 storage)
+    const bool lag = gm && in.orth == FVC_ORTH_CGS2_LAG;
     const int k = in.gmres_k;
-    if (gm && (k < 1 || k > kCgsWide))
-      return fail(FVC_UNSUPPORTED, "mixed-precision GMRES takes k <= %d (asked %d)", kCgsWide, k);
-    if (gm && in.engine == FVC_ENGINE_KERNELS)
-      return fail(FVC_UNSUPPORTED, "mixed-precision GMRES runs as one persistent launch");
+    const int kmax = lag ? kMaxGmresLag : kCgsWide;
+    if (gm && (k < 1 || k > kmax))
+      return fail(FVC_UNSUPPORTED, "mixed-precision GMRES takes k <= %d (asked %d)", kmax, k);
+    if (gm && !lag && in.engine == FVC_ENGINE_KERNELS)
+      return fail(FVC_UNSUPPORTED, "mixed-precision GMRES with CGS2 runs as one persistent launch");
+    if (lag && in.engine == FVC_ENGINE_PERSISTENT)
+      return fail(FVC_UNSUPPORTED, "GMRES with lagged CGS2 runs one kernel per phase (engine %d)", in.engine);
     // BiCGSTAB: one kernel per phase by default, shaped as the FP64 two-phase
-    // BiCGSTAB; GMRES: persistent, shaped as the FP64 CGS2 GMRES
-    const bool persistent = gm || in.engine == FVC_ENGINE_PERSISTENT;
-    const int bs = in.bs > 0 ? in.bs : gm ? 512 : persistent ? kDefaultBlock : kernels_block(FVC_BICGSTAB, true);
+    // BiCGSTAB; GMRES: persistent, shaped as the FP64 CGS2 GMRES, or with
+    // lagged CGS2 one kernel per phase as the FP64 one
+    const bool persistent = (gm && !lag) || in.engine == FVC_ENGINE_PERSISTENT;
+    const int bs = in.bs > 0 ? in.bs : lag ? kLagBlock : gm ? 512 : persistent ? kDefaultBlock
+                                                                   : kernels_block(FVC_BICGSTAB, true);
     if (bs % 64 != 0 || bs > (gm ? 512 : 1024))
       return fail(FVC_BAD_ARGUMENT, "block size %d must be a multiple of 64 <= %d", bs, gm ? 512 : 1024);
+    if (lag && bs != kLagBlock)
+      return fail(FVC_BAD_ARGUMENT, "GMRES with lagged CGS2 runs blocks of %d threads (asked %d)", kLagBlock, bs);
     const void *pfn = gm ? (const void *)&mgmres_persistent<F> : (const void *)&mbicg_persistent<F>;
     const void *kfn = (const void *)&mbicg_kernel<F>;
     int occ = 0;
     int rc = occupancy(pfn, bs, &occ);
     if (rc) return rc;
     if (occ < 1) return fail(FVC_DEVICE_ERROR, "persistent kernel does not fit a CU at block size %d", bs);
-    const int G_req = (gm && in.G_req == 0) ? std::max(8, std::min(3 * d.cus / 2, (neq / 512) & ~7)) : in.G_req;
+    const int G_req = (gm && !lag && in.G_req == 0) ? std::max(8, std::min(3 * d.cus / 2, (neq / 512) & ~7)) : in.G_req;
     const int per_cu = (long)neq >= 24L * 4 * d.cus * bs ? 2 : 4;
     const int G = persistent ? choose_blocks(neq, d.cus, occ, G_req) : kernels_blocks(neq, d.cus, G_req, per_cu);
     if (persistent && G > occ * d.cus)
@@ -929,15 +938,20 @@ int run_mixed_krylov(DevState &d, const SolveIn &in, fvc_stats *stats, hipStream
     }
     constexpr int kW16 = (F == F_ELL4) ? 4 : (F == F_ELL3) ? 3 : 0;
     const size_t n16 = kW16 ? (size_t)kW16 * n : 0;
-    const int nvec = gm ? k + 1 : 9;
+    // GMRES: basis columns 1..k+1; lagged CGS2: columns 1..k, the candidate u and w
+    const int nvec = gm ? k + 1 + (lag ? 1 : 0) : 9;
     const size_t npart = (size_t)(gm ? kCgsChunk : kMaxK) * G;
+    // lagged CGS2: hraw, (nu, a, c), totals; per-block partials
+    const size_t lag_small = lag ? (size_t)(k + 1) * k + 2 * (size_t)k + 3 + (size_t)lag_kp(k) : 0;
+    const size_t lag_parts = lag ? (size_t)lag_kp(k) * G : 0;
     size_t bytes = align_up((kCtrWords + (size_t)G + 1) * sizeof(unsigned)) + align_up(sizeof(KState)) +
                    align_up(npart * sizeof(double)) + align_up(std::max<size_t>(nv, 1) * sizeof(float)) +
                    align_up(n * sizeof(double)) + align_up(n * sizeof(float)) +
                    (size_t)nvec * align_up(n * sizeof(float)) + align_up(2 * sizeof(KState)) +
                    align_up(2 * kMaxK * (size_t)G * sizeof(double)) +
                    align_up(std::max<size_t>(n16, 1) * sizeof(short)) +
-                   align_up((size_t)(k + 2) * (k + 2) * 2 * sizeof(double)) + 4096;
+                   align_up((size_t)(k + 2) * (k + 2) * 2 * sizeof(double)) +
+                   align_up(lag_small * sizeof(double)) + align_up(lag_parts * sizeof(double)) + 4096;
     char *ws = nullptr;
     if ((rc = workspace(d, bytes, &ws))) return rc;
     Carve cv{ws, bytes};
@@ -953,6 +967,8 @@ int run_mixed_krylov(DevState &d, const SolveIn &in, fvc_stats *stats, hipStream
     const size_t fstride = align_up(n * sizeof(float)) / sizeof(float);
     float *fv = cv.take<float>((size_t)nvec * fstride);
     double *gsmall = cv.take<double>((size_t)(k + 2) * (k + 2) * 2);
+    double *lagsmall = cv.take<double>(lag_small);
+    double *lagpart = cv.take<double>(lag_parts);
     if (!cv.fits()) return carve_overflow(cv);
     Mat32 A32{neq, in.A.w, av32, in.A.ia, in.A.ja, nullptr};
     Grid g = make_grid(neq, G, ctr, partials, persistent, in.barrier,
@@ -983,11 +999,57 @@ int run_mixed_krylov(DevState &d, const SolveIn &in, fvc_stats *stats, hipStream
       a.e = sm; sm += k + 2;
       a.y = sm; sm += k + 2;
       a.nrm = sm; sm += k + 2;
-      a.tol = in.tol; a.k = k; a.maxit = in.maxit; a.stop_rule = 1; a.orth = 1;
+      a.tol = in.tol; a.k = k; a.maxit = in.maxit; a.stop_rule = 1; a.orth = lag ? 2 : 1;
       m.A32 = A32;
       m.s = sc;
       m.gv = fv;
-      if ((rc = launch(pfn, true, G, bs, m, s))) return rc;
+      if (lag) {
+        m.gu = fv + (size_t)k * fstride;
+        m.gw = fv + (size_t)(k + 1) * fstride;
+        a.hraw = lagsmall;
+        a.lag = lagsmall + (size_t)(k + 1) * k;
+        a.lag_tot = a.lag + 2 * (size_t)k + 3;
+        a.lag_part = lagpart;
+        a.lag_kp = lag_kp(k);
+        // as the FP64 lagged GMRES: per Arnoldi step D, R, B; a cycle's steps
+        // enqueued in batches, stopped one batch after the device reports
+        // the cycle done
+        const void *mfn = (const void *)&mgmres_kernel<F>;
+        for (int cyc = 0; cyc <= in.maxit; ++cyc) {
+          a.phase = MG_START;
+          if ((rc = launch(mfn, false, G, bs, m, s))) return rc;
+          int j = 1, batch = 4, q = 0;
+          bool waiting[2] = {false, false};
+          while (j <= k + 1) {
+            const int nb = std::min(batch, k + 2 - j);
+            for (int t = 0; t < nb; ++t, ++j) {
+              hipLaunchKernelGGL(mgmres_lag_d_kernel<F>, dim3(G), dim3(bs), 0, s, m);
+              hipLaunchKernelGGL(gmres_lag_r_kernel<F>, dim3(2 * j), dim3(kLagBlock), 0, s, a);
+              if (j <= k) hipLaunchKernelGGL(mgmres_lag_b_kernel<F>, dim3(G), dim3(bs), 0, s, m);
+            }
+            HIPCHK(hipGetLastError());
+            const int slot = q & 1, prev = slot ^ 1;
+            HIPCHK(hipMemcpyAsync(d.host_ring + slot, st, sizeof(KState), hipMemcpyDeviceToHost, s));
+            HIPCHK(hipEventRecord(d.evq[slot], s));
+            waiting[slot] = true;
+            ++q;
+            if (waiting[prev]) {
+              HIPCHK(hipEventSynchronize(d.evq[prev]));
+              waiting[prev] = false;
+              if (d.host_ring[prev].inner_done || d.host_ring[prev].done) break;
+            }
+            batch = std::min(batch * 2, 8);
+          }
+          a.phase = MG_XUPD;
+          if ((rc = launch(mfn, false, G, bs, m, s))) return rc;
+          if ((rc = read_state(d, st, s))) return rc;
+          if (d.host_st->done) break;
+        }
+        a.phase = MG_FINAL;
+        if ((rc = launch(mfn, false, G, bs, m, s))) return rc;
+      } else if ((rc = launch(pfn, true, G, bs, m, s))) {
+        return rc;
+      }
     } else {
       MBicgArgs a{};
       a.A = in.A; a.A32 = A32; a.g = g; a.st = st; a.s = sc; a.dsq = dsq; a.x = in.x; a.b = in.b;
