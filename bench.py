@@ -344,6 +344,34 @@ def spmv_only(n: int, reps: int) -> None:
           flush=True)
 
 
+def gmres_only(n: int, steps: int) -> None:
+    """Only GMRES(30) on the n x n momentum system to true relative residual
+    1e-10, `steps` solves per orthogonalisation (MGS, CGS2, lagged CGS2, and
+    lagged CGS2 in mixed precision): the kernels of each, for a profile."""
+    import fortranfvcell_amd as F
+    from fortranfvcell_amd import problems as P
+    F.init(0)
+    s = P.config_advdiff(n)
+    ell = P.to_ell(s)
+    ds = F.DeviceSystem(ell)
+    out = {}
+    for name, kw in (("mgs", dict(gmres_orth=F.ORTH_MGS)), ("cgs2", dict(gmres_orth=F.ORTH_CGS2)),
+                     ("cgs2_lag", dict(gmres_orth=F.ORTH_CGS2_LAG)),
+                     ("cgs2_lag_mixed", dict(gmres_orth=F.ORTH_CGS2_LAG, precision=F.PREC_MIXED))):
+        ts = []
+        for _ in range(steps):
+            st = ds.solve(F.GMRES, tol=1e-10, gmres_k=30, stop_rule=F.STOP_RELRES, **kw)
+            ts.append(st["seconds"])
+        gb = gmres_solve_bytes(s.neq, ell["nad"], st["iterations"], st["inner_iterations"], 30,
+                               kw["gmres_orth"] == F.ORTH_CGS2, lag=kw["gmres_orth"] == F.ORTH_CGS2_LAG)
+        out[name] = {"cycles": st["iterations"], "arnoldi_steps": st["inner_iterations"],
+                     "ms_per_solve": float(np.mean(ts)) * 1e3, "true_relres": st["true_relres"],
+                     "status": st["status"],
+                     "GBs_algorithmic": (gb / float(np.mean(ts)) / 1e9) if "precision" not in kw else None}
+    print(json.dumps({"workload": f"GMRES(30), {n}x{n} momentum system, true relative residual 1e-10",
+                      "neq": s.neq, "steps": steps, **out}), flush=True)
+
+
 def bench_c3b(n: int, steps: int, maxit: int = 200000) -> None:
     """FP64 and mixed-precision Jacobi-PCG (FP32 matrix and inner iterations,
     FP64 residual recomputation) to true relative residual 1e-10 on the
@@ -398,6 +426,8 @@ def main():
     ap.add_argument("--maxit", type=int, default=200000, help="--c3b: iteration cap of each solve")
     ap.add_argument("--spmv-only", type=int, default=0, metavar="R",
                     help="only run R rotated SpMV launches of the momentum system (byte-counter check)")
+    ap.add_argument("--gmres-only", action="store_true",
+                    help="only run GMRES(30) with each orthogonalisation, --steps solves each (for a profile)")
     args = ap.parse_args()
     if args.calibrate_copy:
         return calibrate_copy(args.calibrate_copy, max(args.steps, 1))
@@ -405,6 +435,8 @@ def main():
         return bench_c3b(args.c3b, max(args.steps, 1), args.maxit)
     if args.spmv_only:
         return spmv_only(args.n, args.spmv_only)
+    if args.gmres_only:
+        return gmres_only(args.n, max(args.steps, 1))
     if args.gpus != 1:
         print(f"bench.py: the solver runs on one GPU only (asked for --gpus {args.gpus}); "
               "there is no multi-GPU path to measure", file=sys.stderr)
