@@ -193,7 +193,8 @@ __global__ __launch_bounds__(kLagBlock) void gmres_lag_r_kernel(GmresArgs a) {
   __shared__ double hc[L + 1];          // H(1:j, 1:j-1) a
   __shared__ double cs[L + 1], sn[L + 1], ee[L + 2];
   __shared__ double col[L + 1];         // column j-1 under rotation; y at the end
-  __shared__ int s_last, s_ni;
+  __shared__ int s_last, s_ni, s_ended;
+  __shared__ double s_aa, s_ab;
   KState *st = a.st;
   if (ld_int(&st->done) || ld_int(&st->inner_done)) return;
   const int j = ld_int(&st->step_i);
@@ -233,39 +234,57 @@ __global__ __launch_bounds__(kLagBlock) void gmres_lag_r_kernel(GmresArgs a) {
     ee[t] = a.e[t];
   }
   __syncthreads();
-  // H(1:j, 1:j-1) a with column j-1 as it stands (its correction follows)
+  // H(1:j, 1:j-1) a with column j-1 as it stands (its correction follows),
+  // one row per thread; a'a and a'b on the first wave
   for (int r = threadIdx.x + 1; r <= j; r += blockDim.x) {
-    double s = 0.0;
-    for (int c = r > 1 ? r - 1 : 1; c <= nv; ++c) s += HR(r, c) * tot[c - 1];   // Hessenberg: c >= r - 1
-    hc[r - 1] = s;
+    double sh = 0.0;
+    for (int c = r > 1 ? r - 1 : 1; c <= nv; ++c) sh += HR(r, c) * tot[c - 1];   // Hessenberg: c >= r - 1
+    hc[r - 1] = sh;
+  }
+  if (threadIdx.x < kWave) {
+    double pa = 0.0, pb = 0.0;
+    for (int q = (int)threadIdx.x; q < nv; q += kWave) {
+      pa += tot[q] * tot[q];
+      pb += tot[q] * tot[j + q];
+    }
+    pa = wave_sum(pa);
+    pb = wave_sum(pb);
+    if (threadIdx.x == 0) {
+      s_aa = pa;
+      s_ab = pb;
+    }
   }
   __syncthreads();
-  if (threadIdx.x == 0) {
-    s_ni = 0;
-    double aa = 0.0, ab = 0.0;
-    for (int q = 0; q < nv; ++q) {
-      aa += tot[q] * tot[q];
-      ab += tot[q] * tot[j + q];
-    }
-    const double beta = tot[nv], gamma = tot[2 * j - 1];
-    const double nu2 = beta - aa;
-    const double nu = nu2 > 0.0 ? sqrt(nu2) : 0.0;
-    bool ended = false;
-    if (j == 1) {
+  const double beta = tot[nv], gamma = tot[2 * j - 1];
+  const double nu2 = beta - s_aa;
+  const double nu = nu2 > 0.0 ? sqrt(nu2) : 0.0;
+  if (threadIdx.x == 0) s_ni = 0;
+  if (j == 1) {
+    if (threadIdx.x == 0) {
       a.e[1] = nu;   // ||r0 / m||: the least-squares right-hand side
-      ee[1] = nu;
       a.nrm[1] = nu;
-    } else {
-      // column i = j-1 is finished: first pass + re-orthogonalisation, nu below
-      const int i = nv;
-      for (int r = 1; r <= i; ++r) HR(r, i) = HR(r, i) + tot[r - 1];
+      s_ended = 0;
+    }
+  } else {
+    // column i = j-1 is finished: first pass + re-orthogonalisation, nu below;
+    // (H a)_r gains a_{j-1} a_r from the correction
+    const int i = nv;
+    for (int r = threadIdx.x + 1; r <= i; r += blockDim.x) {
+      HR(r, i) = HR(r, i) + tot[r - 1];
+      hc[r - 1] += tot[r - 1] * tot[i - 1];
+    }
+    if (threadIdx.x == 0) {
       HR(j, i) = nu;
-      for (int r = 1; r <= j; ++r) Hg(a.hraw, r, i) = HR(r, i);
-      // (H a)_r with the finished column: the a_{j-1} a_r term it gained
-      for (int r = 1; r <= i; ++r) hc[r - 1] += tot[r - 1] * tot[i - 1];
       hc[i] = nu * tot[i - 1];
+    }
+    __syncthreads();
+    for (int r = threadIdx.x + 1; r <= j; r += blockDim.x) {
+      Hg(a.hraw, r, i) = HR(r, i);
+      col[r] = HR(r, i);
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
       // the Givens rotations of src/Iterativos.f:363-375, as gmres_close_step
-      for (int r = 1; r <= j; ++r) col[r] = HR(r, i);
       for (int jj = 1; jj <= i - 1; ++jj) {
         const double aux1 = cs[jj] * col[jj] + sn[jj] * col[jj + 1];
         const double aux2 = -sn[jj] * col[jj] + cs[jj] * col[jj + 1];
@@ -279,23 +298,27 @@ __global__ __launch_bounds__(kLagBlock) void gmres_lag_r_kernel(GmresArgs a) {
       col[i + 1] = 0.0;
       ee[i + 1] = -sn[i] * ee[i];
       ee[i] = cs[i] * ee[i];
-      for (int r = 1; r <= j; ++r) Hg(a.h, r, i) = col[r];
       a.cs[i] = cs[i];
       a.sn[i] = sn[i];
       a.e[i] = ee[i];
       a.e[i + 1] = ee[i + 1];
       st->inner = st->inner + 1;
       const bool conv = fabs(ee[i + 1]) <= st->econv;   // src/Iterativos.f:376
-      if (conv || i == k || !finite(ee[i + 1])) {
+      s_ended = conv || i == k || !finite(ee[i + 1]);
+      if (s_ended) {
         st->ni = i;
         st->inner_done = 1;
         s_ni = i;
-        ended = true;
       }
     }
-    if (!ended) {
-      if (!(nu > 0.0) || !finite(nu)) {
-        // u_j lies in span(V_{j-1}) (or is not finite): no column j
+    __syncthreads();
+    for (int r = threadIdx.x + 1; r <= j; r += blockDim.x) Hg(a.h, r, i) = col[r];
+  }
+  __syncthreads();
+  if (!s_ended) {
+    if (!(nu > 0.0) || !finite(nu)) {
+      // u_j lies in span(V_{j-1}) (or is not finite): no column j
+      if (threadIdx.x == 0) {
         st->ni = nv;
         st->inner_done = 1;
         s_ni = nv;
@@ -304,17 +327,19 @@ __global__ __launch_bounds__(kLagBlock) void gmres_lag_r_kernel(GmresArgs a) {
           st->done = 1;
           s_ni = 0;
         }
-      } else {
-        // column j, first pass: h = (V_j' w - H a) / nu; next candidate's c
-        double *la = a.lag;   // la[0] = nu, la[1..j-1] = a, la[k+1..k+j] = c
+      }
+    } else {
+      // column j, first pass: h = (V_j' w - H a) / nu; next candidate's c
+      double *la = a.lag;   // la[0] = nu, la[1..j-1] = a, la[k+1..k+j] = c
+      for (int r = threadIdx.x + 1; r <= j; r += blockDim.x) {
+        const double vtw = r < j ? tot[j + r - 1] : (gamma - s_ab) / nu;
+        const double h1 = (vtw - hc[r - 1]) / nu;
+        Hg(a.hraw, r, j) = h1;
+        la[k + r] = hc[r - 1] / nu + h1;
+        if (r <= nv) la[r] = tot[r - 1];
+      }
+      if (threadIdx.x == 0) {
         la[0] = nu;
-        for (int r = 1; r <= nv; ++r) la[r] = tot[r - 1];
-        for (int r = 1; r <= j; ++r) {
-          const double vtw = r < j ? tot[j + r - 1] : (gamma - ab) / nu;
-          const double h1 = (vtw - hc[r - 1]) / nu;
-          Hg(a.hraw, r, j) = h1;
-          la[k + r] = hc[r - 1] / nu + h1;
-        }
         a.nrm[j] = nu;
         st->step_i = j + 1;
       }

@@ -19,6 +19,10 @@ def _cases(fvc):
         ("pcg", fvc.PCG, P.config_pressure_lognormal(384), P.config_pressure_lognormal(200), {}),
         ("bicgstab", fvc.BICGSTAB, P.config_advdiff(320), P.config_advdiff(150), {}),
         ("gmres10", fvc.GMRES, P.config_advdiff(160), P.config_advdiff(100), dict(gmres_k=10)),
+        ("gmres10_lagged_cgs2", fvc.GMRES, P.config_advdiff(160), P.config_advdiff(100),
+         dict(gmres_k=10, gmres_orth=fvc.ORTH_CGS2_LAG)),
+        ("gmres10_lagged_cgs2_mixed", fvc.GMRES, P.config_advdiff(160), P.config_advdiff(100),
+         dict(gmres_k=10, gmres_orth=fvc.ORTH_CGS2_LAG, precision=fvc.PREC_MIXED)),
         ("pcg_mixed", fvc.PCG, P.config_pressure_lognormal(300), P.config_pressure_lognormal(120),
          dict(precision=fvc.PREC_MIXED)),
     ]
