@@ -37,6 +37,7 @@ namespace fvc {
 constexpr int kLagChunk = 8;      // basis columns loaded together by B
 constexpr int kLagBlock = 256;    // threads per block of D and B
 constexpr int kLagTile = 1024;    // rows of u and w a D block stages in LDS at a time
+constexpr int kLagRows = 8;       // rows per lane of a basis column D loads together
 constexpr int kLagPerWave = (kMaxGmresLag + kLagBlock / kWave - 1) / (kLagBlock / kWave);
 // arrival counter of the R blocks: a word of the grid counters that the
 // kernel-per-phase reductions leave unused (zeroed with them per solve)
@@ -126,13 +127,13 @@ __device__ __forceinline__ void gmres_lag_d(const P &pol) {
     for (int c = wid; c < nv; c += nwv) {
       double sa = 0.0, sb = 0.0;
       int q = lane;
-      // four rows per lane loaded together, then the rest one at a time
-      for (; q + 3 * kWave < tn; q += 4 * kWave) {
-        double g[4];
+      // kLagRows rows per lane loaded together, then the rest one at a time
+      for (; q + (kLagRows - 1) * kWave < tn; q += kLagRows * kWave) {
+        double g[kLagRows];
 #pragma unroll
-        for (int u = 0; u < 4; ++u) g[u] = pol.col(c, t0 + q + u * kWave);
+        for (int u = 0; u < kLagRows; ++u) g[u] = pol.col(c, t0 + q + u * kWave);
 #pragma unroll
-        for (int u = 0; u < 4; ++u) {
+        for (int u = 0; u < kLagRows; ++u) {
           sa += g[u] * us[q + u * kWave];
           sb += g[u] * ws[q + u * kWave];
         }
