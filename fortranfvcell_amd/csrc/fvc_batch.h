@@ -6,10 +6,10 @@
 // system gets its own group of consecutive blocks (sized by the host from a
 // work estimate), its own barrier counters, reduction partials and solver
 // state, and runs the unchanged single-system phase functions on pointers
-// offset to its rows.  Groups never wait on each other, so a small system
-// finishes in a few hundred fast block-level iterations while large ones
-// keep their blocks busy; all blocks are co-resident (the grid fits the
-// device at once).
+// offset to its rows.  Groups never wait on each other.  A system too small
+// to keep a block busy for the batch's duration shares its block with other
+// small systems: the block solves them one after another, each as a group of
+// one block.  All blocks are co-resident (the grid fits the device at once).
 #pragma once
 #include "fvc_mixed.h"
 
@@ -31,7 +31,8 @@ struct BatchArgs {
   const int *ia;            // CSRD: concatenated row pointers, each local and 1-based
   const int *ja;            // concatenated ja, local 1-based indices
   const SysDesc *desc;
-  const int *block_sys;     // system of each block
+  const int *block_ptr;     // block b runs systems block_list[block_ptr[b] ..
+  const int *block_list;    //   block_ptr[b + 1]) in turn
   KState *st;               // per system
   unsigned *ctr;            // kCtrWords per system
   unsigned *flags;          // one arrival flag per block
@@ -89,33 +90,39 @@ __global__ __launch_bounds__(1024) void pcg_batch_persistent(BatchArgs B) {
   __shared__ unsigned s_ep;
   if (threadIdx.x == 0) s_ep = 0u;
   __syncthreads();
-  const int sys = B.block_sys[blockIdx.x];
-  const SysDesc D = B.desc[sys];
-  const size_t off = (size_t)D.row_off;
-  PcgArgs a{};
-  batch_mat<F>(B, D, a.A);
-  batch_grid(B, sys, D, a.g);
-  a.g.ep = &s_ep;
-  a.st = B.st + sys;
-  a.m = a.A.ad;
-  a.x = B.x + off;
-  a.b = B.b + off;
-  a.r = B.v[0] + off;
-  a.z = B.v[1] + off;
-  a.p0 = B.v[2] + off;
-  a.p1 = B.v[3] + off;
-  a.q = B.v[4] + off;
-  a.tol = B.tol;
-  a.maxit = B.maxit;
-  a.x0_zero = B.x0_zero;
-  a.stop_rule = B.stop_rule;
-  if (!pcg_phase<F>(a, PCG_SETUP, lds)) return;
-  while (!ld_int(&a.st->done)) {
-    if (!pcg_phase<F>(a, PCG_A, lds)) return;
-    if (ld_int(&a.st->done)) break;
-    if (!pcg_phase<F>(a, PCG_B, lds)) return;
+  // s_ep keeps counting across the systems of a block: its arrival flag only
+  // grows, while each system's counters start from zero
+  for (int k = B.block_ptr[blockIdx.x]; k < B.block_ptr[blockIdx.x + 1]; ++k) {
+    const int sys = B.block_list[k];
+    const SysDesc D = B.desc[sys];
+    const size_t off = (size_t)D.row_off;
+    PcgArgs a{};
+    batch_mat<F>(B, D, a.A);
+    batch_grid(B, sys, D, a.g);
+    a.g.ep = &s_ep;
+    a.st = B.st + sys;
+    a.m = a.A.ad;
+    a.x = B.x + off;
+    a.b = B.b + off;
+    a.r = B.v[0] + off;
+    a.z = B.v[1] + off;
+    a.p0 = B.v[2] + off;
+    a.p1 = B.v[3] + off;
+    a.q = B.v[4] + off;
+    a.tol = B.tol;
+    a.maxit = B.maxit;
+    a.x0_zero = B.x0_zero;
+    a.stop_rule = B.stop_rule;
+    if (!pcg_phase<F>(a, PCG_SETUP, lds)) return;
+    bool ok = true;
+    while (ok && !ld_int(&a.st->done)) {
+      if (!(ok = pcg_phase<F>(a, PCG_A, lds))) break;
+      if (ld_int(&a.st->done)) break;
+      ok = pcg_phase<F>(a, PCG_B, lds);
+    }
+    if (!ok || !pcg_phase<F>(a, PCG_FINAL, lds)) return;
+    __syncthreads();   // every thread is past this system before LDS is reused
   }
-  pcg_phase<F>(a, PCG_FINAL, lds);
 }
 
 // Mixed precision (fvc_mixed.h) per system.
@@ -125,33 +132,36 @@ __global__ __launch_bounds__(1024) void mixed_batch_persistent(BatchArgs B) {
   __shared__ unsigned s_ep;
   if (threadIdx.x == 0) s_ep = 0u;
   __syncthreads();
-  const int sys = B.block_sys[blockIdx.x];
-  const SysDesc D = B.desc[sys];
-  const size_t off = (size_t)D.row_off;
-  MixedArgs a{};
-  batch_mat<F>(B, D, a.A);
-  a.A32.neq = D.neq;
-  a.A32.w = B.w;
-  a.A32.av = B.av32 + D.val_off;
-  a.A32.ia = a.A.ia;
-  a.A32.ja = a.A.ja;
-  a.A32.jo = B.jo ? B.jo + off * B.w : nullptr;
-  batch_grid(B, sys, D, a.g);
-  a.g.ep = &s_ep;
-  a.st = B.st + sys;
-  a.m = a.A.ad;
-  a.s = B.s + off;
-  a.dsq = B.dsq + off;
-  a.x = B.x + off;
-  a.b = B.b + off;
-  a.rec = B.rec + off;
-  a.ps = B.fs;
-  a.tol = B.tol;
-  a.delta = B.delta;
-  a.maxit = B.maxit;
-  a.x0_zero = B.x0_zero;
-  a.max_inner = B.max_inner;
-  mixed_run<F>(a, lds);
+  for (int k = B.block_ptr[blockIdx.x]; k < B.block_ptr[blockIdx.x + 1]; ++k) {
+    const int sys = B.block_list[k];
+    const SysDesc D = B.desc[sys];
+    const size_t off = (size_t)D.row_off;
+    MixedArgs a{};
+    batch_mat<F>(B, D, a.A);
+    a.A32.neq = D.neq;
+    a.A32.w = B.w;
+    a.A32.av = B.av32 + D.val_off;
+    a.A32.ia = a.A.ia;
+    a.A32.ja = a.A.ja;
+    a.A32.jo = B.jo ? B.jo + off * B.w : nullptr;
+    batch_grid(B, sys, D, a.g);
+    a.g.ep = &s_ep;
+    a.st = B.st + sys;
+    a.m = a.A.ad;
+    a.s = B.s + off;
+    a.dsq = B.dsq + off;
+    a.x = B.x + off;
+    a.b = B.b + off;
+    a.rec = B.rec + off;
+    a.ps = B.fs;
+    a.tol = B.tol;
+    a.delta = B.delta;
+    a.maxit = B.maxit;
+    a.x0_zero = B.x0_zero;
+    a.max_inner = B.max_inner;
+    if (!mixed_run<F>(a, lds)) return;
+    __syncthreads();
+  }
 }
 
 // S A S in FP32 for every system of the batch (rows are independent; the

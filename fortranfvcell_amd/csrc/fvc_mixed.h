@@ -410,11 +410,11 @@ __device__ __forceinline__ bool mixed_phase(const MixedArgs &a, int ph, double *
 
 // Persistent driver (one system, or one system's block group of a batch).
 template <int F>
-__device__ void mixed_run(const MixedArgs &a, double *lds) {
-  if (!mixed_phase<F>(a, MX_SETUP, lds)) return;
+__device__ bool mixed_run(const MixedArgs &a, double *lds) {
+  if (!mixed_phase<F>(a, MX_SETUP, lds)) return false;
   while (!ld_int(&a.st->done))
-    if (!mixed_phase<F>(a, ld_int(&a.st->next), lds)) return;
-  mixed_phase<F>(a, MX_FINAL, lds);
+    if (!mixed_phase<F>(a, ld_int(&a.st->next), lds)) return false;
+  return mixed_phase<F>(a, MX_FINAL, lds);
 }
 
 template <int F>

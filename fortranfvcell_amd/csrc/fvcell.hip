@@ -1555,7 +1555,8 @@ int run_batch(DevState &d, int nsys, const std::vector<int> &row_off, const std:
   const int cap = std::max(1, o.grid_blocks > 0 ? std::min(o.grid_blocks, occ * d.cus)
                                                : d.cus * std::max(1, std::min(occ, 4)));
   const int total = row_off[nsys];
-  // systems go in waves of at most cap/2 systems (each needs a block)
+  // systems go in waves of at most cap/2 systems (bounds the per-system
+  // state; the plan packs small systems onto shared blocks)
   const int wave = std::max(1, std::min(nsys, cap / 2 > 0 ? cap / 2 : 1));
   const double expo = o.batch_work_exp10 > 0 ? o.batch_work_exp10 / 10.0 : 1.5;
   if (o.batch_rows_per_wave < 0) return fail(FVC_BAD_ARGUMENT, "batch_rows_per_wave %d", o.batch_rows_per_wave);
@@ -1564,7 +1565,7 @@ int run_batch(DevState &d, int nsys, const std::vector<int> &row_off, const std:
   const size_t nv = (size_t)val_off[nsys];   // stored off-diagonal values
   size_t bytes = align_up(((size_t)wave * kCtrWords + cap) * sizeof(unsigned)) + align_up((size_t)wave * sizeof(KState)) +
                  align_up((size_t)kMaxK * cap * sizeof(double)) + align_up((size_t)wave * sizeof(SysDesc)) +
-                 align_up((size_t)cap * sizeof(int)) + 4096;
+                 align_up((size_t)(2 * cap + 1 + wave) * sizeof(int)) + 4096;
   constexpr bool kOff16 = (F == F_ELL4 || F == F_ELL3);
   if (mixed) bytes += 2 * align_up(n * sizeof(float4)) + align_up(std::max<size_t>(nv, 1) * sizeof(float)) +
                       align_up(n * sizeof(double)) + align_up(n * sizeof(float)) +
@@ -1576,7 +1577,7 @@ int run_batch(DevState &d, int nsys, const std::vector<int> &row_off, const std:
   KState *st = cv.take<KState>(wave);
   double *partials = cv.take<double>((size_t)kMaxK * cap);
   SysDesc *ddesc = cv.take<SysDesc>(wave);
-  int *dbsys = cv.take<int>(cap);
+  int *dplan = cv.take<int>((size_t)2 * cap + 1 + wave);   // block_ptr, then block_list
   BatchArgs B{};
   B.w = w;
   B.ad = ad;
@@ -1613,7 +1614,7 @@ int run_batch(DevState &d, int nsys, const std::vector<int> &row_off, const std:
   B.st = st;
   B.partials = partials;
   B.desc = ddesc;
-  B.block_sys = dbsys;
+  B.block_ptr = dplan;
   HIPCHK(hipEventRecord(d.ev0, s));
   if (mixed) {
     if (B.jo) hipLaunchKernelGGL(reset_kernel, dim3(1), dim3(64), 0, s, reinterpret_cast<unsigned *>(B.wide), 1, st, 0);
@@ -1632,23 +1633,26 @@ int run_batch(DevState &d, int nsys, const std::vector<int> &row_off, const std:
     const int m = std::min(wave, nsys - s0);
     std::vector<int> sizes(m);
     for (int i = 0; i < m; ++i) sizes[i] = row_off[s0 + i + 1] - row_off[s0 + i];
-    std::vector<int> g = allocate_blocks(sizes, cap, expo, min_rows);
+    const BlockPlan plan = plan_blocks(sizes, cap, expo, min_rows);
     std::vector<SysDesc> desc(m);
-    std::vector<int> bsys;
-    int first = 0;
-    for (int i = 0; i < m; ++i) {
-      desc[i] = SysDesc{row_off[s0 + i], sizes[i], first, g[i], val_off[s0 + i], row_off[s0 + i] + s0 + i};
-      for (int k = 0; k < g[i]; ++k) bsys.push_back(i);
-      first += g[i];
-    }
-    const int G = first;
+    for (int i = 0; i < m; ++i)
+      desc[i] = SysDesc{row_off[s0 + i], sizes[i], plan.first[i], plan.count[i], val_off[s0 + i],
+                        row_off[s0 + i] + s0 + i};
+    const int G = plan.blocks;
+    // block_ptr (G + 1 entries) then block_list (one per block of a group,
+    // one per packed system: at most cap + m)
+    std::vector<int> hplan(plan.ptr);
+    hplan.insert(hplan.end(), plan.list.begin(), plan.list.end());
+    if (G > cap || hplan.size() > (size_t)2 * cap + 1 + wave)
+      return fail(FVC_DEVICE_ERROR, "batch plan of %d blocks for capacity %d", G, cap);
+    B.block_list = dplan + G + 1;
     HIPCHK(hipMemcpyAsync(ddesc, desc.data(), m * sizeof(SysDesc), hipMemcpyHostToDevice, s));
-    HIPCHK(hipMemcpyAsync(dbsys, bsys.data(), G * sizeof(int), hipMemcpyHostToDevice, s));
+    HIPCHK(hipMemcpyAsync(dplan, hplan.data(), hplan.size() * sizeof(int), hipMemcpyHostToDevice, s));
     hipLaunchKernelGGL(reset_kernel, dim3(1), dim3(256), 0, s, ctr, wave * kCtrWords + cap, st, m);
     HIPCHK(hipGetLastError());
     if ((rc = launch(fn, true, G, bs, B, s))) return rc;
     HIPCHK(hipMemcpyAsync(host_st.data() + s0, st, m * sizeof(KState), hipMemcpyDeviceToHost, s));
-    HIPCHK(hipStreamSynchronize(s));   // desc/bsys host buffers are reused next wave
+    HIPCHK(hipStreamSynchronize(s));   // desc/hplan host buffers are reused next wave
     unsigned err = 0;
     for (int i = 0; i < m; ++i) {
       HIPCHK(hipMemcpy(&err, ctr + (size_t)i * kCtrWords + kCtrErr, sizeof err, hipMemcpyDeviceToHost));

@@ -62,21 +62,80 @@ static void check_blocks(std::mt19937 &rng, int nsys, int cap) {
   std::vector<int> neq(nsys);
   for (int &v : neq) v = 1 + (int)(rng() % 200000u) * ((rng() & 3) ? 1 : 0) + (int)(rng() % 64u);
   for (double expo : {0.5, 1.0, 1.5, 2.0}) {
-    const std::vector<int> g = allocate_blocks(neq, cap, expo);
-    CHECK((int)g.size() == nsys, "size");
-    long sum = 0;
-    for (int i = 0; i < nsys; ++i) {
-      const int lim = std::max(1, (neq[i] + 63) / 64);
-      CHECK(g[i] >= 1, "system %d gets %d blocks", i, g[i]);
-      if (cap > nsys) CHECK(g[i] <= lim, "system %d: %d blocks for %d rows", i, g[i], neq[i]);
-      sum += g[i];
+    const BlockPlan p = plan_blocks(neq, cap, expo);
+    CHECK((int)p.first.size() == nsys && (int)p.count.size() == nsys, "size");
+    CHECK(p.blocks >= 1 && p.blocks <= std::max(cap, 1), "%d blocks for capacity %d", p.blocks, cap);
+    CHECK((int)p.ptr.size() == p.blocks + 1 && p.ptr[0] == 0 && p.ptr.back() == (int)p.list.size(), "ptr");
+    if ((int)p.ptr.size() != p.blocks + 1) continue;
+    // every system runs exactly once per block of its group: a group of g
+    // blocks is g blocks running that system alone; a packed system is one
+    // entry in its shared block's list
+    std::vector<int> seen(nsys, 0);
+    std::vector<double> load(p.blocks, 0.0);
+    for (int b = 0; b < p.blocks; ++b) {
+      CHECK(p.ptr[b] < p.ptr[b + 1], "block %d runs nothing", b);
+      for (int k = p.ptr[b]; k < p.ptr[b + 1]; ++k) {
+        const int i = p.list[k];
+        CHECK(i >= 0 && i < nsys, "system %d", i);
+        if (i < 0 || i >= nsys) continue;
+        ++seen[i];
+        CHECK(b >= p.first[i] && b < p.first[i] + p.count[i], "system %d on block %d outside its group", i, b);
+        if (p.count[i] > 1) CHECK(p.ptr[b + 1] - p.ptr[b] == 1, "group block %d shared", b);
+        load[b] += std::pow((double)neq[i], expo) / p.count[i];
+      }
     }
-    CHECK(sum <= std::max(cap, nsys), "%ld blocks for capacity %d and %d systems", sum, cap, nsys);
-    // shares grow with size: a larger system gets at most one block fewer
-    // than a smaller one (the one a largest-remainder rounding can move)
-    for (int i = 0; i < nsys; ++i)
-      for (int j = 0; j < nsys; ++j)
-        if (cap > nsys && neq[i] > neq[j]) CHECK(g[i] + 1 >= g[j], "monotone %d vs %d", i, j);
+    double mx = 0.0, tot = 0.0;
+    for (int i = 0; i < nsys; ++i) {
+      CHECK(seen[i] == p.count[i], "system %d runs on %d blocks, group of %d", i, seen[i], p.count[i]);
+      const int lim = std::max(1, (neq[i] + 63) / 64);
+      CHECK(p.count[i] >= 1 && p.count[i] <= lim, "system %d: %d blocks for %d rows", i, p.count[i], neq[i]);
+      tot += std::pow((double)neq[i], expo);
+    }
+    for (double l : load) mx = std::max(mx, l);
+    CHECK(std::fabs(mx - p.max_load) <= 1e-9 * mx, "reported load %g, actual %g", p.max_load, mx);
+    // no plan beats the average load, nor a system's work over its limit
+    CHECK(mx >= tot / std::max(cap, 1) * (1 - 1e-12), "load %g below the average %g", mx, tot / cap);
+    // against the plan without sharing (each system its own group, blocks
+    // one at a time to the largest work per block): never worse when every
+    // system can have a block
+    if (cap >= nsys) {
+      std::vector<double> w(nsys);
+      std::vector<int> lim(nsys), idx(nsys), g(nsys);
+      for (int i = 0; i < nsys; ++i) {
+        w[i] = std::pow((double)neq[i], expo);
+        lim[i] = std::max(1, (neq[i] + 63) / 64);
+        idx[i] = i;
+      }
+      const double solo = detail::minimax_groups(w, lim, idx, cap, g);
+      CHECK(mx <= solo * (1 + 1e-12), "plan load %g above the unshared plan's %g", mx, solo);
+    }
+  }
+  // brute force on tiny cases: the groups of the unshared plan are optimal
+  for (int t = 0; t < 50; ++t) {
+    const int n = 1 + (int)(rng() % 3u), c = n + (int)(rng() % 5u);
+    std::vector<double> w(n);
+    std::vector<int> lim(n), idx(n), g(n);
+    for (int i = 0; i < n; ++i) {
+      w[i] = 1.0 + (double)(rng() % 1000u);
+      lim[i] = 1 + (int)(rng() % 4u);
+      idx[i] = i;
+    }
+    const double got = detail::minimax_groups(w, lim, idx, c, g);
+    double best = 1e300;
+    std::vector<int> q(n, 1);
+    for (;;) {
+      int used = 0;
+      double m = 0.0;
+      for (int i = 0; i < n; ++i) {
+        used += q[i];
+        m = std::max(m, w[i] / q[i]);
+      }
+      if (used <= c) best = std::min(best, m);
+      int i = 0;
+      while (i < n && ++q[i] > lim[i]) q[i++] = 1;
+      if (i == n) break;
+    }
+    CHECK(std::fabs(got - best) <= 1e-12 * best, "minimax %g, brute force %g", got, best);
   }
 }
 
