@@ -372,6 +372,39 @@ def gmres_only(n: int, steps: int) -> None:
                       "neq": s.neq, "steps": steps, **out}), flush=True)
 
 
+def batch_only(steps: int) -> None:
+    """Only the 256-system batch (pressure systems, sides log-uniform in
+    [32, 512], each to true relative residual 1e-10), FP64 and mixed, at the
+    options the autotuner picks on the GPU this was written on (1,024 threads,
+    work exponent 1.5), `steps` batched solves each: the batch kernels, for a
+    profile."""
+    import fortranfvcell_amd as F
+    from fortranfvcell_amd import problems as P
+    F.init(0)
+    fmts = [P.to_ell(q) for q in P.config_batch()]
+    sizes = [(int(f["neq"]), int(f["nad"])) for f in fmts]
+    db = F.DeviceBatch(F.concat_ell(fmts))
+    out = {}
+    for name, prec in (("fp64", F.PREC_FP64), ("mixed", F.PREC_MIXED)):
+        kw = dict(tol=1e-10, precision=prec, stop_rule=F.STOP_RELRES, block_size=1024, batch_work_exp10=15)
+        ts = []
+        for _ in range(steps):
+            sts = db.solve(**kw)
+            ts.append(sts[0]["seconds"])
+        if prec == F.PREC_FP64:
+            nb = sum(q["iterations"] * pcg_ref_iter_bytes(ne, nz) for q, (ne, nz) in zip(sts, sizes))
+        else:
+            nb = sum(q["iterations"] * mixed_inner_bytes(ne, nz, bool(q["flags"] & F.FLAG_COL16))
+                     + q["inner_iterations"] * (spmv_bytes(ne, nz) + 40 * ne) for q, (ne, nz) in zip(sts, sizes))
+        t = float(np.mean(ts))
+        out[name] = {"ms": t * 1e3, "solves_per_s": len(sts) / t, "algorithmic_bytes": nb,
+                     "GBs_algorithmic": nb / t / 1e9, "frac": nb / t / 1e9 / HBM_PEAK_GBS,
+                     "max_iterations": max(q["iterations"] for q in sts),
+                     "all_converged": all(q["status"] == 0 for q in sts)}
+    print(json.dumps({"workload": "256 pressure systems, sides log-uniform in [32, 512], true relative residual 1e-10",
+                      "rows": int(sum(ne for ne, _ in sizes)), "steps": steps, **out}), flush=True)
+
+
 def bench_c3b(n: int, steps: int, maxit: int = 200000) -> None:
     """FP64 and mixed-precision Jacobi-PCG (FP32 matrix and inner iterations,
     FP64 residual recomputation) to true relative residual 1e-10 on the
@@ -428,6 +461,8 @@ def main():
                     help="only run R rotated SpMV launches of the momentum system (byte-counter check)")
     ap.add_argument("--gmres-only", action="store_true",
                     help="only run GMRES(30) with each orthogonalisation, --steps solves each (for a profile)")
+    ap.add_argument("--batch-only", action="store_true",
+                    help="only run the 256-system batch, FP64 and mixed, --steps batched solves each (for a profile)")
     args = ap.parse_args()
     if args.calibrate_copy:
         return calibrate_copy(args.calibrate_copy, max(args.steps, 1))
@@ -437,6 +472,8 @@ def main():
         return spmv_only(args.n, args.spmv_only)
     if args.gmres_only:
         return gmres_only(args.n, max(args.steps, 1))
+    if args.batch_only:
+        return batch_only(max(args.steps, 1))
     if args.gpus != 1:
         print(f"bench.py: the solver runs on one GPU only (asked for --gpus {args.gpus}); "
               "there is no multi-GPU path to measure", file=sys.stderr)
