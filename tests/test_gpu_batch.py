@@ -100,6 +100,33 @@ def test_batch_uneven_sizes_and_waves(fvc):
         assert np.linalg.norm(s.b - P.scipy_csr(s) @ xi) <= 1.01e-10 * np.linalg.norm(s.b)
 
 
+@pytest.mark.parametrize("barrier", [0, 1], ids=["flags", "counters"])
+@pytest.mark.parametrize("precision", [0, 1], ids=["fp64", "mixed"])
+def test_batch_shared_blocks_match_solo(fvc, precision, barrier):
+    """Systems too small to keep a block busy share blocks, solved one after
+    another on each.  A system of at most 64 rows runs as a group of one
+    block whether it shares or is alone, so its solution and iteration count
+    must equal those of a batch of its own, bit for bit.  A barrier epoch
+    carried over wrongly from the previous system on the block, or state
+    left behind, would show here."""
+    tiny = _systems(120, 4, 8, 21)                 # 16 to 64 rows
+    big = P.config_pressure_lognormal(160)         # 25,600 rows
+    systems = tiny + [big]
+    fmts = [P.to_ell(s) for s in systems]
+    batch = fvc.concat_ell(fmts)
+    print(f"shared blocks: {len(fmts)} systems, precision={precision} barrier={barrier}", flush=True)
+    kw = dict(tol=1e-10, stop_rule=fvc.STOP_RELRES, precision=precision, barrier=barrier)
+    x, sts = fvc.solve_batch(batch, **kw)
+    ro = batch["row_offsets"]
+    assert all(st["status"] == 0 for st in sts)
+    r = np.linalg.norm(big.b - P.scipy_csr(big) @ x[ro[-2]:ro[-1]]) / np.linalg.norm(big.b)
+    assert r <= 1.01e-10
+    for i in range(0, len(tiny), 7):
+        xs, ss = fvc.solve_batch(fvc.concat_ell([fmts[i]]), **kw)
+        assert ss[0]["iterations"] == sts[i]["iterations"], i
+        assert np.array_equal(xs, x[ro[i]:ro[i + 1]]), i
+
+
 def test_batch_device_resident(fvc):
     systems = _systems(16, 16, 64, 5)
     batch = fvc.concat_ell([P.to_ell(s) for s in systems])
