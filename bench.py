@@ -7,14 +7,15 @@ SIMPLE step hands to its solver (src/CellibSimple.f:347-355: transient term
 rho area/dt with dt = 0.005, under-relaxation underU = 0.8) on a 1024x1024
 jittered quad mesh (1,048,576 cells, largest cell Peclet number 100, ELLPACK
 width 4, FP64), Jacobi-BiCGSTAB to ||b - A x||_2 / ||b||_2 <= 1e-10.  The
-same line reports: GMRES(30) on that system to the same true residual (both
-orthogonalisations); the same system at the reference's defaults (underU =
+same line reports: GMRES(30) on that system to the same true residual (MGS,
+CGS2 and lagged CGS2); the same system at the reference's defaults (underU =
 1, no transient term: the steady a_P = sum a_nb + sum F_f), which Jacobi
 Krylov solvers converge on up to 256x256 cells; Jacobi-PCG on the pinned
 pressure system; the plain ELLPACK SpMV (warm and with the 256 MB last-level
 cache defeated by rotating buffers); FP64 against mixed-precision PCG on the
-lognormal pressure system; the 256-system batch (FP64, mixed, one at a
-time) with its roofline; the roofline of the dominant kernel; and the
+lognormal pressure system, and FP64 against mixed-precision BiCGSTAB and
+GMRES(30) on the momentum system; the 256-system batch (FP64, mixed, one at
+a time) with its roofline; the roofline of the dominant kernel; and the
 reference's own OpenMP Fortran solver (compiled from its sources,
 oracle/_ref) timed on the host cores as the CPU baseline, at the tolerance
 where its own stopping rule reaches the same true residual.
@@ -22,6 +23,9 @@ where its own stopping rule reaches the same true residual.
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W]
        python bench.py --spmv-only R      (rotated SpMV launches only, for counters)
        python bench.py --c3b N [--maxit M] (FP64 vs mixed PCG on 2 N^2 triangles)
+       python bench.py --gmres-only [--steps K] (GMRES(30) per orthogonalisation, for a profile)
+       python bench.py --batch-only [--steps K] (the 256-system batch, for a profile)
+       python bench.py --calibrate-copy MiB  (a plain copy, to scale the byte counters)
 """
 from __future__ import annotations
 
