@@ -239,8 +239,11 @@ int fvc_solve_dev_noalloc(int32_t code, int32_t matriz, int32_t unsym, int32_t n
  * the reference's layout (local 1-based ja), concatenated back to back:
  * system s owns rows row_offsets[s] .. row_offsets[s+1]-1 (row_offsets[0] = 0)
  * of ad, b, x and entries w*row_offsets[s] .. of al and ja.  All systems are
- * solved in one persistent launch, each by its own block group with its own
- * scalars and stopping test; stats receives nsys entries.  x0 is zero unless
+ * solved in one persistent launch (or a few, when they outnumber half the
+ * co-resident blocks), each with its own scalars and stopping test: a
+ * system gets its own group of blocks, or, when it is too small to keep a
+ * block busy for the batch, shares a block with other small systems, which
+ * that block solves one after another.  stats receives nsys entries.  x0 is zero unless
  * opt->x0_rule == FVC_X0_GIVEN.  opt->precision selects FP64 or mixed.  No
  * counterpart in the reference (its solver takes one system per call). */
 int fvc_solve_batch(int32_t code, int32_t nsys, const int32_t *row_offsets, int32_t w,
