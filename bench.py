@@ -116,10 +116,11 @@ def gmres_solve_bytes(neq: int, nnz_off: int, cycles: int, steps: int, k: int, c
 
 def pcg_ref_iter_bytes(neq: int, nnz_off: int) -> int:
     """One iteration of the reference-order PCG phases (batched FP64):
-    the matrix, then 88 bytes of vector traffic per row (A: z, p read, p, q
-    written; B: x, p, r, q read, x, r, z written; the Jacobi diagonal is the
-    matrix's ad)."""
-    return 8 * neq + 12 * nnz_off + 88 * neq
+    phase A's SpMV reads the matrix (ad and the off-diagonals), phase B reads
+    the Jacobi diagonal m = ad again (z = r/m), and 88 bytes of vector
+    traffic per row (A: z, p read, p, q written; B: x, p, r, q read, x, r, z
+    written)."""
+    return 8 * neq + 12 * nnz_off + 8 * neq + 88 * neq
 
 
 def mixed_inner_bytes(neq: int, nnz_off: int, col16: bool) -> int:
