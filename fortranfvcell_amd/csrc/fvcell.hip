@@ -1653,11 +1653,11 @@ int run_batch(DevState &d, int nsys, const std::vector<int> &row_off, const std:
     if ((rc = launch(fn, true, G, bs, B, s))) return rc;
     HIPCHK(hipMemcpyAsync(host_st.data() + s0, st, m * sizeof(KState), hipMemcpyDeviceToHost, s));
     HIPCHK(hipStreamSynchronize(s));   // desc/hplan host buffers are reused next wave
-    unsigned err = 0;
-    for (int i = 0; i < m; ++i) {
-      HIPCHK(hipMemcpy(&err, ctr + (size_t)i * kCtrWords + kCtrErr, sizeof err, hipMemcpyDeviceToHost));
-      if (err) return fail(FVC_DEVICE_ERROR, "batched solve: a grid barrier wait expired");
-    }
+    // every system's barrier words in one copy, then its error word
+    std::vector<unsigned> hctr((size_t)m * kCtrWords);
+    HIPCHK(hipMemcpy(hctr.data(), ctr, hctr.size() * sizeof(unsigned), hipMemcpyDeviceToHost));
+    for (int i = 0; i < m; ++i)
+      if (hctr[(size_t)i * kCtrWords + kCtrErr]) return fail(FVC_DEVICE_ERROR, "batched solve: a grid barrier wait expired");
   }
   HIPCHK(hipEventRecord(d.ev1, s));
   HIPCHK(hipEventSynchronize(d.ev1));
