@@ -47,7 +47,8 @@ class Options(C.Structure):
                 ("grid_blocks", C.c_int32), ("precision", C.c_int32), ("batch_work_exp10", C.c_int32),
                 ("barrier", C.c_int32), ("recurrence", C.c_int32), ("refresh_every", C.c_int32),
                 ("refresh_drop_e3", C.c_int32), ("row_order", C.c_int32), ("lane_exchange", C.c_int32),
-                ("gmres_orth", C.c_int32), ("batch_rows_per_wave", C.c_int32)]
+                ("gmres_orth", C.c_int32), ("batch_rows_per_wave", C.c_int32),
+                ("batch_launch_systems", C.c_int32)]
 
 
 class Stats(C.Structure):

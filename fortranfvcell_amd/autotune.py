@@ -107,6 +107,9 @@ def tune(fmts: dict, method: int, iters: int = 60, use_cache: bool = True, verbo
 
 
 ROWS_PER_WAVE = (16, 64, 256)
+# batches: systems per launch as a fraction of the batch (1/4, 1/8), against
+# the default of half the co-resident blocks
+LAUNCH_DIVISORS = (4, 8)
 
 
 def tune_batch(batch, precision: int = 0, verbose: bool = False) -> dict:
@@ -116,11 +119,14 @@ def tune_batch(batch, precision: int = 0, verbose: bool = False) -> dict:
     batch: one concatenated batch (concat_ell or concat_csrd), or a dict of
     layouts of the same batch, e.g. {"ell": concat_ell(fmts), "csrd":
     concat_csrd(fmts)}.  The search is staged, since each probe is a whole
-    batched solve: threads per block x block-allocation exponent on the first
-    layout; then, at the winner, the rows per wavefront of each system's
-    block group (batch_rows_per_wave: a system gets at most neq / (rows x
-    waves per block) blocks, so small systems keep their wavefronts full);
-    then the other layouts at the winning options.  Returns {"layout": name,
+    batched solve: (1) threads per block x block-allocation exponent on the
+    first layout; (2) at the winning exponent, each block size with a
+    quarter and an eighth of the batch per launch (batch_launch_systems;
+    the launches run one after another, each on all blocks); (3) at the
+    winner, the rows per wavefront of each system's block group
+    (batch_rows_per_wave: a system gets at most neq / (rows x waves per
+    block) blocks, so small systems keep their wavefronts full); (4) the
+    other layouts at the winning options.  Returns {"layout": name,
     "options": {...}, "seconds": t, "trials": [...]}."""
     from . import DeviceBatch
     layouts = batch if "nsys" not in batch else {"ell" if int(batch.get("matriz", 4)) == 4 else "csrd": batch}
@@ -142,6 +148,16 @@ def tune_batch(batch, precision: int = 0, verbose: bool = False) -> dict:
             opts = dict(block_size=bs, batch_work_exp10=exp10)
             t = timed(names[0], db, opts)
             if best is None or t < best["seconds"]:
+                best = {"layout": names[0], "options": opts, "seconds": t}
+    # systems per launch at every block size (how many launches a batch
+    # takes depends on how many blocks of that size a CU holds)
+    nsys = int(layouts[names[0]]["nsys"])
+    exp10 = best["options"]["batch_work_exp10"]
+    for bs in BLOCK_SIZES:
+        for div in LAUNCH_DIVISORS:
+            opts = dict(block_size=bs, batch_work_exp10=exp10, batch_launch_systems=max(1, -(-nsys // div)))
+            t = timed(names[0], db, opts)
+            if t < best["seconds"]:
                 best = {"layout": names[0], "options": opts, "seconds": t}
     for rpw in ROWS_PER_WAVE:
         opts = dict(best["options"], batch_rows_per_wave=rpw)

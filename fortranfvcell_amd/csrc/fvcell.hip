@@ -1555,9 +1555,15 @@ int run_batch(DevState &d, int nsys, const std::vector<int> &row_off, const std:
   const int cap = std::max(1, o.grid_blocks > 0 ? std::min(o.grid_blocks, occ * d.cus)
                                                : d.cus * std::max(1, std::min(occ, 4)));
   const int total = row_off[nsys];
-  // systems go in waves of at most cap/2 systems (bounds the per-system
-  // state; the plan packs small systems onto shared blocks)
-  const int wave = std::max(1, std::min(nsys, cap / 2 > 0 ? cap / 2 : 1));
+  // systems go in launches ("waves") of at most cap/2 systems, or of
+  // batch_launch_systems: each launch runs on all blocks, so fewer systems
+  // per launch give each more blocks (a shorter chain of iterations for the
+  // largest), at the cost of a launch tail each (the plan packs small
+  // systems onto shared blocks)
+  if (o.batch_launch_systems < 0)
+    return fail(FVC_BAD_ARGUMENT, "batch_launch_systems %d", o.batch_launch_systems);
+  const int wave = std::max(1, std::min(nsys, o.batch_launch_systems > 0 ? std::min(o.batch_launch_systems, 4096)
+                                                                          : (cap / 2 > 0 ? cap / 2 : 1)));
   const double expo = o.batch_work_exp10 > 0 ? o.batch_work_exp10 / 10.0 : 1.5;
   if (o.batch_rows_per_wave < 0) return fail(FVC_BAD_ARGUMENT, "batch_rows_per_wave %d", o.batch_rows_per_wave);
   const int min_rows = o.batch_rows_per_wave > 0 ? o.batch_rows_per_wave * (bs / 64) : 64;

@@ -126,6 +126,9 @@ typedef struct fvc_options {
                              system's block group (a system gets at most
                              ceil(neq / (rows x waves per block)) blocks); 0 = the
                              default of 64 rows per block                            */
+  int32_t batch_launch_systems; /* batches: at most this many systems per launch (the
+                             launches run one after another, each on all blocks);
+                             0 = half the co-resident blocks                         */
 } fvc_options;
 
 /* fvc_options.gmres_orth */

@@ -45,6 +45,8 @@ def test_tune_batch(fvc):
     tried = {(q["layout"], q["options"].get("batch_rows_per_wave", 0)) for q in best["trials"]}
     assert ("csrd", best["options"].get("batch_rows_per_wave", 0)) in tried
     assert {r for l, r in tried if l == "ell"} >= set(autotune.ROWS_PER_WAVE)
+    launches = {q["options"].get("batch_launch_systems", 0) for q in best["trials"] if q["layout"] == "ell"}
+    assert launches >= {-(-24 // d) for d in autotune.LAUNCH_DIVISORS}
     assert best["seconds"] == min(q["seconds"] for q in best["trials"])
     x, sts = fvc.solve_batch(layouts[best["layout"]], tol=1e-10, stop_rule=fvc.STOP_RELRES, **best["options"])
     assert all(st["status"] == 0 for st in sts)

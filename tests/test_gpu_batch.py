@@ -127,6 +127,32 @@ def test_batch_shared_blocks_match_solo(fvc, precision, barrier):
         assert np.array_equal(xs, x[ro[i]:ro[i + 1]]), i
 
 
+@pytest.mark.parametrize("precision", [0, 1], ids=["fp64", "mixed"])
+def test_batch_launch_systems(fvc, precision):
+    """batch_launch_systems caps the systems of one launch: the batch then
+    runs in several launches one after another, each on all blocks.  Every
+    system still converges; a system of at most 64 rows runs as a group of
+    one block in any launch, so its solution is the same bits; a negative
+    cap is refused."""
+    systems = _systems(40, 6, 40, 17)
+    fmts = [P.to_ell(s) for s in systems]
+    batch = fvc.concat_ell(fmts)
+    ro = batch["row_offsets"]
+    kw = dict(tol=1e-10, stop_rule=fvc.STOP_RELRES, precision=precision)
+    x0, s0 = fvc.solve_batch(batch, **kw)
+    for per in (1, 7, 40):
+        print(f"batch_launch_systems={per} precision={precision}", flush=True)
+        x, sts = fvc.solve_batch(batch, batch_launch_systems=per, **kw)
+        for i, s in enumerate(systems):
+            assert sts[i]["status"] == 0, (per, i)
+            xi = x[ro[i]:ro[i + 1]]
+            assert np.linalg.norm(s.b - P.scipy_csr(s) @ xi) <= 1.01e-10 * np.linalg.norm(s.b), (per, i)
+            if s.neq <= 64:
+                assert np.array_equal(xi, x0[ro[i]:ro[i + 1]]), (per, i)
+    with pytest.raises(fvc.FvcError):
+        fvc.solve_batch(batch, batch_launch_systems=-1, **kw)
+
+
 def test_batch_device_resident(fvc):
     systems = _systems(16, 16, 64, 5)
     batch = fvc.concat_ell([P.to_ell(s) for s in systems])
