@@ -107,9 +107,9 @@ def tune(fmts: dict, method: int, iters: int = 60, use_cache: bool = True, verbo
 
 
 ROWS_PER_WAVE = (16, 64, 256)
-# batches: systems per launch as a fraction of the batch (1/4, 1/8), against
+# batches: systems per launch as a fraction of the batch (1/3 .. 1/8), against
 # the default of half the co-resident blocks
-LAUNCH_DIVISORS = (4, 8)
+LAUNCH_DIVISORS = (3, 4, 6, 8)
 
 
 def tune_batch(batch, precision: int = 0, verbose: bool = False) -> dict:
@@ -121,8 +121,9 @@ def tune_batch(batch, precision: int = 0, verbose: bool = False) -> dict:
     concat_csrd(fmts)}.  The search is staged, since each probe is a whole
     batched solve: (1) threads per block x block-allocation exponent on the
     first layout; (2) at the winning exponent, each block size with a
-    quarter and an eighth of the batch per launch (batch_launch_systems;
-    the launches run one after another, each on all blocks); (3) at the
+    third, a quarter, a sixth and an eighth of the batch per launch
+    (batch_launch_systems; the launches run one after another, each on all
+    blocks); (3) at the
     winner, the rows per wavefront of each system's block group
     (batch_rows_per_wave: a system gets at most neq / (rows x waves per
     block) blocks, so small systems keep their wavefronts full); (4) the
