@@ -381,8 +381,8 @@ def batch_only(steps: int) -> None:
     """Only the 256-system batch (pressure systems, sides log-uniform in
     [32, 512], each to true relative residual 1e-10), FP64 and mixed, at the
     options the autotuner picks on MI355X (1,024 threads, work exponent 1.5;
-    FP64 64 systems per launch, mixed the default 128), `steps` batched
-    solves each: the batch kernels, for a profile."""
+    FP64 64 systems per launch, mixed 86), `steps` batched solves each: the
+    batch kernels, for a profile."""
     import fortranfvcell_amd as F
     from fortranfvcell_amd import problems as P
     F.init(0)
@@ -390,7 +390,7 @@ def batch_only(steps: int) -> None:
     sizes = [(int(f["neq"]), int(f["nad"])) for f in fmts]
     db = F.DeviceBatch(F.concat_ell(fmts))
     out = {}
-    for name, prec, per in (("fp64", F.PREC_FP64, 64), ("mixed", F.PREC_MIXED, 0)):
+    for name, prec, per in (("fp64", F.PREC_FP64, 64), ("mixed", F.PREC_MIXED, 86)):
         kw = dict(tol=1e-10, precision=prec, stop_rule=F.STOP_RELRES, block_size=1024, batch_work_exp10=15,
                   batch_launch_systems=per)
         ts = []
